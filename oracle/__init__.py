@@ -1,0 +1,268 @@
+"""CPU oracle for the live-dic IC-GN hot path.  TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker.  The product
+(``live_dic`` and ``liblivedic.so``) never imports, links or runs it.
+
+The C part (``oracle.c``) restates the per-frame WGSL kernels; the NumPy part
+(``reference_prep``) restates the FP64 host reference preparation.  ``Pipeline``
+reproduces the reference's coarse-to-fine per-frame command sequence
+(``live_dic/full_pyramid_backend.py:1436-1569``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from . import reference_prep as prep
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    """Compile the C oracle with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(_LIB_PATH))
+        _lib.ora_solve.restype = ctypes.c_int
+    return _lib
+
+
+class SolveCfg(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+        ("grid_width", ctypes.c_int32), ("grid_height", ctypes.c_int32),
+        ("subset_size", ctypes.c_int32), ("subset_stride", ctypes.c_int32),
+        ("motion_model", ctypes.c_int32), ("interpolation", ctypes.c_int32),
+        ("spatial_weighting", ctypes.c_int32), ("roi_enabled", ctypes.c_int32),
+        ("gaussian_sigma", ctypes.c_float), ("iterations", ctypes.c_int32),
+        ("tolerance", ctypes.c_float), ("zncc_min", ctypes.c_float),
+        ("lanes", ctypes.c_int32), ("preserve_disabled", ctypes.c_int32),
+    ]
+
+
+class SeedCfg(ctypes.Structure):
+    _fields_ = [
+        ("source_grid_width", ctypes.c_int32), ("source_grid_height", ctypes.c_int32),
+        ("destination_grid_width", ctypes.c_int32), ("destination_grid_count", ctypes.c_int32),
+        ("source_scale", ctypes.c_float), ("destination_scale", ctypes.c_float),
+        ("subset_size", ctypes.c_float), ("subset_stride", ctypes.c_float),
+        ("motion_model", ctypes.c_int32), ("track_seed_availability", ctypes.c_int32),
+    ]
+
+
+class MedianCfg(ctypes.Structure):
+    _fields_ = [
+        ("grid_width", ctypes.c_int32), ("grid_count", ctypes.c_int32),
+        ("subset_stride", ctypes.c_int32), ("motion_model", ctypes.c_int32),
+        ("outlier_policy", ctypes.c_int32), ("median_filter_size", ctypes.c_int32),
+        ("median_filter_scope", ctypes.c_int32), ("roi_enabled", ctypes.c_int32),
+        ("median_seed_available", ctypes.c_int32),
+    ]
+
+
+MODELS = {"affine": 0, "translation": 1, "quadratic": 2}
+INTERP = {"bspline": 0, "linear": 1, "cubic": 2}
+POLICIES = {"flag": 0, "replace": 1, "disabled": 2, "median": 3}
+DEFAULT_ITERATIONS = {1: (30,), 2: (8, 16), 3: (6, 8, 16), 4: (12, 8, 8, 16)}
+DEFAULT_TOLERANCES = (1e-2, 5e-3, 5e-3)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def reduce(image: np.ndarray) -> np.ndarray:
+    src = np.ascontiguousarray(image, dtype=np.uint8)
+    h, w = src.shape
+    out = np.empty((h // 2, w // 2), dtype=np.uint8)
+    lib().ora_reduce(_p(src), ctypes.c_int(w), ctypes.c_int(h), _p(out))
+    return out
+
+
+def prefilter(image: np.ndarray, fir: np.ndarray | None = None) -> np.ndarray:
+    src = np.ascontiguousarray(image, dtype=np.uint8)
+    h, w = src.shape
+    fir = prep.fir_weights() if fir is None else np.ascontiguousarray(fir, dtype=np.float32)
+    out = np.empty((h, w), dtype=np.float32)
+    scratch = np.empty((h, w), dtype=np.float32)
+    lib().ora_prefilter(_p(src), ctypes.c_int(w), ctypes.c_int(h), _p(fir), _p(out), _p(scratch))
+    return out
+
+
+def solve(cfg: SolveCfg, reference, metadata, inverse, coeff, target, seed=None, active=None) -> np.ndarray:
+    stride = 16 if cfg.motion_model == 2 else 8
+    n = cfg.grid_width * cfg.grid_height
+    state = np.zeros((n, stride), dtype=np.float32)
+    ref = np.ascontiguousarray(reference, dtype=np.float32)
+    meta = np.ascontiguousarray(metadata, dtype=np.float32)
+    inv = np.ascontiguousarray(inverse, dtype=np.float32)
+    co = None if coeff is None else np.ascontiguousarray(coeff, dtype=np.float32)
+    tg = None if target is None else np.ascontiguousarray(target, dtype=np.uint8)
+    sd = None if seed is None else np.ascontiguousarray(seed, dtype=np.float32)
+    ac = None if active is None else np.ascontiguousarray(active, dtype=np.uint32)
+    rc = lib().ora_solve(ctypes.byref(cfg), _p(ref), _p(meta), _p(inv), _p(co), _p(tg), _p(sd), _p(ac), _p(state))
+    if rc != 0:
+        raise ValueError("oracle solve rejected its configuration")
+    return state
+
+
+def transfer_seed(cfg: SeedCfg, source_state, permissions=None):
+    stride = 16 if cfg.motion_model == 2 else 8
+    seed = np.zeros((cfg.destination_grid_count, stride), dtype=np.float32)
+    active = np.zeros(cfg.destination_grid_count, dtype=np.uint32)
+    src = np.ascontiguousarray(source_state, dtype=np.float32)
+    perm = None if permissions is None else np.ascontiguousarray(permissions, dtype=np.uint32)
+    lib().ora_transfer_seed(ctypes.byref(cfg), _p(src), _p(perm), _p(seed), _p(active))
+    return seed, active
+
+
+def median(cfg: MedianCfg, state, metadata, roi_words=None):
+    st = np.ascontiguousarray(state, dtype=np.float32)
+    out = np.empty_like(st)
+    meta = np.ascontiguousarray(metadata, dtype=np.float32)
+    words = None if roi_words is None else np.ascontiguousarray(roi_words, dtype=np.float32)
+    lib().ora_median(ctypes.byref(cfg), _p(st), _p(meta), _p(words), _p(out))
+    return out
+
+
+def affine_zncc(width, height, grid_w, grid_h, size, stride, interpolation, reference, state, coeff, target):
+    out = np.empty(grid_w * grid_h, dtype=np.float32)
+    ref = np.ascontiguousarray(reference, dtype=np.float32)
+    st = np.ascontiguousarray(state, dtype=np.float32)
+    co = None if coeff is None else np.ascontiguousarray(coeff, dtype=np.float32)
+    tg = np.ascontiguousarray(target, dtype=np.uint8)
+    lib().ora_affine_zncc(
+        ctypes.c_int(width), ctypes.c_int(height), ctypes.c_int(grid_w), ctypes.c_int(grid_h),
+        ctypes.c_int(size), ctypes.c_int(stride), ctypes.c_int(interpolation),
+        _p(ref), _p(st), _p(co), _p(tg), _p(out))
+    out[out < -1.0] = np.nan
+    return out
+
+
+class Level:
+    """Prepared reference of one pyramid level."""
+
+    def __init__(self, image, opts, roi=None):
+        self.image = image
+        self.height, self.width = image.shape
+        self.grid_width, self.grid_height = prep.grid_shape(
+            self.width, self.height, opts["subset_size"], opts["subset_stride"])
+        self.reference, self.inverse, self.metadata = prep.reference_arrays(
+            image, interpolation=opts["interpolation"], motion_model=opts["motion_model"],
+            subset_size=opts["subset_size"], subset_stride=opts["subset_stride"],
+            spatial_weighting=opts["spatial_weighting"], gaussian_sigma=opts["gaussian_sigma"])
+        self.roi = roi
+        self.roi_words = None
+        self.seed_links = None
+        if roi is not None and roi.get("effective", False):
+            self.metadata[~roi["selected"].ravel(), 1] = -3.0
+            self.roi_words = np.zeros((self.grid_width * self.grid_height, 4), dtype=np.float32)
+            if roi.get("median_words") is not None:
+                self.roi_words[:] = roi["median_words"]
+            else:
+                self.roi_words[:, 0] = roi["neighbour_bits"].ravel()
+            self.seed_links = roi.get("seed_links")
+
+
+class Pipeline:
+    """Coarse-to-fine oracle registration with the reference's defaults.
+
+    Orders levels fine (index 0) to coarse; schedules are given coarse to fine
+    as in ``ICGNWGPUFullPyramid`` (full_pyramid_backend.py:51-62, 1135-1368).
+    ``rois`` optionally holds one dict per level with ``selected``,
+    ``neighbour_bits``, ``effective`` and, for finer levels, ``seed_links``.
+    """
+
+    def __init__(self, *, pyramid_levels=2, iterations_per_level=None, tolerance_per_level=None,
+                 zncc_min=0.8, motion_model="affine", interpolation="bspline", subset_size=16,
+                 subset_stride=None, spatial_weighting="uniform", gaussian_sigma=None,
+                 outlier_policy="replace", median_filter_size=3, median_filter_scope="rejected",
+                 lanes=16):
+        L = pyramid_levels
+        self.levels_count = L
+        self.iterations = tuple(iterations_per_level or DEFAULT_ITERATIONS[L])
+        default_tol = (1e-2, *DEFAULT_TOLERANCES) if L == 4 else DEFAULT_TOLERANCES[-L:]
+        self.tolerances = tuple(tolerance_per_level or default_tol)
+        self.zncc_min = float(zncc_min)
+        stride = subset_size if subset_stride is None else subset_stride
+        self.opts = dict(motion_model=motion_model, interpolation=interpolation, subset_size=subset_size,
+                         subset_stride=stride, spatial_weighting=spatial_weighting,
+                         gaussian_sigma=0.25 * subset_size if gaussian_sigma is None else gaussian_sigma)
+        self.outlier_policy = outlier_policy
+        self.median_filter_size = median_filter_size
+        self.median_filter_scope = median_filter_scope
+        self.lanes = lanes
+        self.fir = prep.fir_weights()
+        self.levels: list[Level] = []
+
+    def set_reference(self, image, rois=None):
+        images = [np.ascontiguousarray(image, dtype=np.uint8)]
+        for _ in range(1, self.levels_count):
+            images.append(reduce(images[-1]))
+        self.levels = [Level(img, self.opts, None if rois is None else rois[k]) for k, img in enumerate(images)]
+        return self
+
+    def _solve_cfg(self, k):
+        lv = self.levels[k]
+        o = self.opts
+        sched = self.levels_count - 1 - k
+        return SolveCfg(
+            lv.width, lv.height, lv.grid_width, lv.grid_height, o["subset_size"], o["subset_stride"],
+            MODELS[o["motion_model"]], INTERP[o["interpolation"]], 1 if o["spatial_weighting"] == "gaussian" else 0,
+            1 if lv.roi_words is not None else 0, o["gaussian_sigma"], self.iterations[sched],
+            self.tolerances[sched], self.zncc_min, self.lanes, 0)
+
+    def _median(self, k, state):
+        if self.outlier_policy == "disabled":
+            return state
+        lv = self.levels[k]
+        o = self.opts
+        cfg = MedianCfg(
+            lv.grid_width, lv.grid_width * lv.grid_height, o["subset_stride"], MODELS[o["motion_model"]],
+            POLICIES[self.outlier_policy], self.median_filter_size,
+            1 if self.median_filter_scope == "all" else 0, 1 if lv.roi_words is not None else 0,
+            1 if (self.outlier_policy == "median" and k > 0) else 0)
+        return median(cfg, state, lv.metadata, lv.roi_words)
+
+    def register(self, target, return_levels=False):
+        targets = [np.ascontiguousarray(target, dtype=np.uint8)]
+        for _ in range(1, self.levels_count):
+            targets.append(reduce(targets[-1]))
+        outputs = [None] * self.levels_count
+        o = self.opts
+        for k in range(self.levels_count - 1, -1, -1):
+            lv = self.levels[k]
+            coeff = prefilter(targets[k], self.fir) if o["interpolation"] == "bspline" else None
+            cfg = self._solve_cfg(k)
+            if k == self.levels_count - 1:
+                state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k])
+            else:
+                src = self.levels[k + 1]
+                scfg = SeedCfg(src.grid_width, src.grid_height, lv.grid_width, lv.grid_width * lv.grid_height,
+                               float(1 << (k + 1)), float(1 << k), float(o["subset_size"]),
+                               float(o["subset_stride"]), MODELS[o["motion_model"]], 0)
+                seed, active = transfer_seed(scfg, outputs[k + 1], lv.seed_links)
+                state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
+            outputs[k] = self._median(k, state)
+        stride = 16 if o["motion_model"] == "quadratic" else 8
+        fine = outputs[0].reshape(self.levels[0].grid_height, self.levels[0].grid_width, stride)
+        return (fine, outputs) if return_levels else fine
+
+
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
