@@ -1,0 +1,185 @@
+/*
+ * live-dic for MI355X: C ABI of the gfx950 IC-GN engine (liblivedic.so).
+ *
+ * The reference has no native library: its Python supervisor drives WGSL
+ * compute pipelines through wgpu-py.  Each entry point below replaces the part
+ * of that supervisor-to-device boundary named in its comment (paths relative to
+ * DIC/LucasKanadeICGN/python/src/live_dic/).  Plain pointers and sizes only;
+ * host arrays are borrowed for the duration of the call.  Every function
+ * returns 0 on success or a negative LDIC_ERR_* code; ldic_last_error() then
+ * holds a message for the calling thread.  An engine is not thread-safe.
+ */
+#ifndef LIVEDIC_H
+#define LIVEDIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDIC_ABI_VERSION 1
+#define LDIC_MAX_LEVELS 4
+
+enum {
+    LDIC_OK = 0,
+    LDIC_ERR_ARGUMENT = -1,  /* invalid argument; no device work was done */
+    LDIC_ERR_STATE = -2,     /* call out of order (no reference, closed slot) */
+    LDIC_ERR_DEVICE = -3,    /* HIP runtime or kernel launch failure */
+    LDIC_ERR_MEMORY = -4     /* device or pinned allocation failed */
+};
+
+/* Motion model, interpolation and outlier codes follow the WGSL override
+ * constants (motion.py:9-12, interpolation.py:12-16, outlier.py:9-14). */
+typedef struct ldic_options {
+    int32_t motion_model;       /* 0 affine, 1 translation, 2 quadratic */
+    int32_t interpolation;      /* 0 B-spline, 1 bilinear, 2 Keys cubic */
+    int32_t subset_size;        /* S */
+    int32_t subset_stride;      /* D, 1..S */
+    int32_t spatial_weighting;  /* 0 uniform, 1 Gaussian */
+    float gaussian_sigma;       /* pixels, used when spatial_weighting == 1 */
+    int32_t outlier_policy;     /* 0 flag, 1 replace, 2 disabled, 3 median */
+    int32_t median_filter_size; /* 3, 5, 7 or 9 */
+    int32_t median_filter_scope;/* 0 rejected, 1 all */
+    int32_t pyramid_levels;     /* 1..4 */
+    int32_t device_ordinal;     /* HIP device index */
+} ldic_options;
+
+typedef struct ldic_level_config {
+    int32_t iterations; /* 1..30 */
+    float tolerance;
+    float zncc_min;
+} ldic_level_config;
+
+/* Static ROI tables of one level, built by the host planner (roi.py:178-263,
+ * 358-437).  All pointers may be NULL: no ROI at that level. */
+typedef struct ldic_level_roi {
+    const uint8_t* selected;        /* grid_h*grid_w flags: full window inside the mask */
+    const float* neighbour_words;   /* grid*4: neighbour permission words (metadata tail) */
+    const uint32_t* seed_links;     /* grid: transfer permissions into this level */
+} ldic_level_roi;
+
+typedef struct ldic_timing {
+    double upload_ms;    /* host copy into pinned staging and the device copy issue */
+    double gpu_ms;       /* device time from the first upload to the last kernel (HIP events) */
+    double solve_ms;     /* summed device time of the finest-level solver kernel over `frames` */
+    double readback_ms;  /* wait and copy of the packed state into the caller's buffer */
+    double wall_ms;      /* whole call */
+    int32_t frames;      /* frames completed by this call (resident slots may queue several) */
+} ldic_timing;
+
+/* Device selection and kernel set-up: ICGNWGPU.__init__ device/pipeline
+ * creation, wgpu_backend.py:889-1152, and ICGNWGPUFullPyramid.__init__,
+ * full_pyramid_backend.py:202-431. */
+int ldic_create(const ldic_options* options, struct ldic_engine** out_engine);
+/* ICGNWGPU.close / ICGNWGPUFullPyramid.close (wgpu_backend.py:3078-3093,
+ * full_pyramid_backend.py:1904-1919).  NULL is accepted. */
+void ldic_destroy(struct ldic_engine* engine);
+const char* ldic_last_error(void);
+int ldic_abi_version(void);
+/* adapter_info / device_name attributes (wgpu_backend.py:113-144). */
+int ldic_device_name(struct ldic_engine* engine, char* buffer, int32_t capacity);
+
+/* set_reference: the reference pyramid, then per level the Gaussian, the
+ * recursive B-spline prefilter, gradients, per-subset statistics and inverse
+ * Hessians, all on the device (replaces _reduce_half and _reference_arrays on
+ * the host, wgpu_backend.py:318-334, 589-823, and set_reference
+ * full_pyramid_backend.py:1135-1368).  configs and rois hold one entry per
+ * level, finest first.  out_reference_valid (fine grid, nullable) receives the
+ * reference validity before ROI exclusion. */
+int ldic_set_reference(struct ldic_engine* engine, const uint8_t* image, int32_t width, int32_t height,
+                       int64_t row_stride, const ldic_level_config* configs, const ldic_level_roi* rois,
+                       uint8_t* out_reference_valid);
+int ldic_level_shape(struct ldic_engine* engine, int32_t level, int32_t* width, int32_t* height,
+                     int32_t* grid_width, int32_t* grid_height);
+int ldic_set_level_config(struct ldic_engine* engine, int32_t level, const ldic_level_config* config);
+
+/* register: upload, target pyramid, per-level prefilter, solve, outlier pass
+ * and seed transfer, readback of the finest packed state (gh*gw*8 or *16
+ * floats) -- full_pyramid_backend.py:1436-1569, 1693-1832. */
+int ldic_register(struct ldic_engine* engine, const uint8_t* target, int64_t row_stride, float* out_state,
+                  ldic_timing* timing);
+
+/* register_stream slots (full_pyramid_backend.py:1834-1902): submit copies the
+ * target into the slot's pinned staging and queues its work; retire waits and
+ * copies the result out, in submission order chosen by the caller. */
+int ldic_set_slots(struct ldic_engine* engine, int32_t count);
+int ldic_submit(struct ldic_engine* engine, int32_t slot, const uint8_t* target, int64_t row_stride);
+int ldic_retire(struct ldic_engine* engine, int32_t slot, float* out_state, ldic_timing* timing);
+
+/* register_seeded (wgpu_backend.py:2773-2819, solve_seeded icgn.wgsl:1001-1008)
+ * on the finest level, with optional final affine ZNCC (affine_zncc.wgsl).
+ * out_zncc receives NaN where no correlation is defined. */
+int ldic_register_seeded(struct ldic_engine* engine, const uint8_t* target, int64_t row_stride,
+                         const float* seed_state, const uint8_t* active, int32_t include_zncc, float* out_state,
+                         float* out_zncc, ldic_timing* timing);
+
+/* Resident frames for throughput measurement: the target is already in device
+ * memory (row pitch row_stride bytes).  Queues the same per-frame work on the
+ * slot's stream without host copies.  Further resident frames may be queued on
+ * the same slot before waiting (stream order keeps them sequential);
+ * ldic_slot_wait blocks until all are done and reports their summed solver time. */
+int ldic_register_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_target,
+                           int64_t row_stride);
+int ldic_slot_wait(struct ldic_engine* engine, int32_t slot, ldic_timing* timing);
+int ldic_slot_output(struct ldic_engine* engine, int32_t slot, float* out_state);
+
+/* Parity hooks: the prepared reference of one level (the arrays returned by
+ * _reference_arrays, wgpu_backend.py:589-823) and the target coefficient
+ * plane of a slot's last frame (prefilter_y output, icgn.wgsl:181-214). */
+int ldic_export_reference(struct ldic_engine* engine, int32_t level, float* reference4, float* metadata,
+                          float* inverse);
+int ldic_export_coefficients(struct ldic_engine* engine, int32_t slot, int32_t level, float* coefficients);
+int ldic_export_level_state(struct ldic_engine* engine, int32_t slot, int32_t level, float* out_state);
+
+/* Stand-alone operators on host buffers (parity tests): reduce_fused
+ * (full_pyramid.wgsl:215-249) and prefilter_x/prefilter_y (icgn.wgsl:145-214). */
+int ldic_op_reduce(const uint8_t* source, int32_t width, int32_t height, uint8_t* destination);
+int ldic_op_prefilter(const uint8_t* source, int32_t width, int32_t height, float* coefficients);
+
+/* ---- Workload generation and device memory (no counterpart in the reference,
+ * whose benchmarks read PNG fixtures; used for the synthetic speckle pairs of
+ * the measurements and for training data of the learned seed). ---- */
+
+/* Displacement u(X) in reference coordinates: a quadratic polynomial about
+ * (origin_x, origin_y) in the quadratic state order plus one sinusoid per
+ * component, amplitude * sin(kx X + ky Y + phase). */
+typedef struct ldic_field {
+    int32_t enabled;
+    double origin_x, origin_y;
+    double poly[12];
+    double wave[2][4];
+} ldic_field;
+
+/* Gaussian-blob speckle texture: background + sum amplitude exp(-r^2/2 sigma^2).
+ * Blobs (x, y, amplitude, 0 as float32, image coordinates) are sorted into
+ * square cells of `cell` pixels starting at cell_origin; cell_start holds
+ * cells_x*cells_y+1 offsets.  Both live on the device.  Blobs further than
+ * `cell` pixels from a point are ignored. */
+typedef struct ldic_speckle {
+    const float* device_blobs;
+    const int32_t* device_cell_start;
+    int32_t cells_x, cells_y, cell;
+    float cell_origin;  /* image coordinate of the first cell's corner (both axes) */
+    float sigma, background;
+} ldic_speckle;
+
+/* Renders the deformed texture (field NULL: the reference) with Gaussian noise
+ * of noise_sigma grey levels, rounded and clipped to uint8, into device memory.
+ * device_float_out (nullable) receives the noiseless float image. */
+int ldic_render_speckle(const ldic_speckle* speckle, const ldic_field* field, int32_t width, int32_t height,
+                        float noise_sigma, uint64_t noise_seed, uint8_t* device_out, float* device_float_out,
+                        void* stream);
+
+int ldic_device_alloc(size_t bytes, void** out);
+int ldic_device_free(void* pointer);
+int ldic_copy_to_device(void* device_dst, const void* host_src, size_t bytes);
+int ldic_copy_to_host(void* host_dst, const void* device_src, size_t bytes);
+int ldic_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIVEDIC_H */

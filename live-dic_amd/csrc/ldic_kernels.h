@@ -1,0 +1,98 @@
+// Launch interface of the live-dic gfx950 kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ldic {
+
+enum Model : int { kAffine = 0, kTranslation = 1, kQuadratic = 2 };
+enum Interp : int { kBspline = 0, kLinear = 1, kKeys = 2 };
+enum Policy : int { kFlag = 0, kReplace = 1, kDisabled = 2, kMedian = 3 };
+
+inline int param_count(int model) { return model == kQuadratic ? 12 : 6; }
+inline int state_stride(int model) { return model == kQuadratic ? 16 : 8; }
+inline int metadata_stride(int model) { return model == kQuadratic ? 28 : 16; }
+
+struct SolveArgs {
+    int width, height, grid_w, grid_h;
+    int S, D;
+    int iterations;
+    float tolerance, zncc_min;
+    float inv_var;          // 1 / sigma^2 of the Gaussian integration weight
+    int roi_enabled;
+    int preserve_disabled;
+    int tile_cap;           // LDS floats available to each wave
+    const float4* reference;
+    const float* metadata;
+    const float* inverse;
+    const float* coeff;     // B-spline coefficient plane (B-spline mode)
+    const uint8_t* target;  // raw target (direct interpolants)
+    const float* seed;      // optional initial state, same layout as the output
+    const uint32_t* active; // optional per-subset enable flags
+    float* state;
+};
+
+struct SeedArgs {
+    int src_grid_w, src_grid_h, dst_grid_w, dst_count;
+    float src_scale, dst_scale, S, D;
+    int model;
+    int track_availability;
+    const float* src_state;
+    const uint32_t* permissions;  // optional; 31 when absent
+    float* dst_seed;
+    uint32_t* dst_active;
+};
+
+struct MedianArgs {
+    int grid_w, grid_count, D, model, policy, size, scope, roi_enabled, seed_available;
+    const float* state;
+    const float* metadata;
+    const float* roi_words;
+    float* filtered;
+};
+
+struct ZnccArgs {
+    int width, height, grid_w, grid_h, S, D, interpolation;
+    const float4* reference;
+    const float* state;
+    const float* coeff;
+    const uint8_t* target;
+    float* out;
+};
+
+struct PrepArgs {
+    int width, height, grid_w, grid_h, S, D, model, weighted;
+    int direct;                 // 1 for the linear and Keys interpolants
+    const double* intensity;    // Gaussian-filtered (B-spline) or raw intensity, float64
+    const double* grad_x;
+    const double* grad_y;
+    const double* weights;      // S*S integration weights (Gaussian mode) or null
+    double weight_sum;
+    const uint8_t* selected;    // optional ROI selection
+    float4* reference;          // receives the centred intensity when D == S
+    float* metadata;
+    float* inverse;
+    uint8_t* valid;             // reference validity before the ROI override
+};
+
+// Pick the register-cached solver variant for a subset width.
+int solver_pixels_per_lane(int S);
+size_t solver_lds_bytes(int S, int* tile_cap);
+
+hipError_t launch_reduce(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
+hipError_t launch_prefilter(const uint8_t* src, int w, int h, const float* fir31, float* coeff, hipStream_t s);
+hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a, hipStream_t s);
+hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s);
+hipError_t launch_median(const MedianArgs& a, hipStream_t s);
+hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);
+
+// Reference preparation (float64).
+hipError_t launch_gaussian(const uint8_t* src, int w, int h, const double* w7, double* tmp, double* out,
+                           hipStream_t s);
+hipError_t launch_to_double(const uint8_t* src, int w, int h, double* out, hipStream_t s);
+hipError_t launch_bspline(double* plane, double* tmp, int w, int h, hipStream_t s);
+hipError_t launch_gradients(const double* coeff, const double* intensity, int w, int h, int bspline,
+                            double* gx, double* gy, float4* reference, hipStream_t s);
+hipError_t launch_subset_prep(const PrepArgs& a, hipStream_t s);
+
+}  // namespace ldic

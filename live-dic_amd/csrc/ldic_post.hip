@@ -1,0 +1,363 @@
+// Per-subset grid kernels: coarse-to-fine seed transfer, UOD / median outlier
+// pass and the final affine ZNCC.  One thread (or one wave for ZNCC) per
+// subset; the state grid is a few MB, so these are latency-bound side passes.
+#include "ldic_device.h"
+#include "ldic_kernels.h"
+
+namespace ldic {
+
+// ---------------------------------------------------------------------------
+// Seed transfer (shaders/full_pyramid.wgsl:163-433): bilinear blend of the
+// four coarse warps re-centred on the fine subset centre, translation scaled
+// by source/destination scale and curvature by its inverse.
+// ---------------------------------------------------------------------------
+__device__ void accumulate_seed(const SeedArgs& a, int si, float weight, float spx, float spy, float dpx, float dpy,
+                                float ts, float acc[12], float& wsum) {
+    const bool q = a.model == kQuadratic;
+    const float* s = a.src_state + (size_t)si * (q ? 16 : 8);
+    float status = q ? s[12] : s[6];
+    if (status < -0.5f) return;
+    float dx = dpx - spx, dy = dpy - spy;
+    float tu = s[0] + s[1] * dx + s[2] * dy;
+    float tv = s[3] + s[4] * dx + s[5] * dy;
+    float ux = s[1], uy = s[2], vx = s[4], vy = s[5];
+    if (q) {
+        tu += 0.5f * s[6] * dx * dx + s[7] * dx * dy + 0.5f * s[8] * dy * dy;
+        tv += 0.5f * s[9] * dx * dx + s[10] * dx * dy + 0.5f * s[11] * dy * dy;
+        ux += s[6] * dx + s[7] * dy;
+        uy += s[7] * dx + s[8] * dy;
+        vx += s[9] * dx + s[10] * dy;
+        vy += s[10] * dx + s[11] * dy;
+    }
+    acc[0] += weight * tu * ts;
+    acc[1] += weight * ux;
+    acc[2] += weight * uy;
+    acc[3] += weight * tv * ts;
+    acc[4] += weight * vx;
+    acc[5] += weight * vy;
+    if (q) {
+        float cs = 1.0f / ts;
+        for (int k = 6; k < 12; ++k) acc[k] += weight * s[k] * cs;
+    }
+    wsum += weight;
+}
+
+__global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
+    int di = blockIdx.x * 256 + threadIdx.x;
+    if (di >= a.dst_count) return;
+    const bool q = a.model == kQuadratic;
+    const int stride = q ? 16 : 8;
+    uint32_t perm = a.permissions ? a.permissions[di] : 31u;
+    int dxg = di % a.dst_grid_w, dyg = di / a.dst_grid_w;
+    float so = (1.0f - a.src_scale) * 0.5f, dof = (1.0f - a.dst_scale) * 0.5f;
+    float dcx = 0.5f * a.S + a.D * (float)dxg, dcy = 0.5f * a.S + a.D * (float)dyg;
+    float scx = (a.dst_scale * dcx + (dof - so)) / a.src_scale;
+    float scy = (a.dst_scale * dcy + (dof - so)) / a.src_scale;
+    float sgx = (scx - 0.5f * a.S) / a.D, sgy = (scy - 0.5f * a.S) / a.D;
+    int lx = clampi((int)floorf(sgx), 0, max(0, a.src_grid_w - 2));
+    int ly = clampi((int)floorf(sgy), 0, max(0, a.src_grid_h - 2));
+    float fx = fminf(fmaxf(sgx - (float)lx, 0.0f), 1.0f);
+    float fy = fminf(fmaxf(sgy - (float)ly, 0.0f), 1.0f);
+    float ts = a.src_scale / a.dst_scale;
+    float acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float wsum = 0.0f;
+    for (int corner = 0; corner < 4; ++corner) {
+        if ((perm & (1u << corner)) == 0u) continue;
+        int ox = corner & 1, oy = corner >> 1;
+        int sx = lx + ox, sy = ly + oy;
+        if (sx >= a.src_grid_w || sy >= a.src_grid_h) continue;
+        float wx = ox ? fx : 1.0f - fx, wy = oy ? fy : 1.0f - fy;
+        accumulate_seed(a, sy * a.src_grid_w + sx, wx * wy, 0.5f * a.S + a.D * (float)sx,
+                        0.5f * a.S + a.D * (float)sy, scx, scy, ts, acc, wsum);
+    }
+    uint32_t fallback = perm >> 5u;
+    if (wsum == 0.0f && fallback != 0u) {
+        int si = (int)fallback - 1;
+        float st = a.src_state[(size_t)si * stride + (q ? 12 : 6)];
+        if (st >= 0.5f) {
+            int sx = si % a.src_grid_w, sy = si / a.src_grid_w;
+            accumulate_seed(a, si, 1.0f, 0.5f * a.S + a.D * (float)sx, 0.5f * a.S + a.D * (float)sy, scx, scy, ts,
+                            acc, wsum);
+        }
+    }
+    if (wsum > 0.0f)
+        for (int k = 0; k < 12; ++k) acc[k] /= wsum;
+    float4* o = reinterpret_cast<float4*>(a.dst_seed + (size_t)di * stride);
+    float marker = wsum >= 0.99999f ? 0.0f : -1.0f;
+    if (q) {
+        o[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        o[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+        o[2] = make_float4(acc[8], acc[9], acc[10], acc[11]);
+        o[3] = make_float4(a.track_availability ? marker : 0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        o[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        o[1] = make_float4(acc[4], acc[5], a.track_availability ? marker : 0.0f, 0.0f);
+    }
+    a.dst_active[di] = (perm >> 4u) & 1u;
+}
+
+hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s) {
+    if (a.dst_count <= 0) return hipSuccess;
+    k_transfer_seed<<<(a.dst_count + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Outlier pass (shaders/icgn.wgsl:1087-1605).  Median values are exact order
+// statistics, so an insertion sort reproduces both the bubble-sorted 3x3 UOD
+// median and the quickselect used by the wider median policy.
+// ---------------------------------------------------------------------------
+template <int CAP>
+__device__ __forceinline__ float sorted_median(float* v, int n) {
+    for (int i = 1; i < n; ++i) {
+        float x = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j] > x) {
+            v[j + 1] = v[j];
+            --j;
+        }
+        v[j + 1] = x;
+    }
+    return (n & 1) ? v[n / 2] : 0.5f * (v[n / 2 - 1] + v[n / 2]);
+}
+
+__device__ __forceinline__ float recentred(const float* s, int model, int comp, float ox, float oy) {
+    if (model == kQuadratic) {
+        switch (comp) {
+            case 0: return s[0] + s[1] * ox + s[2] * oy + 0.5f * s[6] * ox * ox + s[7] * ox * oy + 0.5f * s[8] * oy * oy;
+            case 1: return s[1] + s[6] * ox + s[7] * oy;
+            case 2: return s[2] + s[7] * ox + s[8] * oy;
+            case 3: return s[3] + s[4] * ox + s[5] * oy + 0.5f * s[9] * ox * ox + s[10] * ox * oy + 0.5f * s[11] * oy * oy;
+            case 4: return s[4] + s[9] * ox + s[10] * oy;
+            case 5: return s[5] + s[10] * ox + s[11] * oy;
+            default: return s[comp];
+        }
+    }
+    switch (comp) {
+        case 0: return s[0] + s[1] * ox + s[2] * oy;
+        case 3: return s[3] + s[4] * ox + s[5] * oy;
+        default: return s[comp];
+    }
+}
+
+__constant__ int kComponentOrder[12] = {0, 3, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11};
+
+template <int CAP>
+__device__ void median_policy(const MedianArgs& a, int i) {
+    const int model = a.model;
+    const int stride = model == kQuadratic ? 16 : 8;
+    const float* own = a.state + (size_t)i * stride;
+    float* out = a.filtered + (size_t)i * stride;
+    float f[16];
+    for (int k = 0; k < stride; ++k) f[k] = own[k];
+    const float own_status = model == kQuadratic ? own[12] : own[6];
+    const float ref_status = a.metadata[(size_t)i * (model == kQuadratic ? 28 : 16) + 1];
+    bool skip = (own_status >= 0.5f && !a.scope) || (ref_status < 0.5f && ref_status != -4.0f);
+    if (!skip) {
+        const int px = i % a.grid_w, py = i / a.grid_w, gh = a.grid_count / a.grid_w;
+        const int radius = a.size / 2;
+        const int active = model == kTranslation ? 2 : (model == kQuadratic ? 12 : 6);
+        const float* perm = a.roi_words ? a.roi_words + (size_t)i * 4 : nullptr;
+        float values[CAP];
+        float filtered[12];
+        for (int k = 0; k < 12; ++k) filtered[k] = k < stride ? own[k] : 0.0f;
+        bool complete = true;
+        for (int ai = 0; ai < active && complete; ++ai) {
+            int comp = kComponentOrder[ai];
+            int count = 0;
+            if (own_status >= 0.5f) values[count++] = recentred(own, model, comp, 0.0f, 0.0f);
+            for (int dy = -radius; dy <= radius; ++dy)
+                for (int dx = -radius; dx <= radius; ++dx) {
+                    if (dx == 0 && dy == 0) continue;
+                    int nx = px + dx, ny = py + dy;
+                    if (nx < 0 || ny < 0 || nx >= a.grid_w || ny >= gh) continue;
+                    if (a.roi_enabled) {
+                        uint32_t bit = (uint32_t)((dy + 1) * 3 + dx + 1);
+                        uint32_t word = (uint32_t)perm[0];
+                        if (a.size > 3) {
+                            bit = (uint32_t)((dy + 4) * 9 + dx + 4);
+                            word = (uint32_t)perm[bit / 24u];
+                        }
+                        if ((word & (1u << (bit % 24u))) == 0u) continue;
+                    }
+                    const float* ns = a.state + (size_t)(ny * a.grid_w + nx) * stride;
+                    if ((model == kQuadratic ? ns[12] : ns[6]) < 0.5f) continue;
+                    values[count++] = recentred(ns, model, comp, -(float)a.D * dx, -(float)a.D * dy);
+                }
+            int minimum = (a.roi_enabled && a.size == 3 && own_status < 0.5f) ? 3 : 4;
+            if (count < minimum) {
+                complete = false;
+                break;
+            }
+            filtered[comp] = sorted_median<CAP>(values, count);
+        }
+        if (complete) {
+            for (int k = 0; k < 8; ++k) f[k] = filtered[k];
+            if (model == kQuadratic) {
+                for (int k = 8; k < 12; ++k) f[k] = filtered[k];
+                f[12] = (a.seed_available && own_status < 0.5f) ? 0.0f : own[12];
+            } else {
+                f[6] = (a.seed_available && own_status < 0.5f) ? 0.0f : own[6];
+                f[7] = own[7];
+            }
+        }
+    }
+    for (int k = 0; k < stride; ++k) out[k] = f[k];
+}
+
+__device__ void uod_policy(const MedianArgs& a, int i) {
+    const int model = a.model;
+    const int stride = model == kQuadratic ? 16 : 8;
+    const int ncomp = model == kQuadratic ? 12 : 6;
+    const float* own = a.state + (size_t)i * stride;
+    float* out = a.filtered + (size_t)i * stride;
+    float f[16];
+    for (int k = 0; k < stride; ++k) f[k] = own[k];
+    const float own_status = model == kQuadratic ? own[12] : own[6];
+    if (own_status >= 0.5f) {
+        const int px = i % a.grid_w, py = i / a.grid_w, gh = a.grid_count / a.grid_w;
+        float nb[8][12];
+        int count = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                int nx = px + dx, ny = py + dy;
+                if (nx < 0 || ny < 0 || nx >= a.grid_w || ny >= gh) continue;
+                if (a.roi_enabled && model != kQuadratic) {
+                    uint32_t perm = (uint32_t)a.roi_words[(size_t)i * 4];
+                    if ((perm & (1u << ((dy + 1) * 3 + dx + 1))) == 0u) continue;
+                }
+                const float* ns = a.state + (size_t)(ny * a.grid_w + nx) * stride;
+                if ((model == kQuadratic ? ns[12] : ns[6]) < 0.5f) continue;
+                for (int k = 0; k < ncomp; ++k) nb[count][k] = recentred(ns, model, k, -(float)a.D * dx, -(float)a.D * dy);
+                ++count;
+            }
+        if (count >= 4) {
+            float med[12], values[9];
+            for (int k = 0; k < ncomp; ++k) med[k] = own[k];
+            int model_count = model == kTranslation ? 2 : ncomp;
+            int median_count = a.policy == kReplace ? model_count : 2;
+            for (int ai = 0; ai < median_count; ++ai) {
+                int comp = kComponentOrder[ai];
+                for (int k = 0; k < count; ++k) values[k] = nb[k][comp];
+                med[comp] = sorted_median<9>(values, count);
+            }
+            bool flagged = false;
+            for (int ai = 0; ai < 2; ++ai) {
+                int comp = kComponentOrder[ai];
+                for (int k = 0; k < count; ++k) values[k] = fabsf(nb[k][comp] - med[comp]);
+                float spread = sorted_median<9>(values, count);
+                if (fabsf(own[comp] - med[comp]) / (spread + 0.1f) > 2.0f) flagged = true;
+            }
+            if (flagged) {
+                if (a.policy == kReplace)
+                    for (int ai = 0; ai < model_count; ++ai) f[kComponentOrder[ai]] = med[kComponentOrder[ai]];
+                if (model == kQuadratic) {
+                    f[12] = 2.0f;
+                    f[14] = 0.0f;
+                    f[15] = 0.0f;
+                } else {
+                    f[6] = 2.0f;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < stride; ++k) out[k] = f[k];
+}
+
+__global__ __launch_bounds__(256) void k_median(MedianArgs a) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.grid_count) return;
+    if (a.policy == kMedian) {
+        if (a.size <= 3)
+            median_policy<9>(a, i);
+        else
+            median_policy<81>(a, i);
+    } else {
+        uod_policy(a, i);
+    }
+}
+
+hipError_t launch_median(const MedianArgs& a, hipStream_t s) {
+    if (a.grid_count <= 0) return hipSuccess;
+    k_median<<<(a.grid_count + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Final affine ZNCC at the solved warp (shaders/affine_zncc.wgsl:57-99).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_affine_zncc(ZnccArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int index = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int N = a.grid_w * a.grid_h;
+    if (index >= N) return;
+    const float* p = a.state + (size_t)index * 8;
+    if (p[6] != 1.0f) {
+        if (lane == 0) a.out[index] = -2.0f;
+        return;
+    }
+    const int gx = index % a.grid_w, gy = index / a.grid_w;
+    const float half = ((float)a.S - 1.0f) * 0.5f;
+    float sf = 0, sg = 0, sff = 0, sgg = 0, sfg = 0, n = 0;
+    for (int k = lane; k < a.S * a.S; k += 64) {
+        int cx = k % a.S, cy = k / a.S;
+        int px = gx * a.D + cx, py = gy * a.D + cy;
+        float dx = (float)cx - half, dy = (float)cy - half;
+        float tx = (float)px + (p[0] + (p[1] * dx + p[2] * dy));
+        float ty = (float)py + (p[3] + (p[4] * dx + p[5] * dy));
+        if (tx >= 2.0f && ty >= 2.0f && tx <= (float)(a.width - 3) && ty <= (float)(a.height - 3)) {
+            float fx = floorf(tx), fy = floorf(ty);
+            int ix = (int)fx, iy = (int)fy;
+            float ax = tx - fx, ay = ty - fy;
+            auto raw = [&](int x, int y) { return (float)a.target[(size_t)y * a.width + x]; };
+            float g;
+            if (a.interpolation == kLinear) {
+                float v00 = raw(ix, iy) / 255.0f, v10 = raw(ix + 1, iy) / 255.0f;
+                float v01 = raw(ix, iy + 1) / 255.0f, v11 = raw(ix + 1, iy + 1) / 255.0f;
+                float top = v00 * (1.0f - ax) + v10 * ax, bottom = v01 * (1.0f - ax) + v11 * ax;
+                g = 255.0f * (top * (1.0f - ay) + bottom * ay);
+            } else {
+                float wx[4], wy[4];
+                float t2x = ax * ax, t3x = t2x * ax, t2y = ay * ay, t3y = t2y * ay;
+                if (a.interpolation == kKeys) {
+                    wx[0] = -0.5f * ax + t2x - 0.5f * t3x; wx[1] = 1.0f - 2.5f * t2x + 1.5f * t3x;
+                    wx[2] = 0.5f * ax + 2.0f * t2x - 1.5f * t3x; wx[3] = -0.5f * t2x + 0.5f * t3x;
+                    wy[0] = -0.5f * ay + t2y - 0.5f * t3y; wy[1] = 1.0f - 2.5f * t2y + 1.5f * t3y;
+                    wy[2] = 0.5f * ay + 2.0f * t2y - 1.5f * t3y; wy[3] = -0.5f * t2y + 0.5f * t3y;
+                } else {
+                    float ix1 = 1.0f - ax, iy1 = 1.0f - ay;
+                    wx[0] = ix1 * ix1 * ix1 / 6.0f; wx[1] = (3.0f * t3x - 6.0f * t2x + 4.0f) / 6.0f;
+                    wx[2] = (-3.0f * t3x + 3.0f * t2x + 3.0f * ax + 1.0f) / 6.0f; wx[3] = t3x / 6.0f;
+                    wy[0] = iy1 * iy1 * iy1 / 6.0f; wy[1] = (3.0f * t3y - 6.0f * t2y + 4.0f) / 6.0f;
+                    wy[2] = (-3.0f * t3y + 3.0f * t2y + 3.0f * ay + 1.0f) / 6.0f; wy[3] = t3y / 6.0f;
+                }
+                g = 0.0f;
+                for (int j = 0; j < 4; ++j)
+                    for (int i = 0; i < 4; ++i) {
+                        float pix = a.interpolation == kBspline ? a.coeff[(size_t)(iy + j - 1) * a.width + ix + i - 1]
+                                                                : raw(ix + i - 1, iy + j - 1);
+                        g += wx[i] * wy[j] * pix;
+                    }
+            }
+            float f = a.reference[(size_t)py * a.width + px].x;
+            sf += f; sg += g; sff += f * f; sgg += g * g; sfg += f * g; n += 1.0f;
+        }
+    }
+    sf = wave_sum(sf); sg = wave_sum(sg); sff = wave_sum(sff); sgg = wave_sum(sgg); sfg = wave_sum(sfg); n = wave_sum(n);
+    if (lane == 0) {
+        float vf = sff - sf * sf / fmaxf(n, 1.0f), vg = sgg - sg * sg / fmaxf(n, 1.0f);
+        a.out[index] = (n == (float)(a.S * a.S) && vf > 1e-9f && vg > 1e-9f)
+                           ? fminf(fmaxf((sfg - sf * sg / n) / sqrtf(vf * vg), -1.0f), 1.0f)
+                           : -2.0f;
+    }
+}
+
+hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s) {
+    int N = a.grid_w * a.grid_h;
+    if (N <= 0) return hipSuccess;
+    k_affine_zncc<<<(N + 3) / 4, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace ldic

@@ -1,0 +1,288 @@
+// Per-subset reference statistics in float64 (wgpu_backend.py:639-823): the
+// anchored mean, energy, J^T f, sum J, H = sum w J J^T and its inverse by
+// Gauss-Jordan elimination with partial pivoting.  One wave64 per subset:
+// lanes stride over the subset pixels, DPP wave sums give every lane the
+// totals, and the inversion runs column-parallel on the wave's LDS slice.
+#include "ldic_device.h"
+#include "ldic_kernels.h"
+
+#include <float.h>
+
+namespace ldic {
+
+constexpr int kPrepWaves = 4;
+
+// Row-major upper-triangle enumeration of a P x P symmetric matrix.
+__host__ __device__ constexpr int tri_row(int e, int P) {
+    int i = 0;
+    while (e >= P - i) {
+        e -= P - i;
+        ++i;
+    }
+    return i;
+}
+__host__ __device__ constexpr int tri_col(int e, int P) {
+    int i = 0;
+    while (e >= P - i) {
+        e -= P - i;
+        ++i;
+    }
+    return i + e;
+}
+
+template <int P>
+__device__ __forceinline__ void jacobian(double gx, double gy, double lx, double ly, double J[P]) {
+#pragma clang fp contract(off)
+    J[0] = gx;
+    J[3] = gy;
+    if (P > 2) {
+        J[1] = gx * lx;
+        J[2] = gx * ly;
+        J[4] = gy * lx;
+        J[5] = gy * ly;
+    }
+    if (P == 12) {
+        double hx2 = 0.5 * lx * lx, xy = lx * ly, hy2 = 0.5 * ly * ly;
+        J[6] = gx * hx2;
+        J[7] = gx * xy;
+        J[8] = gx * hy2;
+        J[9] = gy * hx2;
+        J[10] = gy * xy;
+        J[11] = gy * hy2;
+    }
+}
+
+struct PixelRef {
+    double f, gx, gy, lx, ly, w;
+};
+
+__device__ __forceinline__ PixelRef pixel(const PrepArgs& a, int p, int ox, int oy, double half, double mean) {
+    int row = p / a.S, col = p - row * a.S;
+    size_t idx = (size_t)(oy + row) * a.width + ox + col;
+    PixelRef r;
+    r.f = a.intensity[idx] - mean;
+    r.gx = a.grad_x[idx];
+    r.gy = a.grad_y[idx];
+    r.lx = (double)col - half;
+    r.ly = (double)row - half;
+    r.w = a.weighted ? a.weights[p] : 1.0;
+    return r;
+}
+
+// Accumulate upper-triangle entries [E0, E1) of H into LDS matrix `h` (P x P).
+template <int P, int E0, int E1>
+__device__ void accumulate_hessian(const PrepArgs& a, int lane, int ox, int oy, double half, double* h, int pitch) {
+    constexpr int NE = E1 - E0;
+    double acc[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) acc[e] = 0.0;
+    const int SS = a.S * a.S;
+    for (int p = lane; p < SS; p += kWave) {
+        PixelRef r = pixel(a, p, ox, oy, half, 0.0);
+        double J[12];
+        jacobian<P == 2 ? 6 : P>(r.gx, r.gy, r.lx, r.ly, J);
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int i = tri_row(E0 + e, P == 2 ? 2 : P), j = tri_col(E0 + e, P == 2 ? 2 : P);
+            // Translation uses the (gx, gy) pair only: indices 0 and 3.
+            const int ii = P == 2 ? 3 * i : i, jj = P == 2 ? 3 * j : j;
+            acc[e] += J[ii] * J[jj] * r.w;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        double v = wave_sum(acc[e]);
+        const int i = tri_row(E0 + e, P == 2 ? 2 : P), j = tri_col(E0 + e, P == 2 ? 2 : P);
+        if (lane == 0) {
+            h[i * pitch + j] = v;
+            h[j * pitch + i] = v;
+        }
+    }
+}
+
+// Column-parallel Gauss-Jordan elimination with partial pivoting on the
+// P x 2P augmented matrix held in LDS.  Returns false for an exactly singular
+// matrix (a zero pivot), which is when LAPACK's LU factorisation fails too.
+template <int P>
+__device__ bool invert(double* m, int lane) {
+    constexpr int C = 2 * P;
+    // m is P rows x C columns; columns P..2P-1 start as the identity.
+    if (lane < P)
+        for (int r = 0; r < P; ++r) m[r * C + P + lane] = r == lane ? 1.0 : 0.0;
+    wave_lds_sync();
+    bool singular = false;
+    for (int col = 0; col < P; ++col) {
+        int pr = col;
+        double ps = fabs(m[col * C + col]);
+        for (int cand = col + 1; cand < P; ++cand) {
+            double cs = fabs(m[cand * C + col]);
+            if (cs > ps) {
+                pr = cand;
+                ps = cs;
+            }
+        }
+        if (ps == 0.0) {
+            singular = true;
+            break;
+        }
+        if (pr != col && lane < C) {
+            double x = m[col * C + lane];
+            m[col * C + lane] = m[pr * C + lane];
+            m[pr * C + lane] = x;
+        }
+        wave_lds_sync();
+        double pivot = m[col * C + col];
+        double factors[P];
+#pragma unroll
+        for (int r = 0; r < P; ++r) factors[r] = m[r * C + col];
+        wave_lds_sync();
+        if (lane < C) {
+            double top = m[col * C + lane] / pivot;
+            m[col * C + lane] = top;
+#pragma unroll
+            for (int r = 0; r < P; ++r)
+                if (r != col) m[r * C + lane] -= factors[r] * top;
+        }
+        wave_lds_sync();
+    }
+    return !singular;
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
+#pragma clang fp contract(off)
+    constexpr int P = MODEL == kQuadratic ? 12 : 6;
+    constexpr int M = MODEL == kQuadratic ? 28 : 16;
+    __shared__ double mats[kPrepWaves][P * 2 * P];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int subset = xcd_block(blockIdx.x, gridDim.x) * kPrepWaves + wave;
+    const int N = a.grid_w * a.grid_h;
+    if (subset >= N) return;
+    double* mat = mats[wave];
+    const int gx = subset % a.grid_w, gy = subset / a.grid_w;
+    const int ox = gx * a.D, oy = gy * a.D;
+    const double half = 0.5 * (double)(a.S - 1);
+    const int SS = a.S * a.S;
+
+    // Pass 0: weighted mean of the raw (or Gaussian-filtered) intensity.
+    double s0 = 0.0;
+    for (int p = lane; p < SS; p += kWave) {
+        int row = p / a.S, col = p - row * a.S;
+        double w = a.weighted ? a.weights[p] : 1.0;
+        s0 += a.intensity[(size_t)(oy + row) * a.width + ox + col] * w;
+    }
+    const double mean = wave_sum(s0) / a.weight_sum;
+
+    // Pass 1: energy, J^T f and sum J.
+    double energy = 0.0, jtf[P], sj[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) jtf[k] = sj[k] = 0.0;
+    for (int p = lane; p < SS; p += kWave) {
+        PixelRef r = pixel(a, p, ox, oy, half, mean);
+        double J[P];
+        jacobian<P>(r.gx, r.gy, r.lx, r.ly, J);
+        energy += r.f * r.f * r.w;
+        if (MODEL == kTranslation) {
+            jtf[0] += J[0] * r.f * r.w;
+            jtf[3] += J[3] * r.f * r.w;
+            sj[0] += J[0] * r.w;
+            sj[3] += J[3] * r.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                jtf[k] += J[k] * r.f * r.w;
+                sj[k] += J[k] * r.w;
+            }
+        }
+        if (a.D == a.S) {
+            // Non-overlapping tiles: store the tile-centred intensity.
+            int row = p / a.S, col = p - row * a.S;
+            a.reference[(size_t)(oy + row) * a.width + ox + col].x = (float)r.f;
+        }
+    }
+    energy = wave_sum(energy);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        jtf[k] = wave_sum(jtf[k]);
+        sj[k] = wave_sum(sj[k]);
+    }
+
+    // Pass 2: Hessian, accumulated in chunks that bound register use, written
+    // straight into the augmented P x 2P layout used by the inversion.
+    constexpr int C = 2 * P;
+    if (MODEL == kQuadratic) {
+        accumulate_hessian<12, 0, 33>(a, lane, ox, oy, half, mat, C);
+        accumulate_hessian<12, 33, 63>(a, lane, ox, oy, half, mat, C);
+        accumulate_hessian<12, 63, 78>(a, lane, ox, oy, half, mat, C);
+    } else if (MODEL == kAffine) {
+        accumulate_hessian<6, 0, 21>(a, lane, ox, oy, half, mat, C);
+    } else {
+        accumulate_hessian<2, 0, 3>(a, lane, ox, oy, half, mat, C);
+    }
+    wave_lds_sync();
+
+    bool valid;
+    double t00 = 0.0, t03 = 0.0, t33 = 0.0;
+    if (MODEL == kTranslation) {
+        double hxx = mat[0], hxy = mat[1], hyy = mat[C + 1];
+        double det = hxx * hyy - hxy * hxy;
+        double scale = fmax(fabs(hxx * hyy), 1.0);
+        valid = isfinite(det) && fabs(det) > DBL_EPSILON * scale && energy > 1e-12;
+        double idet = valid ? 1.0 / det : 0.0;
+        t00 = hyy * idet;
+        t03 = -hxy * idet;
+        t33 = hxx * idet;
+        valid = valid && fmax(fmax(fabs(t00), fabs(t03)), fabs(t33)) <= (double)FLT_MAX;
+    } else {
+        bool ok = invert<P>(mat, lane);
+        double peak = 0.0;
+        bool finite = true;
+        for (int k = 0; k < P * P; ++k) {
+            double v = mat[(k / P) * C + P + k % P];
+            finite = finite && isfinite(v);
+            peak = fmax(peak, fabs(v));
+        }
+        valid = ok && finite && energy > 1e-12 && peak <= (double)FLT_MAX;
+    }
+
+    float* meta = a.metadata + (size_t)subset * M;
+    float* invp = a.inverse + (size_t)subset * P * P;
+    if (lane == 0) {
+        float status = valid ? 1.0f : 0.0f;
+        if (a.valid) a.valid[subset] = valid ? 1 : 0;
+        if (a.selected && a.selected[subset] == 0) status = -3.0f;
+        meta[0] = (float)sqrt(energy);
+        meta[1] = status;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            meta[2 + k] = (float)jtf[k];
+            meta[2 + P + k] = (float)sj[k];
+        }
+        meta[2 + 2 * P] = a.D == a.S ? 0.0f : (float)mean;
+        meta[3 + 2 * P] = (float)a.weight_sum;
+    }
+    for (int k = lane; k < P * P; k += kWave) {
+        double v;
+        if (MODEL == kTranslation)
+            v = k == 0 ? t00 : (k == 3 || k == 18) ? t03 : k == 21 ? t33 : 0.0;
+        else
+            v = mat[(k / P) * C + P + k % P];
+        invp[k] = valid ? (float)v : 0.0f;
+    }
+}
+
+hipError_t launch_subset_prep(const PrepArgs& a, hipStream_t s) {
+    int N = a.grid_w * a.grid_h;
+    if (N <= 0) return hipSuccess;
+    dim3 grid((N + kPrepWaves - 1) / kPrepWaves);
+    switch (a.model) {
+        case kAffine: k_subset_prep<kAffine><<<grid, 256, 0, s>>>(a); break;
+        case kTranslation: k_subset_prep<kTranslation><<<grid, 256, 0, s>>>(a); break;
+        case kQuadratic: k_subset_prep<kQuadratic><<<grid, 256, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ldic

@@ -1,0 +1,879 @@
+// Host runtime of liblivedic.so: engine, reference preparation, per-frame
+// launch sequence, streaming slots and the C ABI declared in livedic.h.
+//
+// Memory plan (MI355X, 288 GB HBM): everything a frame touches is allocated
+// once per reference -- reference planes, metadata and inverse Hessians per
+// level, and per slot the target pyramid, coefficient planes, state, seed and
+// outlier buffers plus pinned host staging.  A frame performs no allocation.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ldic_kernels.h"
+#include "livedic.h"
+
+using namespace ldic;
+
+namespace {
+
+thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                                                \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return set_error(LDIC_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK(expr)            \
+    do {                       \
+        int r_ = (expr);       \
+        if (r_ != LDIC_OK) return r_; \
+    } while (0)
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Device allocations owned by one reference generation or one slot.
+struct Arena {
+    std::vector<void*> device;
+    std::vector<void*> pinned;
+    template <class T>
+    int alloc(T** out, size_t count) {
+        void* p = nullptr;
+        size_t bytes = count * sizeof(T);
+        if (bytes == 0) bytes = sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return set_error(LDIC_ERR_MEMORY, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        device.push_back(p);
+        *out = static_cast<T*>(p);
+        return LDIC_OK;
+    }
+    template <class T>
+    int alloc_pinned(T** out, size_t count) {
+        void* p = nullptr;
+        hipError_t e = hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) return set_error(LDIC_ERR_MEMORY, "hipHostMalloc failed: %s", hipGetErrorString(e));
+        pinned.push_back(p);
+        *out = static_cast<T*>(p);
+        return LDIC_OK;
+    }
+    void release() {
+        for (void* p : device) (void)hipFree(p);
+        for (void* p : pinned) (void)hipHostFree(p);
+        device.clear();
+        pinned.clear();
+    }
+};
+
+struct Level {
+    int width = 0, height = 0, gw = 0, gh = 0;
+    uint8_t* image = nullptr;  // reference image of this level
+    float4* reference = nullptr;
+    float* metadata = nullptr;
+    float* inverse = nullptr;
+    uint8_t* valid = nullptr;
+    uint8_t* selected = nullptr;
+    float* roi_words = nullptr;
+    uint32_t* seed_links = nullptr;
+    bool roi = false;
+    ldic_level_config config{30, 1e-3f, 0.8f};
+    int count() const { return gw * gh; }
+};
+
+struct SlotLevel {
+    uint8_t* target = nullptr;
+    float* coeff = nullptr;
+    float* state = nullptr;
+    float* filtered = nullptr;
+    float* seed = nullptr;
+    uint32_t* active = nullptr;
+};
+
+constexpr int kEventRing = 256;
+
+struct Slot {
+    Arena arena;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    // Per-frame solver events; frames queued since the last wait own
+    // ring entries [ring_first, ring_first + ring_count).
+    hipEvent_t ev_solve0[kEventRing] = {}, ev_solve1[kEventRing] = {};
+    int ring_first = 0, ring_count = 0;
+    double drained_solve_ms = 0.0;
+    int drained_frames = 0;
+    SlotLevel lv[LDIC_MAX_LEVELS];
+    uint8_t* pinned_in = nullptr;
+    float* pinned_out = nullptr;
+    float* zncc = nullptr;
+    float* pinned_zncc = nullptr;
+    bool pending = false;
+    bool resident = false;
+    double upload_ms = 0.0;
+    double submitted_at = 0.0;
+};
+
+}  // namespace
+
+struct ldic_engine {
+    ldic_options opt{};
+    int device = 0;
+    hipDeviceProp_t prop{};
+    int levels = 0;
+    Level lv[LDIC_MAX_LEVELS];
+    Arena ref_arena;
+    std::vector<Slot*> slots;
+    float* fir = nullptr;       // 31 FIR taps on the device
+    Arena const_arena;
+    hipStream_t prep_stream = nullptr;
+    int tile_cap = 0;
+    size_t solve_lds = 0;
+    bool ready = false;
+    double gauss7[7];
+    std::vector<double> weights;  // S*S integration weights (Gaussian weighting)
+    double weight_sum = 0.0;
+};
+
+namespace {
+
+int output_floats(const ldic_engine* e) { return e->lv[0].count() * state_stride(e->opt.motion_model); }
+
+void free_slots(ldic_engine* e) {
+    for (Slot* s : e->slots) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        s->arena.release();
+        if (s->ev_begin) (void)hipEventDestroy(s->ev_begin);
+        for (int i = 0; i < kEventRing; ++i) {
+            if (s->ev_solve0[i]) (void)hipEventDestroy(s->ev_solve0[i]);
+            if (s->ev_solve1[i]) (void)hipEventDestroy(s->ev_solve1[i]);
+        }
+        if (s->ev_end) (void)hipEventDestroy(s->ev_end);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+    }
+    e->slots.clear();
+}
+
+int create_slot(ldic_engine* e, Slot** out) {
+    Slot* s = new Slot();
+    e->slots.push_back(s);
+    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreate(&s->ev_begin));
+    for (int i = 0; i < kEventRing; ++i) {
+        HIP_CHECK(hipEventCreate(&s->ev_solve0[i]));
+        HIP_CHECK(hipEventCreate(&s->ev_solve1[i]));
+    }
+    HIP_CHECK(hipEventCreate(&s->ev_end));
+    const int stride = state_stride(e->opt.motion_model);
+    for (int k = 0; k < e->levels; ++k) {
+        const Level& L = e->lv[k];
+        SlotLevel& sl = s->lv[k];
+        size_t px = (size_t)L.width * L.height;
+        size_t n = (size_t)L.count();
+        CHECK(s->arena.alloc(&sl.target, px));
+        if (e->opt.interpolation == kBspline) CHECK(s->arena.alloc(&sl.coeff, px));
+        CHECK(s->arena.alloc(&sl.state, n * stride));
+        CHECK(s->arena.alloc(&sl.filtered, n * stride));
+        CHECK(s->arena.alloc(&sl.seed, n * stride));
+        CHECK(s->arena.alloc(&sl.active, n));
+    }
+    CHECK(s->arena.alloc(&s->zncc, (size_t)e->lv[0].count()));
+    CHECK(s->arena.alloc_pinned(&s->pinned_in, (size_t)e->lv[0].width * e->lv[0].height));
+    CHECK(s->arena.alloc_pinned(&s->pinned_out, (size_t)output_floats(e)));
+    CHECK(s->arena.alloc_pinned(&s->pinned_zncc, (size_t)e->lv[0].count()));
+    *out = s;
+    return LDIC_OK;
+}
+
+int ensure_slots(ldic_engine* e, int count) {
+    HIP_CHECK(hipSetDevice(e->device));
+    while ((int)e->slots.size() < count) {
+        Slot* s = nullptr;
+        CHECK(create_slot(e, &s));
+    }
+    return LDIC_OK;
+}
+
+const float* level_output(const ldic_engine* e, const SlotLevel& sl) {
+    return e->opt.outlier_policy == kDisabled ? sl.state : sl.filtered;
+}
+
+SolveArgs solve_args(const ldic_engine* e, int k, const SlotLevel& sl, const float* seed, const uint32_t* active,
+                     int preserve) {
+    const Level& L = e->lv[k];
+    SolveArgs a{};
+    a.width = L.width;
+    a.height = L.height;
+    a.grid_w = L.gw;
+    a.grid_h = L.gh;
+    a.S = e->opt.subset_size;
+    a.D = e->opt.subset_stride;
+    a.iterations = L.config.iterations;
+    a.tolerance = L.config.tolerance;
+    a.zncc_min = L.config.zncc_min;
+    a.inv_var = 1.0f / (e->opt.gaussian_sigma * e->opt.gaussian_sigma);
+    a.roi_enabled = L.roi ? 1 : 0;
+    a.preserve_disabled = preserve;
+    a.tile_cap = e->tile_cap;
+    a.reference = L.reference;
+    a.metadata = L.metadata;
+    a.inverse = L.inverse;
+    a.coeff = sl.coeff;
+    a.target = sl.target;
+    a.seed = seed;
+    a.active = active;
+    a.state = sl.state;
+    return a;
+}
+
+int launch_median_level(ldic_engine* e, int k, SlotLevel& sl, bool seed_available, hipStream_t st) {
+    if (e->opt.outlier_policy == kDisabled) return LDIC_OK;
+    const Level& L = e->lv[k];
+    MedianArgs m{};
+    m.grid_w = L.gw;
+    m.grid_count = L.count();
+    m.D = e->opt.subset_stride;
+    m.model = e->opt.motion_model;
+    m.policy = e->opt.outlier_policy;
+    m.size = e->opt.median_filter_size;
+    m.scope = e->opt.median_filter_scope;
+    m.roi_enabled = L.roi ? 1 : 0;
+    m.seed_available = seed_available ? 1 : 0;
+    m.state = sl.state;
+    m.metadata = L.metadata;
+    m.roi_words = L.roi_words;
+    m.filtered = sl.filtered;
+    HIP_CHECK(launch_median(m, st));
+    return LDIC_OK;
+}
+
+// The per-frame command sequence of full_pyramid_backend.py:1436-1569: target
+// reductions fine-to-coarse, then per level (coarse-to-fine) seed transfer,
+// prefilter, solve and outlier pass.  The target of level 0 is in place.
+// Sum the solver times of all queued frames (waits for them).
+int drain_ring(Slot* s) {
+    for (int i = 0; i < s->ring_count; ++i) {
+        int r = (s->ring_first + i) % kEventRing;
+        HIP_CHECK(hipEventSynchronize(s->ev_solve1[r]));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, s->ev_solve0[r], s->ev_solve1[r]));
+        s->drained_solve_ms += ms;
+        s->drained_frames += 1;
+    }
+    s->ring_first = (s->ring_first + s->ring_count) % kEventRing;
+    s->ring_count = 0;
+    return LDIC_OK;
+}
+
+int enqueue_frame(ldic_engine* e, Slot* s) {
+    hipStream_t st = s->stream;
+    const int L = e->levels;
+    if (s->ring_count == kEventRing) CHECK(drain_ring(s));
+    const int ring = (s->ring_first + s->ring_count) % kEventRing;
+    s->ring_count += 1;
+    for (int k = 1; k < L; ++k)
+        HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, st));
+    for (int k = L - 1; k >= 0; --k) {
+        SlotLevel& sl = s->lv[k];
+        const Level& lev = e->lv[k];
+        const float* seed = nullptr;
+        const uint32_t* active = nullptr;
+        if (k < L - 1) {
+            const Level& src = e->lv[k + 1];
+            SeedArgs sa{};
+            sa.src_grid_w = src.gw;
+            sa.src_grid_h = src.gh;
+            sa.dst_grid_w = lev.gw;
+            sa.dst_count = lev.count();
+            sa.src_scale = (float)(1 << (k + 1));
+            sa.dst_scale = (float)(1 << k);
+            sa.S = (float)e->opt.subset_size;
+            sa.D = (float)e->opt.subset_stride;
+            sa.model = e->opt.motion_model;
+            sa.track_availability = 0;
+            sa.src_state = level_output(e, s->lv[k + 1]);
+            sa.permissions = lev.seed_links;
+            sa.dst_seed = sl.seed;
+            sa.dst_active = sl.active;
+            HIP_CHECK(launch_transfer_seed(sa, st));
+            seed = sl.seed;
+            active = sl.active;
+        }
+        if (e->opt.interpolation == kBspline)
+            HIP_CHECK(launch_prefilter(sl.target, lev.width, lev.height, e->fir, sl.coeff, st));
+        if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
+        HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting,
+                               solve_args(e, k, sl, seed, active, 0), st));
+        if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
+        CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
+    }
+    return LDIC_OK;
+}
+
+int copy_in(ldic_engine* e, Slot* s, const uint8_t* target, int64_t row_stride) {
+    const Level& L = e->lv[0];
+    double t0 = now_ms();
+    if (row_stride == L.width) {
+        std::memcpy(s->pinned_in, target, (size_t)L.width * L.height);
+    } else {
+        for (int y = 0; y < L.height; ++y)
+            std::memcpy(s->pinned_in + (size_t)y * L.width, target + (size_t)y * row_stride, (size_t)L.width);
+    }
+    HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->lv[0].target, s->pinned_in, (size_t)L.width * L.height, hipMemcpyHostToDevice,
+                             s->stream));
+    s->upload_ms = now_ms() - t0;
+    return LDIC_OK;
+}
+
+int check_ready(ldic_engine* e) {
+    if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
+    if (!e->ready) return set_error(LDIC_ERR_STATE, "call ldic_set_reference before registering targets");
+    return LDIC_OK;
+}
+
+int check_slot(ldic_engine* e, int slot) {
+    CHECK(check_ready(e));
+    if (slot < 0 || slot >= (int)e->slots.size())
+        return set_error(LDIC_ERR_ARGUMENT, "slot %d is outside the %zu allocated slots", slot, e->slots.size());
+    return LDIC_OK;
+}
+
+int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wall_start) {
+    double t0 = now_ms();
+    HIP_CHECK(hipEventSynchronize(s->ev_end));
+    CHECK(drain_ring(s));
+    s->pending = false;
+    if (out_state) std::memcpy(out_state, s->pinned_out, sizeof(float) * (size_t)output_floats(e));
+    if (t) {
+        float gpu = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&gpu, s->ev_begin, s->ev_end));
+        t->upload_ms = s->upload_ms;
+        t->gpu_ms = gpu;
+        t->solve_ms = s->drained_solve_ms;
+        t->frames = s->drained_frames;
+        t->readback_ms = now_ms() - t0;
+        t->wall_ms = now_ms() - wall_start;
+    }
+    s->drained_solve_ms = 0.0;
+    s->drained_frames = 0;
+    return LDIC_OK;
+}
+
+int submit_frame(ldic_engine* e, Slot* s, const uint8_t* target, int64_t row_stride) {
+    if (s->pending) return set_error(LDIC_ERR_STATE, "slot still holds an unretired frame");
+    CHECK(copy_in(e, s, target, row_stride));
+    CHECK(enqueue_frame(e, s));
+    const float* out = level_output(e, s->lv[0]);
+    HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)output_floats(e), hipMemcpyDeviceToHost,
+                             s->stream));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->pending = true;
+    s->resident = false;
+    s->submitted_at = now_ms();
+    return LDIC_OK;
+}
+
+void gaussian_taps(double w[7]) {
+    double sum = 0.0;
+    for (int i = 0; i < 7; ++i) {
+        double o = (double)(i - 3);
+        w[i] = std::exp(-0.5 * o * o);
+    }
+    for (int i = 0; i < 7; ++i) sum += w[i];
+    for (int i = 0; i < 7; ++i) w[i] /= sum;
+}
+
+// 31 float32 FIR taps: Gaussian(sigma=1) convolved with sqrt(3) z^|k|, |k|<=12
+// (wgpu_backend.py:527-541).
+void fir_taps(float out[31]) {
+    double g[7];
+    gaussian_taps(g);
+    double w[31] = {0};
+    const double z = std::sqrt(3.0) - 2.0;
+    for (int gi = 0; gi < 7; ++gi)
+        for (int k = -12; k <= 12; ++k) w[(gi - 3) + k + 15] += g[gi] * std::sqrt(3.0) * std::pow(z, std::abs(k));
+    for (int i = 0; i < 31; ++i) out[i] = (float)w[i];
+}
+
+int validate_options(const ldic_options* o) {
+    if (!o) return set_error(LDIC_ERR_ARGUMENT, "options is NULL");
+    if (o->motion_model < 0 || o->motion_model > 2) return set_error(LDIC_ERR_ARGUMENT, "motion_model must be 0, 1 or 2");
+    if (o->interpolation < 0 || o->interpolation > 2) return set_error(LDIC_ERR_ARGUMENT, "interpolation must be 0, 1 or 2");
+    if (o->subset_size < 4 || o->subset_size > 46340) return set_error(LDIC_ERR_ARGUMENT, "subset_size must be 4..46340");
+    if (o->subset_stride < 1 || o->subset_stride > o->subset_size)
+        return set_error(LDIC_ERR_ARGUMENT, "subset_stride must be between 1 and subset_size");
+    if (o->spatial_weighting != 0 && o->spatial_weighting != 1)
+        return set_error(LDIC_ERR_ARGUMENT, "spatial_weighting must be 0 or 1");
+    if (!(o->gaussian_sigma > 0.0f) || !std::isfinite(o->gaussian_sigma))
+        return set_error(LDIC_ERR_ARGUMENT, "gaussian_sigma must be finite and positive");
+    if (o->outlier_policy < 0 || o->outlier_policy > 3) return set_error(LDIC_ERR_ARGUMENT, "outlier_policy must be 0..3");
+    if (o->median_filter_size != 3 && o->median_filter_size != 5 && o->median_filter_size != 7 &&
+        o->median_filter_size != 9)
+        return set_error(LDIC_ERR_ARGUMENT, "median_filter_size must be 3, 5, 7 or 9");
+    if (o->median_filter_scope != 0 && o->median_filter_scope != 1)
+        return set_error(LDIC_ERR_ARGUMENT, "median_filter_scope must be 0 or 1");
+    if (o->pyramid_levels < 1 || o->pyramid_levels > LDIC_MAX_LEVELS)
+        return set_error(LDIC_ERR_ARGUMENT, "pyramid_levels must be 1..4");
+    return LDIC_OK;
+}
+
+int validate_config(const ldic_level_config* c) {
+    if (c->iterations < 1 || c->iterations > 30) return set_error(LDIC_ERR_ARGUMENT, "iterations must be 1..30");
+    if (!(c->tolerance > 0.0f) || !std::isfinite(c->tolerance))
+        return set_error(LDIC_ERR_ARGUMENT, "tolerance must be finite and positive");
+    if (!(c->zncc_min >= 0.0f && c->zncc_min <= 1.0f)) return set_error(LDIC_ERR_ARGUMENT, "zncc_min must be in [0, 1]");
+    return LDIC_OK;
+}
+
+int prepare_level(ldic_engine* e, int k, double* G, double* A, double* B, double* GX, double* GY, double* weights,
+                  hipStream_t st) {
+    Level& L = e->lv[k];
+    const int w = L.width, h = L.height;
+    const bool bspline = e->opt.interpolation == kBspline;
+    if (bspline) {
+        HIP_CHECK(launch_gaussian(L.image, w, h, e->gauss7, B, G, st));
+        HIP_CHECK(hipMemcpyAsync(A, G, sizeof(double) * (size_t)w * h, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(launch_bspline(A, B, w, h, st));
+        HIP_CHECK(launch_gradients(A, G, w, h, 1, GX, GY, L.reference, st));
+    } else {
+        HIP_CHECK(launch_to_double(L.image, w, h, G, st));
+        HIP_CHECK(launch_gradients(G, G, w, h, 0, GX, GY, L.reference, st));
+    }
+    PrepArgs p{};
+    p.width = w;
+    p.height = h;
+    p.grid_w = L.gw;
+    p.grid_h = L.gh;
+    p.S = e->opt.subset_size;
+    p.D = e->opt.subset_stride;
+    p.model = e->opt.motion_model;
+    p.weighted = e->opt.spatial_weighting;
+    p.direct = bspline ? 0 : 1;
+    p.intensity = G;
+    p.grad_x = GX;
+    p.grad_y = GY;
+    p.weights = weights;
+    p.weight_sum = e->weight_sum;
+    p.selected = L.roi ? L.selected : nullptr;
+    p.reference = L.reference;
+    p.metadata = L.metadata;
+    p.inverse = L.inverse;
+    p.valid = L.valid;
+    HIP_CHECK(launch_subset_prep(p, st));
+    return LDIC_OK;
+}
+
+int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int height, int64_t row_stride,
+                       const ldic_level_config* configs, const ldic_level_roi* rois, uint8_t* out_valid) {
+    if (!image || !configs) return set_error(LDIC_ERR_ARGUMENT, "image and configs are required");
+    if (width < 16 || height < 16) return set_error(LDIC_ERR_ARGUMENT, "image dimensions must be at least 16");
+    if (row_stride < width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
+    const int S = e->opt.subset_size, D = e->opt.subset_stride;
+    // Validate every level before touching the previous reference.
+    int w = width, h = height;
+    int dims[LDIC_MAX_LEVELS][4];
+    for (int k = 0; k < e->opt.pyramid_levels; ++k) {
+        CHECK(validate_config(&configs[k]));
+        if (k > 0) {
+            w /= 2;
+            h /= 2;
+        }
+        if (w < S || h < S)
+            return set_error(LDIC_ERR_ARGUMENT, "level %d (%dx%d) is smaller than the subset size %d", k, w, h, S);
+        dims[k][0] = w;
+        dims[k][1] = h;
+        dims[k][2] = 1 + (w - S) / D;
+        dims[k][3] = 1 + (h - S) / D;
+    }
+    HIP_CHECK(hipSetDevice(e->device));
+    e->ready = false;
+    free_slots(e);
+    e->ref_arena.release();
+    e->levels = e->opt.pyramid_levels;
+    const int M = metadata_stride(e->opt.motion_model), P = param_count(e->opt.motion_model);
+    for (int k = 0; k < e->levels; ++k) {
+        Level& L = e->lv[k];
+        L = Level();
+        L.width = dims[k][0];
+        L.height = dims[k][1];
+        L.gw = dims[k][2];
+        L.gh = dims[k][3];
+        L.config = configs[k];
+        size_t px = (size_t)L.width * L.height, n = (size_t)L.count();
+        CHECK(e->ref_arena.alloc(&L.image, px));
+        CHECK(e->ref_arena.alloc(&L.reference, px));
+        CHECK(e->ref_arena.alloc(&L.metadata, n * M));
+        CHECK(e->ref_arena.alloc(&L.inverse, n * P * P));
+        CHECK(e->ref_arena.alloc(&L.valid, n));
+        const ldic_level_roi* r = rois ? &rois[k] : nullptr;
+        if (r && r->selected) {
+            L.roi = true;
+            CHECK(e->ref_arena.alloc(&L.selected, n));
+            HIP_CHECK(hipMemcpy(L.selected, r->selected, n, hipMemcpyHostToDevice));
+            CHECK(e->ref_arena.alloc(&L.roi_words, n * 4));
+            if (r->neighbour_words) {
+                HIP_CHECK(hipMemcpy(L.roi_words, r->neighbour_words, n * 4 * sizeof(float), hipMemcpyHostToDevice));
+            } else {
+                HIP_CHECK(hipMemset(L.roi_words, 0, n * 4 * sizeof(float)));
+            }
+        }
+        if (r && r->seed_links) {
+            CHECK(e->ref_arena.alloc(&L.seed_links, n));
+            HIP_CHECK(hipMemcpy(L.seed_links, r->seed_links, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
+    }
+    // Integration weights of one subset (Gaussian mode).
+    double* d_weights = nullptr;
+    if (e->opt.spatial_weighting == 1) {
+        std::vector<double> wts((size_t)S * S);
+        double half = 0.5 * (S - 1), sigma = e->opt.gaussian_sigma;
+        for (int r = 0; r < S; ++r)
+            for (int c = 0; c < S; ++c) {
+                double lx = c - half, ly = r - half;
+                wts[(size_t)r * S + c] = std::exp(-0.5 * (lx * lx + ly * ly) / (sigma * sigma));
+            }
+        double sum = 0.0;
+        for (double v : wts) sum += v;
+        e->weight_sum = sum;
+        CHECK(e->ref_arena.alloc(&d_weights, wts.size()));
+        HIP_CHECK(hipMemcpy(d_weights, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        e->weight_sum = (double)S * S;
+    }
+    // Upload the reference, build its pyramid and prepare every level.
+    hipStream_t st = e->prep_stream;
+    const Level& L0 = e->lv[0];
+    HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, image, (size_t)row_stride, L0.width, L0.height,
+                               hipMemcpyHostToDevice, st));
+    for (int k = 1; k < e->levels; ++k)
+        HIP_CHECK(launch_reduce(e->lv[k - 1].image, e->lv[k - 1].width, e->lv[k - 1].height, e->lv[k].image, st));
+    Arena scratch;
+    size_t px0 = (size_t)L0.width * L0.height;
+    double *G, *A, *B, *GX, *GY;
+    int rc = LDIC_OK;
+    if ((rc = scratch.alloc(&G, px0)) || (rc = scratch.alloc(&A, px0)) || (rc = scratch.alloc(&B, px0)) ||
+        (rc = scratch.alloc(&GX, px0)) || (rc = scratch.alloc(&GY, px0))) {
+        scratch.release();
+        return rc;
+    }
+    for (int k = 0; k < e->levels && rc == LDIC_OK; ++k) rc = prepare_level(e, k, G, A, B, GX, GY, d_weights, st);
+    hipError_t sync = hipStreamSynchronize(st);
+    scratch.release();
+    if (rc != LDIC_OK) return rc;
+    if (sync != hipSuccess) return set_error(LDIC_ERR_DEVICE, "reference preparation failed: %s", hipGetErrorString(sync));
+    if (out_valid) HIP_CHECK(hipMemcpy(out_valid, L0.valid, (size_t)L0.count(), hipMemcpyDeviceToHost));
+    CHECK(ensure_slots(e, 1));
+    e->ready = true;
+    return LDIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ldic_abi_version(void) { return LDIC_ABI_VERSION; }
+
+const char* ldic_last_error(void) { return g_error.c_str(); }
+
+int ldic_create(const ldic_options* options, ldic_engine** out_engine) {
+    if (!out_engine) return set_error(LDIC_ERR_ARGUMENT, "out_engine is NULL");
+    *out_engine = nullptr;
+    CHECK(validate_options(options));
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    if (options->device_ordinal < 0 || options->device_ordinal >= count)
+        return set_error(LDIC_ERR_ARGUMENT, "device_ordinal %d is not one of the %d HIP devices", options->device_ordinal,
+                         count);
+    ldic_engine* e = new ldic_engine();
+    e->opt = *options;
+    e->device = options->device_ordinal;
+    int rc = LDIC_OK;
+    auto fail = [&](int code) {
+        ldic_destroy(e);
+        return code;
+    };
+    hipError_t he = hipSetDevice(e->device);
+    if (he == hipSuccess) he = hipGetDeviceProperties(&e->prop, e->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->prep_stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "device set-up failed: %s", hipGetErrorString(he)));
+    float taps[31];
+    fir_taps(taps);
+    if ((rc = e->const_arena.alloc(&e->fir, 31)) != LDIC_OK) return fail(rc);
+    he = hipMemcpy(e->fir, taps, sizeof taps, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "FIR upload failed: %s", hipGetErrorString(he)));
+    gaussian_taps(e->gauss7);
+    e->solve_lds = solver_lds_bytes(options->subset_size, &e->tile_cap);
+    *out_engine = e;
+    return LDIC_OK;
+}
+
+void ldic_destroy(ldic_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    free_slots(e);
+    e->ref_arena.release();
+    e->const_arena.release();
+    if (e->prep_stream) (void)hipStreamDestroy(e->prep_stream);
+    delete e;
+}
+
+int ldic_device_name(ldic_engine* e, char* buffer, int32_t capacity) {
+    if (!e || !buffer || capacity <= 0) return set_error(LDIC_ERR_ARGUMENT, "invalid buffer");
+    std::snprintf(buffer, (size_t)capacity, "%s (%s, %d CUs)", e->prop.name, e->prop.gcnArchName,
+                  e->prop.multiProcessorCount);
+    return LDIC_OK;
+}
+
+int ldic_set_reference(ldic_engine* e, const uint8_t* image, int32_t width, int32_t height, int64_t row_stride,
+                       const ldic_level_config* configs, const ldic_level_roi* rois, uint8_t* out_reference_valid) {
+    if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
+    return set_reference_impl(e, image, width, height, row_stride, configs, rois, out_reference_valid);
+}
+
+int ldic_level_shape(ldic_engine* e, int32_t level, int32_t* width, int32_t* height, int32_t* gw, int32_t* gh) {
+    CHECK(check_ready(e));
+    if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
+    const Level& L = e->lv[level];
+    if (width) *width = L.width;
+    if (height) *height = L.height;
+    if (gw) *gw = L.gw;
+    if (gh) *gh = L.gh;
+    return LDIC_OK;
+}
+
+int ldic_set_level_config(ldic_engine* e, int32_t level, const ldic_level_config* c) {
+    CHECK(check_ready(e));
+    if (level < 0 || level >= e->levels || !c) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
+    CHECK(validate_config(c));
+    e->lv[level].config = *c;
+    return LDIC_OK;
+}
+
+int ldic_set_slots(ldic_engine* e, int32_t count) {
+    CHECK(check_ready(e));
+    if (count < 1 || count > 64) return set_error(LDIC_ERR_ARGUMENT, "slot count must be 1..64");
+    return ensure_slots(e, count);
+}
+
+int ldic_register(ldic_engine* e, const uint8_t* target, int64_t row_stride, float* out_state, ldic_timing* t) {
+    double wall = now_ms();
+    CHECK(check_ready(e));
+    if (!target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "target and out_state are required");
+    if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
+    HIP_CHECK(hipSetDevice(e->device));
+    Slot* s = e->slots[0];
+    CHECK(submit_frame(e, s, target, row_stride));
+    return finish(e, s, out_state, t, wall);
+}
+
+int ldic_submit(ldic_engine* e, int32_t slot, const uint8_t* target, int64_t row_stride) {
+    CHECK(check_slot(e, slot));
+    if (!target) return set_error(LDIC_ERR_ARGUMENT, "target is NULL");
+    if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
+    HIP_CHECK(hipSetDevice(e->device));
+    return submit_frame(e, e->slots[slot], target, row_stride);
+}
+
+int ldic_retire(ldic_engine* e, int32_t slot, float* out_state, ldic_timing* t) {
+    CHECK(check_slot(e, slot));
+    Slot* s = e->slots[slot];
+    if (!s->pending || s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds no submitted frame", slot);
+    HIP_CHECK(hipSetDevice(e->device));
+    return finish(e, s, out_state, t, s->submitted_at - s->upload_ms);
+}
+
+int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stride, const float* seed_state,
+                         const uint8_t* active, int32_t include_zncc, float* out_state, float* out_zncc,
+                         ldic_timing* t) {
+    double wall = now_ms();
+    CHECK(check_ready(e));
+    if (!target || !seed_state || !active || !out_state)
+        return set_error(LDIC_ERR_ARGUMENT, "target, seed_state, active and out_state are required");
+    if (include_zncc && (e->opt.motion_model != kAffine || !out_zncc))
+        return set_error(LDIC_ERR_ARGUMENT, "include_zncc requires the affine model and an output buffer");
+    HIP_CHECK(hipSetDevice(e->device));
+    Slot* s = e->slots[0];
+    if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
+    const Level& L = e->lv[0];
+    const int n = L.count(), stride = state_stride(e->opt.motion_model);
+    SlotLevel& sl = s->lv[0];
+    CHECK(copy_in(e, s, target, row_stride));
+    HIP_CHECK(hipMemcpyAsync(sl.seed, seed_state, sizeof(float) * (size_t)n * stride, hipMemcpyHostToDevice, s->stream));
+    std::vector<uint32_t> act((size_t)n);
+    for (int i = 0; i < n; ++i) act[i] = active[i] ? 1u : 0u;
+    HIP_CHECK(hipMemcpyAsync(sl.active, act.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, s->stream));
+    if (e->opt.interpolation == kBspline) HIP_CHECK(launch_prefilter(sl.target, L.width, L.height, e->fir, sl.coeff, s->stream));
+    if (s->ring_count == kEventRing) CHECK(drain_ring(s));
+    const int ring = (s->ring_first + s->ring_count) % kEventRing;
+    s->ring_count += 1;
+    HIP_CHECK(hipEventRecord(s->ev_solve0[ring], s->stream));
+    HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting,
+                           solve_args(e, 0, sl, sl.seed, sl.active, 0), s->stream));
+    HIP_CHECK(hipEventRecord(s->ev_solve1[ring], s->stream));
+    CHECK(launch_median_level(e, 0, sl, false, s->stream));
+    const float* out = level_output(e, sl);
+    if (include_zncc) {
+        ZnccArgs z{};
+        z.width = L.width;
+        z.height = L.height;
+        z.grid_w = L.gw;
+        z.grid_h = L.gh;
+        z.S = e->opt.subset_size;
+        z.D = e->opt.subset_stride;
+        z.interpolation = e->opt.interpolation;
+        z.reference = L.reference;
+        z.state = out;
+        z.coeff = sl.coeff;
+        z.target = sl.target;
+        z.out = s->zncc;
+        HIP_CHECK(launch_affine_zncc(z, s->stream));
+        HIP_CHECK(hipMemcpyAsync(s->pinned_zncc, s->zncc, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)n * stride, hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->pending = true;
+    s->resident = false;
+    CHECK(finish(e, s, out_state, t, wall));
+    if (include_zncc) {
+        for (int i = 0; i < n; ++i) out_zncc[i] = s->pinned_zncc[i] < -1.0f ? NAN : s->pinned_zncc[i];
+    }
+    return LDIC_OK;
+}
+
+int ldic_register_resident(ldic_engine* e, int32_t slot, const uint8_t* device_target, int64_t row_stride) {
+    CHECK(check_slot(e, slot));
+    if (!device_target) return set_error(LDIC_ERR_ARGUMENT, "device_target is NULL");
+    Slot* s = e->slots[slot];
+    if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    HIP_CHECK(hipSetDevice(e->device));
+    const Level& L = e->lv[0];
+    s->upload_ms = 0.0;
+    if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    HIP_CHECK(hipMemcpy2DAsync(s->lv[0].target, L.width, device_target, (size_t)row_stride, L.width, L.height,
+                               hipMemcpyDeviceToDevice, s->stream));
+    CHECK(enqueue_frame(e, s));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->pending = true;
+    s->resident = true;
+    s->submitted_at = now_ms();
+    return LDIC_OK;
+}
+
+int ldic_slot_wait(ldic_engine* e, int32_t slot, ldic_timing* t) {
+    CHECK(check_slot(e, slot));
+    Slot* s = e->slots[slot];
+    if (!s->pending) return LDIC_OK;
+    HIP_CHECK(hipSetDevice(e->device));
+    return finish(e, s, nullptr, t, s->submitted_at);
+}
+
+int ldic_slot_output(ldic_engine* e, int32_t slot, float* out_state) {
+    return ldic_export_level_state(e, slot, 0, out_state);
+}
+
+int ldic_export_level_state(ldic_engine* e, int32_t slot, int32_t level, float* out_state) {
+    CHECK(check_slot(e, slot));
+    if (level < 0 || level >= e->levels || !out_state) return set_error(LDIC_ERR_ARGUMENT, "invalid level or buffer");
+    Slot* s = e->slots[slot];
+    HIP_CHECK(hipSetDevice(e->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (s->pending) {
+        CHECK(drain_ring(s));
+        s->drained_solve_ms = 0.0;
+        s->drained_frames = 0;
+    }
+    s->pending = false;
+    size_t n = (size_t)e->lv[level].count() * state_stride(e->opt.motion_model);
+    HIP_CHECK(hipMemcpy(out_state, level_output(e, s->lv[level]), n * sizeof(float), hipMemcpyDeviceToHost));
+    return LDIC_OK;
+}
+
+int ldic_export_reference(ldic_engine* e, int32_t level, float* reference4, float* metadata, float* inverse) {
+    CHECK(check_ready(e));
+    if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
+    HIP_CHECK(hipSetDevice(e->device));
+    const Level& L = e->lv[level];
+    size_t n = (size_t)L.count();
+    int P = param_count(e->opt.motion_model);
+    if (reference4)
+        HIP_CHECK(hipMemcpy(reference4, L.reference, sizeof(float4) * (size_t)L.width * L.height, hipMemcpyDeviceToHost));
+    if (metadata)
+        HIP_CHECK(hipMemcpy(metadata, L.metadata, sizeof(float) * n * metadata_stride(e->opt.motion_model),
+                            hipMemcpyDeviceToHost));
+    if (inverse) HIP_CHECK(hipMemcpy(inverse, L.inverse, sizeof(float) * n * P * P, hipMemcpyDeviceToHost));
+    return LDIC_OK;
+}
+
+int ldic_export_coefficients(ldic_engine* e, int32_t slot, int32_t level, float* coefficients) {
+    CHECK(check_slot(e, slot));
+    if (level < 0 || level >= e->levels || !coefficients) return set_error(LDIC_ERR_ARGUMENT, "invalid level or buffer");
+    if (e->opt.interpolation != kBspline) return set_error(LDIC_ERR_STATE, "no coefficient plane in direct modes");
+    Slot* s = e->slots[slot];
+    HIP_CHECK(hipSetDevice(e->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    const Level& L = e->lv[level];
+    HIP_CHECK(hipMemcpy(coefficients, s->lv[level].coeff, sizeof(float) * (size_t)L.width * L.height,
+                        hipMemcpyDeviceToHost));
+    return LDIC_OK;
+}
+
+int ldic_op_reduce(const uint8_t* src, int32_t w, int32_t h, uint8_t* dst) {
+    if (!src || !dst || w < 2 || h < 2) return set_error(LDIC_ERR_ARGUMENT, "invalid reduction arguments");
+    uint8_t *ds = nullptr, *dd = nullptr;
+    Arena a;
+    int rc = a.alloc(&ds, (size_t)w * h);
+    if (rc == LDIC_OK) rc = a.alloc(&dd, (size_t)(w / 2) * (h / 2));
+    hipError_t he = hipSuccess;
+    if (rc == LDIC_OK) {
+        he = hipMemcpy(ds, src, (size_t)w * h, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = launch_reduce(ds, w, h, dd, nullptr);
+        if (he == hipSuccess) he = hipMemcpy(dst, dd, (size_t)(w / 2) * (h / 2), hipMemcpyDeviceToHost);
+    }
+    a.release();
+    if (rc != LDIC_OK) return rc;
+    if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "reduction failed: %s", hipGetErrorString(he));
+    return LDIC_OK;
+}
+
+int ldic_op_prefilter(const uint8_t* src, int32_t w, int32_t h, float* coeff) {
+    if (!src || !coeff || w < 1 || h < 1) return set_error(LDIC_ERR_ARGUMENT, "invalid prefilter arguments");
+    float taps[31];
+    fir_taps(taps);
+    uint8_t* ds = nullptr;
+    float *dc = nullptr, *df = nullptr;
+    Arena a;
+    int rc = a.alloc(&ds, (size_t)w * h);
+    if (rc == LDIC_OK) rc = a.alloc(&dc, (size_t)w * h);
+    if (rc == LDIC_OK) rc = a.alloc(&df, 31);
+    hipError_t he = hipSuccess;
+    if (rc == LDIC_OK) {
+        he = hipMemcpy(ds, src, (size_t)w * h, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(df, taps, sizeof taps, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = launch_prefilter(ds, w, h, df, dc, nullptr);
+        if (he == hipSuccess) he = hipMemcpy(coeff, dc, sizeof(float) * (size_t)w * h, hipMemcpyDeviceToHost);
+    }
+    a.release();
+    if (rc != LDIC_OK) return rc;
+    if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "prefilter failed: %s", hipGetErrorString(he));
+    return LDIC_OK;
+}
+
+}  // extern "C"

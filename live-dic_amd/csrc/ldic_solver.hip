@@ -1,0 +1,627 @@
+// Inverse-compositional Gauss-Newton subset solver for gfx950.
+//
+// One wave64 owns one subset (four independent waves per workgroup, no block
+// barriers).  The reference tile (f, gx, gy) of the subset is loaded once into
+// VGPRs; the target footprint around the current warp is staged in the wave's
+// LDS slice and re-staged only when the warp leaves it, so the per-iteration
+// B-spline gathers (16 per pixel) hit LDS instead of L2.  Sums are reduced with
+// DPP inside the wave and every lane evaluates the (wave-uniform) increment,
+// composition and convergence test, so no lane waits on another.
+//
+// Equations: shaders/icgn.wgsl:216-958 (sampling 230-309, increments 321-384,
+// compositions 386-549, change metrics 551-593, solve loop 595-958).
+#include "ldic_device.h"
+#include "ldic_kernels.h"
+
+namespace ldic {
+
+constexpr int kWavesPerBlock = 4;
+constexpr float kSixth = 1.0f / 6.0f;
+
+__device__ __forceinline__ void bspline_weights(float t, float w[4]) {
+    float t2 = t * t, t3 = t2 * t, inv = 1.0f - t;
+    w[0] = inv * inv * inv * kSixth;
+    w[1] = (3.0f * t3 - 6.0f * t2 + 4.0f) * kSixth;
+    w[2] = (-3.0f * t3 + 3.0f * t2 + 3.0f * t + 1.0f) * kSixth;
+    w[3] = t3 * kSixth;
+}
+
+__device__ __forceinline__ void keys_weights(float t, float w[4]) {
+    float t2 = t * t, t3 = t2 * t;
+    w[0] = -0.5f * t + t2 - 0.5f * t3;
+    w[1] = 1.0f - 2.5f * t2 + 1.5f * t3;
+    w[2] = 0.5f * t + 2.0f * t2 - 1.5f * t3;
+    w[3] = -0.5f * t2 + 0.5f * t3;
+}
+
+// Target footprint held in LDS: box origin (x0, y0) and pitch.
+struct Tile {
+    int x0, y0, w, h;
+    bool valid;
+};
+
+template <int INTERP>
+__device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, float px, float py) {
+    float fx = floorf(px), fy = floorf(py);
+    int ix = (int)fx, iy = (int)fy;
+    float tx = px - fx, ty = py - fy;
+    if (INTERP == kLinear) {
+        const float* b = tile + (iy - t.y0) * t.w + (ix - t.x0);
+        float top = b[0] * (1.0f - tx) + b[1] * tx;
+        float bottom = b[t.w] * (1.0f - tx) + b[t.w + 1] * tx;
+        return top * (1.0f - ty) + bottom * ty;
+    }
+    float wx[4], wy[4];
+    if (INTERP == kKeys) {
+        keys_weights(tx, wx);
+        keys_weights(ty, wy);
+    } else {
+        bspline_weights(tx, wx);
+        bspline_weights(ty, wy);
+    }
+    const float* b = tile + (iy - 1 - t.y0) * t.w + (ix - 1 - t.x0);
+    float v = 0.0f;
+#pragma unroll
+    for (int yy = 0; yy < 4; ++yy) {
+        const float* r = b + yy * t.w;
+        float row = 0.0f;
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) row += wx[xx] * r[xx];
+        v += wy[yy] * row;
+    }
+    return v;
+}
+
+// Direct gathers with clamped indices, for footprints that do not fit LDS.
+template <int INTERP>
+__device__ __forceinline__ float global_sample(const SolveArgs& a, float px, float py) {
+    float fx = floorf(px), fy = floorf(py);
+    int ix = (int)fx, iy = (int)fy;
+    float tx = px - fx, ty = py - fy;
+    const int W = a.width, H = a.height;
+    auto at = [&](int x, int y) -> float {
+        x = clampi(x, 0, W - 1);
+        y = clampi(y, 0, H - 1);
+        return INTERP == kBspline ? a.coeff[(size_t)y * W + x] : (float)a.target[(size_t)y * W + x];
+    };
+    if (INTERP == kLinear) {
+        float top = at(ix, iy) * (1.0f - tx) + at(ix + 1, iy) * tx;
+        float bottom = at(ix, iy + 1) * (1.0f - tx) + at(ix + 1, iy + 1) * tx;
+        return top * (1.0f - ty) + bottom * ty;
+    }
+    float wx[4], wy[4];
+    if (INTERP == kKeys) {
+        keys_weights(tx, wx);
+        keys_weights(ty, wy);
+    } else {
+        bspline_weights(tx, wx);
+        bspline_weights(ty, wy);
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int yy = 0; yy < 4; ++yy) {
+        float row = 0.0f;
+#pragma unroll
+        for (int xx = 0; xx < 4; ++xx) row += wx[xx] * at(ix + xx - 1, iy + yy - 1);
+        v += wy[yy] * row;
+    }
+    return v;
+}
+
+// Warp state, wave-uniform.
+struct Warp {
+    float T[2];
+    float L[4];
+    float C[6];
+};
+
+template <int MODEL>
+__device__ __forceinline__ void displacement(const Warp& q, float lx, float ly, float& dx, float& dy) {
+    dx = q.T[0];
+    dy = q.T[1];
+    if (MODEL != kTranslation) {
+        dx = dx + (q.L[0] * lx + q.L[1] * ly);
+        dy = dy + (q.L[2] * lx + q.L[3] * ly);
+        if (MODEL == kQuadratic) {
+            dx += 0.5f * q.C[0] * lx * lx + q.C[1] * lx * ly + 0.5f * q.C[2] * ly * ly;
+            dy += 0.5f * q.C[3] * lx * lx + q.C[4] * lx * ly + 0.5f * q.C[5] * ly * ly;
+        }
+    }
+}
+
+// Corner (or 3x3 for quadratic) bounds test, icgn.wgsl:648-685.
+template <int MODEL>
+__device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, float he, float W, float H) {
+    if (MODEL == kTranslation) {
+        float minx = cx - 0.5f + -he + q.T[0], miny = cy - 0.5f + -he + q.T[1];
+        float maxx = cx - 0.5f + he + q.T[0], maxy = cy - 0.5f + he + q.T[1];
+        return minx >= 2.0f && miny >= 2.0f && maxx <= W - 3.0f && maxy <= H - 3.0f;
+    }
+    bool inside = true;
+    const int count = MODEL == kQuadratic ? 9 : 4;
+#pragma unroll
+    for (int p = 0; p < count; ++p) {
+        float lx, ly;
+        if (MODEL == kQuadratic) {
+            lx = (float)(p % 3 - 1) * he;
+            ly = (float)(p / 3 - 1) * he;
+        } else {
+            lx = (p == 1 || p == 3) ? he : -he;
+            ly = p >= 2 ? he : -he;
+        }
+        float dx, dy;
+        displacement<MODEL>(q, lx, ly, dx, dy);
+        float px = cx - 0.5f + lx + dx, py = cy - 0.5f + ly + dy;
+        inside = inside && px >= 2.0f && py >= 2.0f && px <= W - 3.0f && py <= H - 3.0f;
+    }
+    return inside;
+}
+
+// icgn.wgsl:437-513; false when a pivot falls below 1e-8.
+__device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12]) {
+    float a[6][12];
+    {
+        float u = d[0], ux = d[1], uy = d[2], v = d[3], vx = d[4], vy = d[5];
+        float uxx = d[6], uxy = d[7], uyy = d[8], vxx = d[9], vxy = d[10], vyy = d[11];
+        float m[6][6] = {
+            {1.0f + 2.0f * ux + ux * ux + u * uxx, 2.0f * u * uxy + 2.0f * (1.0f + ux) * uy, uy * uy + u * uyy,
+             2.0f * u * (1.0f + ux), 2.0f * u * uy, u * u},
+            {0.5f * (v * uxx + 2.0f * (1.0f + ux) * vx + u * vxx), 1.0f + uy * vx + ux * vy + v * uxy + u * vxy + vy + ux,
+             0.5f * (v * uyy + 2.0f * uy * (1.0f + vy) + u * vyy), v + v * ux + u * vx, u + v * uy + u * vy, u * v},
+            {vx * vx + v * vxx, 2.0f * v * vxy + 2.0f * vx * (1.0f + vy), 1.0f + 2.0f * vy + vy * vy + v * vyy,
+             2.0f * v * vx, 2.0f * v * (1.0f + vy), v * v},
+            {0.5f * uxx, uxy, 0.5f * uyy, 1.0f + ux, uy, u},
+            {0.5f * vxx, vxy, 0.5f * vyy, vx, 1.0f + vy, v},
+            {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f}};
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                a[r][c] = m[r][c];
+                a[r][6 + c] = r == c ? 1.0f : 0.0f;
+            }
+    }
+#pragma unroll
+    for (int col = 0; col < 6; ++col) {
+        int pr = col;
+        float ps = fabsf(a[col][col]);
+#pragma unroll
+        for (int cand = col + 1; cand < 6; ++cand) {
+            float cs = fabsf(a[cand][col]);
+            if (cs > ps) {
+                pr = cand;
+                ps = cs;
+            }
+        }
+        if (ps < 1e-8f) return false;
+        // Branch-free row exchange keeps the matrix in registers.
+#pragma unroll
+        for (int r = col + 1; r < 6; ++r) {
+            bool sw = r == pr;
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {
+                float x = a[col][e], y = a[r][e];
+                a[col][e] = sw ? y : x;
+                a[r][e] = sw ? x : y;
+            }
+        }
+        float pivot = a[col][col];
+#pragma unroll
+        for (int e = 0; e < 12; ++e) a[col][e] /= pivot;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            if (r == col) continue;
+            float f = a[r][col];
+#pragma unroll
+            for (int e = 0; e < 12; ++e) a[r][e] -= f * a[col][e];
+        }
+    }
+    float cu[6] = {0.5f * q.C[0], q.C[1], 0.5f * q.C[2], 1.0f + q.L[0], q.L[1], q.T[0]};
+    float cv[6] = {0.5f * q.C[3], q.C[4], 0.5f * q.C[5], q.L[2], 1.0f + q.L[3], q.T[1]};
+    float uu[6], uv[6];
+#pragma unroll
+    for (int col = 0; col < 6; ++col) {
+        float su = 0.0f, sv = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            su += cu[e] * a[e][6 + col];
+            sv += cv[e] * a[e][6 + col];
+        }
+        uu[col] = su;
+        uv[col] = sv;
+    }
+    q.C[0] = 2.0f * uu[0];
+    q.C[1] = uu[1];
+    q.C[2] = 2.0f * uu[2];
+    q.C[3] = 2.0f * uv[0];
+    q.C[4] = uv[1];
+    q.C[5] = 2.0f * uv[2];
+    q.T[0] = uu[5];
+    q.T[1] = uv[5];
+    q.L[0] = uu[3] - 1.0f;
+    q.L[1] = uu[4];
+    q.L[2] = uv[3];
+    q.L[3] = uv[4] - 1.0f;
+    return true;
+}
+
+// icgn.wgsl:515-549
+template <int MODEL>
+__device__ __forceinline__ void compose_standard(Warp& q, const float d[6], float det) {
+    if (MODEL == kTranslation) {
+        q.T[0] = q.T[0] - d[0];
+        q.T[1] = q.T[1] - d[3];
+        return;
+    }
+    float i11 = (1.0f + d[5]) / det, i12 = -d[2] / det, i21 = -d[4] / det, i22 = (1.0f + d[1]) / det;
+    float w11 = 1.0f + q.L[0], w12 = q.L[1], w21 = q.L[2], w22 = 1.0f + q.L[3];
+    float itx = -(i11 * d[0] + i12 * d[3]);
+    float ity = -(i21 * d[0] + i22 * d[3]);
+    q.T[0] = q.T[0] + (w11 * itx + w12 * ity);
+    q.T[1] = q.T[1] + (w21 * itx + w22 * ity);
+    float u11 = w11 * i11 + w12 * i21, u12 = w11 * i12 + w12 * i22;
+    float u21 = w21 * i11 + w22 * i21, u22 = w21 * i12 + w22 * i22;
+    q.L[0] = u11 - 1.0f;
+    q.L[1] = u12;
+    q.L[2] = u21;
+    q.L[3] = u22 - 1.0f;
+}
+
+__device__ __forceinline__ float hypot2(float a, float b) { return sqrtf(a * a + b * b); }
+
+template <int MODEL>
+__device__ __forceinline__ float increment_change(const float* d, float he) {
+    if (MODEL == kTranslation) return hypot2(d[0], d[3]);
+    float m = 0.0f;
+    if (MODEL == kQuadratic) {
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            float x = (float)(p % 3 - 1) * he, y = (float)(p / 3 - 1) * he;
+            m = fmaxf(m, hypot2(d[0] + d[1] * x + d[2] * y + 0.5f * d[6] * x * x + d[7] * x * y + 0.5f * d[8] * y * y,
+                                d[3] + d[4] * x + d[5] * y + 0.5f * d[9] * x * x + d[10] * x * y + 0.5f * d[11] * y * y));
+        }
+        return m;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float x = (c == 1 || c == 3) ? he : -he, y = c >= 2 ? he : -he;
+        m = fmaxf(m, hypot2(d[0] + d[1] * x + d[2] * y, d[3] + d[4] * x + d[5] * y));
+    }
+    return m;
+}
+
+// Stage the target footprint [x0, x0+w) x [y0, y0+h) of the wave's LDS slice.
+template <int INTERP>
+__device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, const Tile& t, int lane) {
+    const int n = t.w * t.h;
+    const float inv_w = 1.0f / (float)t.w;
+    for (int i = lane; i < n; i += kWave) {
+        int r = (int)(((float)i + 0.5f) * inv_w);
+        int c = i - r * t.w;
+        size_t g = (size_t)(t.y0 + r) * a.width + (t.x0 + c);
+        tile[i] = INTERP == kBspline ? a.coeff[g] : (float)a.target[g];
+    }
+    wave_lds_sync();
+}
+
+// PPL > 0: the reference tile of each lane lives in registers (PPL pixels per
+// lane, pixel p = lane + 64 k).  PPL == 0: any subset width, reference values
+// are re-read from the plane every iteration.
+template <int MODEL, int INTERP, int PPL, bool WEIGHTED>
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+    extern __shared__ float lds[];
+    constexpr int P = MODEL == kQuadratic ? 12 : 6;
+    constexpr int M = MODEL == kQuadratic ? 28 : 16;
+    constexpr int STRIDE = MODEL == kQuadratic ? 16 : 8;
+    constexpr int NP = PPL > 0 ? PPL : 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int subset = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    const int N = a.grid_w * a.grid_h;
+    if (subset >= N) return;
+    float* tile = lds + wave * a.tile_cap;
+    const float* meta = a.metadata + (size_t)subset * M;
+    const float* inv = a.inverse + (size_t)subset * (P * P);
+
+    const bool enabled = a.active ? a.active[subset] != 0u : true;
+    if (!enabled && a.preserve_disabled) return;
+
+    Warp q;
+    q.T[0] = q.T[1] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q.L[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.C[k] = 0.0f;
+    if (a.seed) {
+        const float* sd = a.seed + (size_t)subset * STRIDE;
+        q.T[0] = sd[0];
+        q.T[1] = sd[3];
+        if (MODEL != kTranslation) {
+            q.L[0] = sd[1];
+            q.L[1] = sd[2];
+            q.L[2] = sd[4];
+            q.L[3] = sd[5];
+        }
+        if (MODEL == kQuadratic)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) q.C[k] = sd[6 + k];
+    }
+    const float meta1 = meta[1];
+    float status = (enabled && meta1 >= 0.5f) ? 0.0f : (a.roi_enabled ? fminf(-1.0f, meta1) : -1.0f);
+    float iters = 0.0f;
+
+    const int S = a.S, D = a.D, SS = S * S;
+    const int gx = subset % a.grid_w, gy = subset / a.grid_w;
+    const float he = 0.5f * ((float)S - 1.0f);
+    const float cx = 0.5f * (float)S + (float)gx * (float)D;
+    const float cy = 0.5f * (float)S + (float)gy * (float)D;
+    const float ref_mean = (D != S) ? meta[MODEL == kQuadratic ? 26 : 14] : 0.0f;
+    const float wsum = WEIGHTED ? meta[MODEL == kQuadratic ? 27 : 15] : (float)SS;
+    const float Wf = (float)a.width, Hf = (float)a.height;
+    const int ox = gx * D, oy = gy * D;
+
+    // Register-resident reference tile.
+    float rf[NP], rgx[NP], rgy[NP], plx[NP], ply[NP], pw[WEIGHTED ? NP : 1];
+    if (PPL > 0 && status == 0.0f) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            int p = lane + kWave * k;
+            if (p < SS) {
+                int row = p / S, col = p - row * S;
+                float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
+                rf[k] = r.x - ref_mean;
+                rgx[k] = r.y;
+                rgy[k] = r.z;
+                plx[k] = (float)col - he;
+                ply[k] = (float)row - he;
+                if (WEIGHTED) pw[k] = expf(-0.5f * (plx[k] * plx[k] + ply[k] * ply[k]) * a.inv_var);
+            } else {
+                rf[k] = rgx[k] = rgy[k] = plx[k] = ply[k] = 0.0f;
+                if (WEIGHTED) pw[k] = 0.0f;
+            }
+        }
+    }
+
+    Tile t;
+    t.valid = false;
+    t.x0 = t.y0 = t.w = t.h = 0;
+
+    for (int it = 0; it < 30; ++it) {
+        bool inside = warp_inside<MODEL>(q, cx, cy, he, Wf, Hf);
+        if (!inside && (!a.roi_enabled || status > -2.5f)) status = -1.0f;
+        if (it >= a.iterations || fabsf(status) > 0.5f) break;
+
+        // Conservative footprint of the warped subset (+1 px rounding slack).
+        float ex = (fabsf(q.L[0]) + fabsf(q.L[1])) * he, ey = (fabsf(q.L[2]) + fabsf(q.L[3])) * he;
+        if (MODEL == kQuadratic) {
+            ex += (0.5f * fabsf(q.C[0]) + fabsf(q.C[1]) + 0.5f * fabsf(q.C[2])) * he * he;
+            ey += (0.5f * fabsf(q.C[3]) + fabsf(q.C[4]) + 0.5f * fabsf(q.C[5])) * he * he;
+        }
+        const int lo = INTERP == kLinear ? 1 : 2, hi = INTERP == kLinear ? 2 : 3;
+        int rx0 = (int)floorf(cx - 0.5f - he + q.T[0] - ex) - lo;
+        int rx1 = (int)floorf(cx - 0.5f + he + q.T[0] + ex) + hi;
+        int ry0 = (int)floorf(cy - 0.5f - he + q.T[1] - ey) - lo;
+        int ry1 = (int)floorf(cy - 0.5f + he + q.T[1] + ey) + hi;
+        bool in_image = rx0 >= 0 && ry0 >= 0 && rx1 <= a.width - 1 && ry1 <= a.height - 1;
+        bool use_tile = false;
+        if (in_image) {
+            if (t.valid && rx0 >= t.x0 && ry0 >= t.y0 && rx1 < t.x0 + t.w && ry1 < t.y0 + t.h) {
+                use_tile = true;
+            } else {
+                const int m = 2;
+                Tile nt;
+                nt.x0 = max(0, rx0 - m);
+                nt.y0 = max(0, ry0 - m);
+                nt.w = min(a.width - 1, rx1 + m) - nt.x0 + 1;
+                nt.h = min(a.height - 1, ry1 + m) - nt.y0 + 1;
+                if (nt.w * nt.h > a.tile_cap) {
+                    nt.x0 = rx0;
+                    nt.y0 = ry0;
+                    nt.w = rx1 - rx0 + 1;
+                    nt.h = ry1 - ry0 + 1;
+                }
+                if (nt.w * nt.h <= a.tile_cap) {
+                    nt.valid = true;
+                    t = nt;
+                    stage_tile<INTERP>(a, tile, t, lane);
+                    use_tile = true;
+                }
+            }
+        }
+
+        float sc = 0.0f, sc2 = 0.0f, src = 0.0f;
+        float sj[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float sq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        auto accumulate = [&](float lx, float ly, float f, float gxv, float gyv, float iw) {
+            float dx, dy;
+            displacement<MODEL>(q, lx, ly, dx, dy);
+            float px = cx - 0.5f + lx + dx, py = cy - 0.5f + ly + dy;
+            float cur = use_tile ? tile_sample<INTERP>(tile, t, px, py) : global_sample<INTERP>(a, px, py);
+            float wc = iw * cur;
+            float gxg = gxv * wc, gyg = gyv * wc;
+            sc += wc;
+            sc2 += wc * cur;
+            src += iw * f * cur;
+            sj[0] += gxg;
+            sj[3] += gyg;
+            if (MODEL != kTranslation) {
+                sj[1] += gxg * lx;
+                sj[2] += gxg * ly;
+                sj[4] += gyg * lx;
+                sj[5] += gyg * ly;
+                if (MODEL == kQuadratic) {
+                    float hx2 = 0.5f * lx * lx, xy = lx * ly, hy2 = 0.5f * ly * ly;
+                    sq[0] += gxg * hx2;
+                    sq[1] += gxg * xy;
+                    sq[2] += gxg * hy2;
+                    sq[3] += gyg * hx2;
+                    sq[4] += gyg * xy;
+                    sq[5] += gyg * hy2;
+                }
+            }
+        };
+        if (PPL > 0) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                if (lane + kWave * k < SS)
+                    accumulate(plx[k], ply[k], rf[k], rgx[k], rgy[k], WEIGHTED ? pw[k] : 1.0f);
+            }
+        } else {
+            for (int p = lane; p < SS; p += kWave) {
+                int row = p / S, col = p - row * S;
+                float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
+                float lx = (float)col - he, ly = (float)row - he;
+                float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
+                accumulate(lx, ly, r.x - ref_mean, r.y, r.z, iw);
+            }
+        }
+
+        sc = wave_sum(sc);
+        sc2 = wave_sum(sc2);
+        src = wave_sum(src);
+        sj[0] = wave_sum(sj[0]);
+        sj[3] = wave_sum(sj[3]);
+        if (MODEL != kTranslation) {
+            sj[1] = wave_sum(sj[1]);
+            sj[2] = wave_sum(sj[2]);
+            sj[4] = wave_sum(sj[4]);
+            sj[5] = wave_sum(sj[5]);
+        }
+        if (MODEL == kQuadratic)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sq[k] = wave_sum(sq[k]);
+
+        const float mean = sc / wsum;
+        const float norm = sqrtf(fmaxf(0.0f, sc2 - sc * mean));
+        const float ref_norm = meta[0];
+        iters = (float)(it + 1);
+        if (norm < 1e-9f || ref_norm < 1e-9f) {
+            status = -2.0f;
+            continue;
+        }
+        const float ratio = ref_norm / norm;
+        const float zncc = src / (ref_norm * norm);
+        float change;
+        if (MODEL == kQuadratic) {
+            float rj[12] = {sj[0], sj[1], sj[2], sj[3], sj[4], sj[5], sq[0], sq[1], sq[2], sq[3], sq[4], sq[5]};
+            float res[12], d[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) res[k] = meta[2 + k] - ratio * (rj[k] - mean * meta[14 + k]);
+#pragma unroll
+            for (int r = 0; r < 12; ++r) {
+                float v = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) v += inv[12 * r + k] * res[k];
+                d[r] = -v;
+            }
+            if (!compose_quadratic(q, d)) {
+                status = -1.0f;
+                continue;
+            }
+            change = increment_change<MODEL>(d, he);
+        } else {
+            float d[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (MODEL == kTranslation) {
+                float rx = meta[2] - ratio * (sj[0] - mean * meta[8]);
+                float ry = meta[5] - ratio * (sj[3] - mean * meta[11]);
+                d[0] = -(inv[0] * rx + inv[3] * ry);
+                d[3] = -(inv[18] * rx + inv[21] * ry);
+            } else {
+                float res[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) res[k] = meta[2 + k] - ratio * (sj[k] - mean * meta[8 + k]);
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    float v = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v += inv[6 * r + k] * res[k];
+                    d[r] = -v;
+                }
+            }
+            float det = MODEL == kTranslation ? 1.0f : (1.0f + d[1]) * (1.0f + d[5]) - d[2] * d[4];
+            if (fabsf(det) < 1e-8f) {
+                status = -1.0f;
+                continue;
+            }
+            compose_standard<MODEL>(q, d, det);
+            change = increment_change<MODEL>(d, he);
+        }
+        status = change >= a.tolerance ? 0.0f : (zncc >= a.zncc_min ? 1.0f : -2.0f);
+    }
+
+    if (lane == 0) {
+        float4* out = reinterpret_cast<float4*>(a.state + (size_t)subset * STRIDE);
+        out[0] = make_float4(q.T[0], q.L[0], q.L[1], q.T[1]);
+        if (MODEL == kQuadratic) {
+            out[1] = make_float4(q.L[2], q.L[3], q.C[0], q.C[1]);
+            out[2] = make_float4(q.C[2], q.C[3], q.C[4], q.C[5]);
+            out[3] = make_float4(status, iters, 0.0f, 0.0f);
+        } else {
+            out[1] = make_float4(q.L[2], q.L[3], status, iters);
+        }
+    }
+}
+
+int solver_pixels_per_lane(int S) {
+    int need = (S * S + kWave - 1) / kWave;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    if (need <= 16) return 16;
+    if (need <= 27) return 27;
+    return 0;
+}
+
+size_t solver_lds_bytes(int S, int* tile_cap) {
+    // Footprint of a moderately strained subset plus the interpolation halo
+    // and the re-staging margin; larger footprints fall back to L2 gathers.
+    int side = S + 12;
+    int cap = side * side;
+    if (cap > 4096) cap = 4096;  // 64 KB of LDS per four-wave block
+    *tile_cap = cap;
+    return (size_t)cap * sizeof(float) * kWavesPerBlock;
+}
+
+template <int MODEL, int INTERP, bool WEIGHTED>
+static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
+    int N = a.grid_w * a.grid_h;
+    if (N <= 0) return hipSuccess;
+    int cap = 0;
+    size_t lds = solver_lds_bytes(a.S, &cap);
+    if (cap != a.tile_cap) return hipErrorInvalidValue;
+    dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
+    switch (ppl) {
+        case 4: k_solve<MODEL, INTERP, 4, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+        case 8: k_solve<MODEL, INTERP, 8, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+        case 16: k_solve<MODEL, INTERP, 16, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+        case 27: k_solve<MODEL, INTERP, 27, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+        default: k_solve<MODEL, INTERP, 0, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+template <int MODEL, int INTERP>
+static hipError_t launch_w(int weighted, const SolveArgs& a, hipStream_t s) {
+    int ppl = solver_pixels_per_lane(a.S);
+    return weighted ? launch_ppl<MODEL, INTERP, true>(ppl, a, s) : launch_ppl<MODEL, INTERP, false>(ppl, a, s);
+}
+
+template <int MODEL>
+static hipError_t launch_i(int interp, int weighted, const SolveArgs& a, hipStream_t s) {
+    switch (interp) {
+        case kBspline: return launch_w<MODEL, kBspline>(weighted, a, s);
+        case kLinear: return launch_w<MODEL, kLinear>(weighted, a, s);
+        case kKeys: return launch_w<MODEL, kKeys>(weighted, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a, hipStream_t s) {
+    switch (model) {
+        case kAffine: return launch_i<kAffine>(interp, weighted, a, s);
+        case kTranslation: return launch_i<kTranslation>(interp, weighted, a, s);
+        case kQuadratic: return launch_i<kQuadratic>(interp, weighted, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ldic

@@ -1,0 +1,180 @@
+"""ctypes binding of liblivedic.so (C ABI declared in include/livedic.h).
+
+The library is built in-tree by ``live-dic_amd/build.py``.  There is no
+fallback: a missing library or device is an ``ICGNError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+from .engine import ICGNError
+
+LIBRARY_PATH = Path(os.environ.get("LIVEDIC_LIBRARY", Path(__file__).resolve().parent / "liblivedic.so"))
+MAX_LEVELS = 4
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("motion_model", ctypes.c_int32),
+        ("interpolation", ctypes.c_int32),
+        ("subset_size", ctypes.c_int32),
+        ("subset_stride", ctypes.c_int32),
+        ("spatial_weighting", ctypes.c_int32),
+        ("gaussian_sigma", ctypes.c_float),
+        ("outlier_policy", ctypes.c_int32),
+        ("median_filter_size", ctypes.c_int32),
+        ("median_filter_scope", ctypes.c_int32),
+        ("pyramid_levels", ctypes.c_int32),
+        ("device_ordinal", ctypes.c_int32),
+    ]
+
+
+class LevelConfig(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("tolerance", ctypes.c_float), ("zncc_min", ctypes.c_float)]
+
+
+class LevelRoi(ctypes.Structure):
+    _fields_ = [
+        ("selected", ctypes.c_void_p),
+        ("neighbour_words", ctypes.c_void_p),
+        ("seed_links", ctypes.c_void_p),
+    ]
+
+
+class Field(ctypes.Structure):
+    _fields_ = [
+        ("enabled", ctypes.c_int32),
+        ("origin_x", ctypes.c_double),
+        ("origin_y", ctypes.c_double),
+        ("poly", ctypes.c_double * 12),
+        ("wave", (ctypes.c_double * 4) * 2),
+    ]
+
+
+class Speckle(ctypes.Structure):
+    _fields_ = [
+        ("device_blobs", ctypes.c_void_p),
+        ("device_cell_start", ctypes.c_void_p),
+        ("cells_x", ctypes.c_int32),
+        ("cells_y", ctypes.c_int32),
+        ("cell", ctypes.c_int32),
+        ("cell_origin", ctypes.c_float),
+        ("sigma", ctypes.c_float),
+        ("background", ctypes.c_float),
+    ]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("upload_ms", ctypes.c_double),
+        ("gpu_ms", ctypes.c_double),
+        ("solve_ms", ctypes.c_double),
+        ("readback_ms", ctypes.c_double),
+        ("wall_ms", ctypes.c_double),
+        ("frames", ctypes.c_int32),
+    ]
+
+
+# Every symbol include/livedic.h declares, with its argument types.
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+SIGNATURES = {
+    "ldic_abi_version": (ctypes.c_int, []),
+    "ldic_last_error": (ctypes.c_char_p, []),
+    "ldic_create": (ctypes.c_int, [ctypes.POINTER(Options), ctypes.POINTER(_P)]),
+    "ldic_destroy": (None, [_P]),
+    "ldic_device_name": (ctypes.c_int, [_P, ctypes.c_char_p, _I32]),
+    "ldic_set_reference": (ctypes.c_int, [_P, _P, _I32, _I32, _I64, _P, _P, _P]),
+    "ldic_level_shape": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
+    "ldic_set_level_config": (ctypes.c_int, [_P, _I32, ctypes.POINTER(LevelConfig)]),
+    "ldic_register": (ctypes.c_int, [_P, _P, _I64, _P, ctypes.POINTER(Timing)]),
+    "ldic_set_slots": (ctypes.c_int, [_P, _I32]),
+    "ldic_submit": (ctypes.c_int, [_P, _I32, _P, _I64]),
+    "ldic_retire": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(Timing)]),
+    "ldic_register_seeded": (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _P, _P, ctypes.POINTER(Timing)]),
+    "ldic_register_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
+    "ldic_slot_wait": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Timing)]),
+    "ldic_slot_output": (ctypes.c_int, [_P, _I32, _P]),
+    "ldic_export_reference": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "ldic_export_coefficients": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_export_level_state": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_op_reduce": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_op_prefilter": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_render_speckle": (ctypes.c_int, [ctypes.POINTER(Speckle), ctypes.POINTER(Field), _I32, _I32,
+                                           ctypes.c_float, ctypes.c_uint64, _P, _P, _P]),
+    "ldic_device_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "ldic_device_free": (ctypes.c_int, [_P]),
+    "ldic_copy_to_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "ldic_copy_to_host": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "ldic_device_synchronize": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load liblivedic.so once and declare its signatures."""
+    global _lib
+    if _lib is None:
+        if not LIBRARY_PATH.exists():
+            raise ICGNError(
+                f"{LIBRARY_PATH} is missing; build it with `python live-dic_amd/build.py` (hipcc, gfx950)"
+            )
+        try:
+            lib = ctypes.CDLL(str(LIBRARY_PATH))
+        except OSError as error:
+            raise ICGNError(f"cannot load {LIBRARY_PATH}: {error}") from error
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = lib
+    return _lib
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (freed on close or collection)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(library().ldic_device_alloc(self.nbytes, ctypes.byref(self.ptr)))
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def upload(self, array) -> None:
+        if array.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        check(library().ldic_copy_to_device(self.ptr, array.ctypes.data, array.nbytes))
+
+    def download(self, array) -> None:
+        if array.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        check(library().ldic_copy_to_host(array.ctypes.data, self.ptr, array.nbytes))
+
+    def close(self) -> None:
+        if self.ptr.value:
+            library().ldic_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def check(code: int) -> None:
+    if code != 0:
+        message = library().ldic_last_error()
+        raise ICGNError((message or b"").decode("utf-8", "replace") or f"native error {code}")
+
+
+def ptr(array) -> int | None:
+    return None if array is None else array.ctypes.data

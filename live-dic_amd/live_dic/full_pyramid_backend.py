@@ -1,0 +1,162 @@
+"""Coarse-to-fine engine (reference: live_dic/full_pyramid_backend.py).
+
+Each target is uploaded once; its pyramid, per-level prefilters, solves,
+outlier passes and seed transfers run as one queued sequence on a HIP stream,
+and only the finest field is read back.  ``register_stream`` keeps
+``pipeline_depth`` slots in flight and returns results in input order.
+"""
+
+from __future__ import annotations
+
+import operator
+import time
+from collections.abc import Iterable, Iterator
+
+import numpy as np
+
+from .engine import ICGNError, ImageInput, ReferenceInfo, RegistrationResult
+from .hip_backend import (
+    DEFAULT_IN_FLIGHT_FRAMES,
+    _exclusive,
+    _exclusive_stream,
+    _NativeEngine,
+    _validated_config,
+)
+from .subset import DEFAULT_SUBSET_SIZE
+
+DEFAULT_FULL_PYRAMID_LEVELS = 2
+# Iteration ceilings and tolerances per level, coarsest first
+# (full_pyramid_backend.py:51-62).
+_DEFAULT_ITERATION_SCHEDULES = {1: (30,), 2: (8, 16), 3: (6, 8, 16), 4: (12, 8, 8, 16)}
+_DEFAULT_TOLERANCES = (1e-2, 5e-3, 5e-3)
+_FOUR_LEVEL_TOLERANCES = (1e-2, *_DEFAULT_TOLERANCES)
+
+
+def _level_count(value: int) -> int:
+    try:
+        count = operator.index(value)
+    except TypeError as error:
+        raise ValueError("pyramid_levels must be an integer from 1 to 4") from error
+    if isinstance(value, bool) or not 1 <= count <= 4:
+        raise ValueError("pyramid_levels must be an integer from 1 to 4")
+    return count
+
+
+def _schedule(values, *, level_count: int, defaults, name: str):
+    selected = tuple(defaults[-level_count:] if values is None else values)
+    if len(selected) != level_count:
+        raise ValueError(f"{name} must contain one value per pyramid level")
+    return selected
+
+
+class ICGNWGPUFullPyramid(_NativeEngine):
+    """GPU-resident image pyramid with a coarse-to-fine IC-GN solve.
+
+    ``iterations_per_level`` and ``tolerance_per_level`` are ordered from the
+    coarsest level to the finest.  Only ``initialization="zero"`` and the
+    fixed pyramid policy are built; the reference's opt-in temporal, Fourier
+    and matching initialisers are not.
+    """
+
+    def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
+                 iterations_per_level: tuple[int, ...] | None = None,
+                 tolerance_per_level: tuple[float, ...] | None = None,
+                 power_preference: str = "high-performance", device_type: str = "auto",
+                 solver_workgroup_size: int = 16, interpolation: str = "bspline", motion_model: str = "affine",
+                 outlier_policy: str = "replace", median_filter_size: int = 3,
+                 median_filter_scope: str = "rejected", subset_size: int = DEFAULT_SUBSET_SIZE,
+                 subset_stride: int | None = None, spatial_weighting: str = "uniform",
+                 gaussian_sigma: float | None = None, initialization: str = "zero",
+                 pyramid_policy: str = "fixed", **unsupported) -> None:
+        if unsupported:
+            raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
+        if not isinstance(pyramid_policy, str) or pyramid_policy not in ("fixed", "adaptive"):
+            raise ValueError("pyramid_policy must be 'fixed' or 'adaptive'")
+        if pyramid_policy != "fixed":
+            raise ValueError("adaptive pyramid requires temporal initialization, which is not built")
+        if not isinstance(initialization, str) or initialization not in (
+                "zero", "temporal", "fourier", "wgpu-matching"):
+            raise ValueError("initialization must be 'zero', 'temporal', 'fourier' or 'wgpu-matching'")
+        if initialization != "zero":
+            raise ValueError(f"initialization={initialization!r} is not built; use 'zero'")
+        self.pyramid_levels = _level_count(pyramid_levels)
+        self.pyramid_policy = pyramid_policy
+        self.initialization = initialization
+        L = self.pyramid_levels
+        tol_defaults = _FOUR_LEVEL_TOLERANCES if L == 4 else _DEFAULT_TOLERANCES
+        self.iterations_per_level = tuple(int(v) for v in _schedule(
+            iterations_per_level, level_count=L, defaults=_DEFAULT_ITERATION_SCHEDULES[L],
+            name="iterations_per_level"))
+        self.tolerance_per_level = tuple(float(v) for v in _schedule(
+            tolerance_per_level, level_count=L, defaults=tol_defaults, name="tolerance_per_level"))
+        for it, tol in zip(self.iterations_per_level, self.tolerance_per_level, strict=True):
+            _validated_config(it, tol, 0.8)
+        super().__init__(levels=L, power_preference=power_preference, device_type=device_type,
+                         solver_workgroup_size=solver_workgroup_size, interpolation=interpolation,
+                         motion_model=motion_model, outlier_policy=outlier_policy,
+                         median_filter_size=median_filter_size, median_filter_scope=median_filter_scope,
+                         subset_size=subset_size, subset_stride=subset_stride,
+                         spatial_weighting=spatial_weighting, gaussian_sigma=gaussian_sigma)
+        self._zncc_min = 0.8
+
+    def set_reference(self, image: ImageInput, *, iterations_per_level: tuple[int, ...] | None = None,
+                      tolerance_per_level: tuple[float, ...] | None = None, zncc_min: float = 0.8,
+                      roi_mask: np.ndarray | None = None, roi_policy: str = "full",
+                      roi_truncation: str = "none") -> ReferenceInfo:
+        """Prepare every level's reference on the device (full_pyramid_backend.py:1135-1368)."""
+        started = time.perf_counter()
+        L = self.pyramid_levels
+        tol_defaults = _FOUR_LEVEL_TOLERANCES if L == 4 else _DEFAULT_TOLERANCES
+        iterations = self.iterations_per_level if iterations_per_level is None else tuple(
+            int(v) for v in _schedule(iterations_per_level, level_count=L,
+                                      defaults=_DEFAULT_ITERATION_SCHEDULES[L], name="iterations_per_level"))
+        tolerances = self.tolerance_per_level if tolerance_per_level is None else tuple(
+            float(v) for v in _schedule(tolerance_per_level, level_count=L, defaults=tol_defaults,
+                                        name="tolerance_per_level"))
+        configs_coarse_first = [_validated_config(i, t, zncc_min) for i, t in zip(iterations, tolerances)]
+        configs = configs_coarse_first[::-1]  # finest first for the native engine
+        build_ms = self._prepare_reference(image, configs, roi_mask, roi_policy, roi_truncation)
+        self.iterations_per_level = iterations
+        self.tolerance_per_level = tolerances
+        self._zncc_min = float(zncc_min)
+        self._config = configs[0]
+        self._pyramid_config = {
+            "pyramidLevels": L,
+            "pyramidStrategy": "full",
+            "iterationsPerLevel": list(iterations),
+            "tolerancePerLevel": list(tolerances),
+        }
+        return self._reference_info(started, build_ms, {
+            **configs[0],
+            **({"roi": {**self.roi.summary(), "selectedPerLevel": self._roi_level_counts}} if self.roi else {}),
+            **self._option_config()})
+
+    @_exclusive
+    def register(self, target: ImageInput) -> RegistrationResult:
+        """One upload, one queued coarse-to-fine sequence and one readback."""
+        if not self._reference_ready:
+            raise ICGNError("Call set_reference() before register().")
+        started = time.perf_counter()
+        frame = self._prepare_target(target)
+        if self.roi is not None and self.roi.empty:
+            return self._empty_roi_result(started, self._config)
+        prep_ms = (time.perf_counter() - started) * 1e3
+        result = self._run(frame, started, prep_ms, self._config)
+        result.timing["pyramidLevels"] = float(self.pyramid_levels)
+        return result
+
+    @_exclusive_stream
+    def register_stream(self, targets: Iterable[ImageInput], *,
+                        pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES) -> Iterator[RegistrationResult]:
+        """Ordered results with at most ``pipeline_depth`` frames in flight."""
+        if not self._reference_ready:
+            raise ICGNError("Call set_reference() before register_stream().")
+        if isinstance(pipeline_depth, bool):
+            raise ValueError("pipeline_depth must be a positive integer")
+        for result in self._stream(targets, pipeline_depth, self._config):
+            result.timing["pyramidLevels"] = float(self.pyramid_levels)
+            yield result
+
+    def register_many(self, targets: Iterable[ImageInput], *,
+                      pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES) -> list[RegistrationResult]:
+        return list(self.register_stream(targets, pipeline_depth=pipeline_depth))

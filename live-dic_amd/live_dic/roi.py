@@ -1,0 +1,218 @@
+"""Static reference-coordinate ROI planning (reference: live_dic/roi.py).
+
+Runs once per reference on the host; nothing here runs per target.  A subset
+is selected when its whole S x S window lies inside the mask (``full`` policy).
+Neighbour and seed links require the closed pixel box around the segment
+joining two subset centres to be free of masked pixels.
+
+The reference rejects ROIs for the quadratic model (roi.py:45-46); this build
+supports them with the same full-window rule, since the solver only needs the
+selection flags and link words.  The ``partial`` policy is not built.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import struct
+from dataclasses import dataclass
+
+import numpy as np
+
+from .outlier import normalize_median_filter_size
+from .subset import subset_grid_shape
+
+
+def immutable(array: np.ndarray) -> np.ndarray:
+    """Read-only copy owning its bytes."""
+    return np.frombuffer(array.tobytes(order="C"), dtype=array.dtype).reshape(array.shape)
+
+
+def validate_roi(mask, policy: str, shape: tuple[int, int], motion_model: str, truncation: str = "none"):
+    """Checks of roi.py:29-60 (policy, truncation, dtype, shape) before any device work."""
+    if not isinstance(policy, str) or policy not in ("full", "partial"):
+        raise ValueError("roi_policy must be 'full' or 'partial'")
+    if not isinstance(truncation, str) or truncation not in ("none", "visibility"):
+        raise ValueError("roi_truncation must be 'none' or 'visibility'")
+    if truncation != "none" and (policy != "partial" or mask is None):
+        raise ValueError("roi_truncation requires roi_policy='partial' and a mask")
+    if mask is None:
+        return None
+    if policy == "partial":
+        raise ValueError("roi_policy='partial' is not supported by the HIP engine; use 'full'")
+    if not isinstance(mask, np.ndarray) or mask.dtype != np.bool_:
+        raise TypeError("roi_mask must be a two-dimensional boolean NumPy array")
+    if mask.ndim != 2 or mask.shape != tuple(shape):
+        raise ValueError(f"roi_mask must have reference shape {tuple(shape)}")
+    return immutable(np.logical_and(mask, True))
+
+
+def integral(mask: np.ndarray) -> np.ndarray:
+    """Exclusive 2-D prefix count of forbidden pixels, shape (h+1, w+1), int64."""
+    forbidden = (~mask).astype(np.int64)
+    out = np.zeros((mask.shape[0] + 1, mask.shape[1] + 1), dtype=np.int64)
+    out[1:, 1:] = forbidden.cumsum(0).cumsum(1)
+    return out
+
+
+def boxes_clear(prefix, x0, y0, x1, y1) -> np.ndarray:
+    """True where the half-open box [x0, x1) x [y0, y1) holds no forbidden pixel."""
+    return (prefix[y1, x1] - prefix[y0, x1] - prefix[y1, x0] + prefix[y0, x0]) == 0
+
+
+def select_windows(mask: np.ndarray, size: int, stride: int, *, summed_area=None) -> np.ndarray:
+    h, w = mask.shape
+    gw, gh = subset_grid_shape(w, h, subset_size=size, subset_stride=stride)
+    ys, xs = np.mgrid[:gh, :gw] * stride
+    prefix = integral(mask) if summed_area is None else summed_area
+    return boxes_clear(prefix, xs, ys, xs + size, ys + size)
+
+
+def reduce_mask(mask: np.ndarray) -> np.ndarray:
+    """Coarse pixel allowed iff all 25 reflected source pixels at (2x, 2y) are (roi.py:89-97)."""
+    h, w = mask.shape
+    padded = np.pad(mask, 2, mode="reflect")
+    out = np.ones((h // 2, w // 2), dtype=bool)
+    for dy in range(5):
+        for dx in range(5):
+            out &= padded[dy : dy + 2 * (h // 2) : 2, dx : dx + 2 * (w // 2) : 2]
+    return immutable(out)
+
+
+def _segment_clear(prefix, ax, ay, bx, by) -> np.ndarray:
+    """Closed pixel box enclosing the segment (ax, ay)-(bx, by) is mask-free."""
+    h, w = prefix.shape[0] - 1, prefix.shape[1] - 1
+    x0 = np.maximum(0, np.ceil(np.minimum(ax, bx) - 0.5).astype(np.int64))
+    y0 = np.maximum(0, np.ceil(np.minimum(ay, by) - 0.5).astype(np.int64))
+    x1 = np.minimum(w, np.floor(np.maximum(ax, bx) + 0.5).astype(np.int64) + 1)
+    y1 = np.minimum(h, np.floor(np.maximum(ay, by) + 0.5).astype(np.int64) + 1)
+    return boxes_clear(prefix, x0, y0, x1, y1)
+
+
+@dataclass(frozen=True, slots=True)
+class RoiPlan:
+    """Immutable per-level ROI tables shared by a reference and its results."""
+
+    mask: np.ndarray
+    selected: np.ndarray
+    neighbour_bits: np.ndarray
+    mask_hash: str
+    effective: bool
+    selected_count: int
+    policy: str = "full"
+    truncation: str = "none"
+    partial_indices: np.ndarray | None = None
+    median_neighbour_words: np.ndarray | None = None
+
+    @property
+    def empty(self) -> bool:
+        return self.selected_count == 0
+
+    def summary(self) -> dict:
+        return {
+            "policy": self.policy,
+            "truncation": self.truncation,
+            "partial": 0,
+            "maskShape": list(self.mask.shape),
+            "maskSha256": self.mask_hash,
+            "gridTotal": int(self.selected.size),
+            "selected": self.selected_count,
+            "excluded": int(self.selected.size) - self.selected_count,
+            "hashFormat": "live-dic-roi-v1\\0 + uint64le(height,width) + C-order uint8 bool",
+        }
+
+    def archive_fields(self) -> dict:
+        return {"roi_mask": self.mask, "roi_selected": self.selected}
+
+    def neighbour_words(self) -> np.ndarray:
+        """(grid, 4) float32 link words as appended to the solver metadata."""
+        if self.median_neighbour_words is not None:
+            return np.ascontiguousarray(self.median_neighbour_words, dtype=np.float32)
+        words = np.zeros((self.selected.size, 4), dtype=np.float32)
+        words[:, 0] = self.neighbour_bits.ravel()
+        return words
+
+
+def prepare_roi(mask, size: int, stride: int, *, policy: str = "full", truncation: str = "none",
+                median_filter_size: int = 3) -> RoiPlan | None:
+    """Selection, 3x3 neighbour bits and optional 9x9 median words (roi.py:178-263)."""
+    median_filter_size = normalize_median_filter_size(median_filter_size)
+    if mask is None:
+        return None
+    if policy != "full" or truncation != "none":
+        raise ValueError("only the full ROI policy is supported")
+    h, w = mask.shape
+    every = bool(mask.all())
+    prefix = None if every else integral(mask)
+    gw, gh = subset_grid_shape(w, h, subset_size=size, subset_stride=stride)
+    selected = np.ones((gh, gw), dtype=bool) if every else select_windows(mask, size, stride, summed_area=prefix)
+    gy, gx = np.indices(selected.shape)
+    cx = gx * stride + (size - 1) / 2
+    cy = gy * stride + (size - 1) / 2
+
+    def linked(dx: int, dy: int) -> np.ndarray:
+        nx, ny = gx + dx, gy + dy
+        inside = (nx >= 0) & (ny >= 0) & (nx < gw) & (ny < gh)
+        ok = inside & selected & selected[ny.clip(0, gh - 1), nx.clip(0, gw - 1)]
+        if prefix is not None:
+            ok &= _segment_clear(prefix, cx, cy, cx + dx * stride, cy + dy * stride)
+        return ok
+
+    bits = np.zeros(selected.shape, dtype=np.uint32)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            bits |= linked(dx, dy).astype(np.uint32) << np.uint32((dy + 1) * 3 + dx + 1)
+    words = None
+    if median_filter_size > 3:
+        r = median_filter_size // 2
+        words = np.zeros((gh, gw, 4), dtype=np.float32)
+        for dy in range(-r, r + 1):
+            for dx in range(-r, r + 1):
+                bit = (dy + 4) * 9 + dx + 4
+                words[..., bit // 24] += linked(dx, dy).astype(np.float32) * float(1 << (bit % 24))
+        words = immutable(words.reshape(-1, 4))
+    digest = hashlib.sha256(b"live-dic-roi-v1\0" + struct.pack("<QQ", *mask.shape) + mask.tobytes(order="C")).hexdigest()
+    return RoiPlan(mask, immutable(selected), immutable(bits), digest, not every, int(selected.sum()),
+                   policy, truncation, None, words)
+
+
+def seed_permissions(source: RoiPlan, destination: RoiPlan, *, source_scale: int, destination_scale: int,
+                     size: int, stride: int, reference_mask: np.ndarray, reference_prefix=None) -> np.ndarray:
+    """Transfer link words (roi.py:358-437): bits 0-3 corners, bit 4 selection,
+    bits 5.. one-based index of a fallback coarse subset."""
+    sel_s, sel_d = source.selected, destination.selected
+    sh, sw = sel_s.shape
+    y, x = np.indices(sel_d.shape)
+    c = size / 2
+    sx = (destination_scale * (c + x * stride) + (source_scale - destination_scale) / 2) / source_scale
+    sy = (destination_scale * (c + y * stride) + (source_scale - destination_scale) / 2) / source_scale
+    lx = np.clip(np.floor((sx - c) / stride), 0, max(0, sw - 2)).astype(int)
+    ly = np.clip(np.floor((sy - c) / stride), 0, max(0, sh - 2)).astype(int)
+    prefix = reference_prefix if reference_prefix is not None else integral(reference_mask)
+    half = (size - 1) / 2
+    bits = sel_d.astype(np.uint32) << np.uint32(4)
+    for corner in range(4):
+        nx, ny = lx + (corner & 1), ly + (corner >> 1)
+        inside = (nx < sw) & (ny < sh)
+        nx, ny = nx.clip(0, sw - 1), ny.clip(0, sh - 1)
+        ok = inside & sel_s[ny, nx] & sel_d
+        ok &= _segment_clear(prefix, destination_scale * (half + x * stride), destination_scale * (half + y * stride),
+                             source_scale * (half + nx * stride), source_scale * (half + ny * stride))
+        bits |= ok.astype(np.uint32) << np.uint32(corner)
+    if source_scale > destination_scale and sel_s.size < (1 << 27) - 1:
+        my, mx = np.nonzero(sel_d & ((bits & 15) == 0))
+        best = np.full(mx.shape, np.inf)
+        chosen = np.zeros(mx.shape, dtype=np.uint32)
+        for dy in range(-2, 4):
+            for dx in range(-2, 4):
+                nx, ny = lx[my, mx] + dx, ly[my, mx] + dy
+                inside = (nx >= 0) & (ny >= 0) & (nx < sw) & (ny < sh)
+                cxn, cyn = nx.clip(0, sw - 1), ny.clip(0, sh - 1)
+                dist = (sx[my, mx] - (c + cxn * stride)) ** 2 + (sy[my, mx] - (c + cyn * stride)) ** 2
+                ok = inside & sel_s[cyn, cxn] & (dist < best)
+                ok &= _segment_clear(prefix, destination_scale * (half + mx * stride),
+                                     destination_scale * (half + my * stride), source_scale * (half + cxn * stride),
+                                     source_scale * (half + cyn * stride))
+                best[ok] = dist[ok]
+                chosen[ok] = (cyn[ok] * sw + cxn[ok] + 1).astype(np.uint32)
+        bits[my, mx] |= chosen << np.uint32(5)
+    return immutable(bits)

@@ -1,0 +1,52 @@
+"""Extract the reference's own fixtures and WGSL golden fields used by the parity tests.
+
+Reads (as data, never executes) files of the reference checkout:
+- DIC/LucasKanadeICGN/assets/frame_000.png, frame_010.png (512x512 translation fixtures,
+  assets/source.json: 0.2/-0.1 px per frame);
+- DIC/LucasKanadeICGN/assets/uhd_tensile/reference.png, frame_040.png (native UHD tensile pair);
+- DIC/LucasKanadeICGN/benchmarks/results/binding-budget-20261031/before.npz, the WGSL fields
+  captured by benchmarks/qualify_binding_budget.py (ICGNWGPUFullPyramid, outlier_policy="median",
+  16/16, B-spline; ROI mask [160:2000,160:3680] minus [880:1200,1800:2040]).
+Writes tests/golden/*.png (copied bytes) and tests/golden/uhd_goldens.npz (selected cases).
+The assets and golden fields are GPL-2.0-or-later data of the reference project.
+"""
+
+from __future__ import annotations
+
+import shutil
+from pathlib import Path
+
+import numpy as np
+
+SRC = Path("/root/reference/DIC/LucasKanadeICGN")
+OUT = Path(__file__).resolve().parent
+CASES = (
+    "affine-l1-none-zero",
+    "translation-l1-none-zero",
+    "quadratic-l1-none-zero",
+    "affine-l2-none-zero",
+    "affine-l3-none-zero",
+    "quadratic-l2-none-zero",
+    "affine-l1-full-zero",
+    "translation-l2-full-zero",
+)
+
+
+def main() -> None:
+    shutil.copyfile(SRC / "assets/frame_000.png", OUT / "frame_000.png")
+    shutil.copyfile(SRC / "assets/frame_010.png", OUT / "frame_010.png")
+    shutil.copyfile(SRC / "assets/uhd_tensile/reference.png", OUT / "uhd_reference.png")
+    shutil.copyfile(SRC / "assets/uhd_tensile/frame_040.png", OUT / "uhd_frame_040.png")
+    fields = {}
+    with np.load(SRC / "benchmarks/results/binding-budget-20261031/before.npz", allow_pickle=False) as data:
+        for case in CASES:
+            for frame in (0, 1):
+                for key in ("parameters", "status", "iterations"):
+                    name = f"{case}-f{frame}-{key}"
+                    fields[name] = data[name]
+    np.savez_compressed(OUT / "uhd_goldens.npz", **fields)
+    print("wrote", len(fields), "arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,37 @@
+"""Shared comparison helpers for the parity tests."""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def load_gray(name: str) -> np.ndarray:
+    from PIL import Image
+
+    with Image.open(GOLDEN / name) as im:
+        return np.asarray(im.convert("L"), dtype=np.uint8).copy()
+
+
+def compare_fields(candidate: np.ndarray, expected: np.ndarray, params: int) -> dict:
+    """Status identity and parameter agreement of two packed (gh, gw, stride) fields.
+
+    The bar: identical status on >= 99.9 % of subsets and |du| <= 1e-3 px on the
+    subsets both accept (after the reference's own implementation-vs-implementation
+    comparator, wgpu_backend.py:222-299, tightened to the north-star tolerance).
+    """
+    cs, es = candidate[..., params], expected[..., params]
+    both = (cs > 0.5) & (es > 0.5)
+    diff = np.abs(candidate[..., :params] - expected[..., :params])[both]
+    disp = np.maximum(diff[:, 0], diff[:, 3]) if diff.size else np.zeros(0)
+    return {
+        "status_equal": float(np.mean(cs == es)),
+        "accepted_candidate": int(np.count_nonzero(cs > 0.5)),
+        "accepted_expected": int(np.count_nonzero(es > 0.5)),
+        "disp_within_1e3": float(np.mean(disp <= 1e-3)) if disp.size else 1.0,
+        "disp_max": float(disp.max()) if disp.size else 0.0,
+        "param_rmse": float(np.sqrt(np.mean(diff.astype(np.float64) ** 2))) if diff.size else 0.0,
+    }

@@ -1,0 +1,217 @@
+"""GPU parity: the HIP library against the CPU oracle and the reference's WGSL goldens.
+
+Each test prints the case it runs before touching the device.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import compare_fields, load_gray
+from oracle import reference_prep as prep
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*args):
+    print("[case]", *args, flush=True)
+
+
+@pytest.fixture(scope="module")
+def native():
+    from live_dic import _native
+
+    return _native
+
+
+def test_reduce_is_bit_exact(native):
+    rng = np.random.default_rng(1)
+    for h, w in ((37, 29), (300, 512), (2160, 3840)):
+        _say("reduce", h, w)
+        img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+        out = np.empty((h // 2, w // 2), np.uint8)
+        native.check(native.library().ldic_op_reduce(img.ctypes.data, w, h, out.ctypes.data))
+        np.testing.assert_array_equal(out, oracle.reduce(img))
+
+
+def test_prefilter_is_bit_exact(native):
+    rng = np.random.default_rng(2)
+    for h, w in ((20, 17), (77, 100), (512, 512)):
+        _say("prefilter", h, w)
+        img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+        out = np.empty((h, w), np.float32)
+        native.check(native.library().ldic_op_prefilter(img.ctypes.data, w, h, out.ctypes.data))
+        np.testing.assert_array_equal(out, oracle.prefilter(img))
+
+
+@pytest.fixture(scope="module")
+def small_pair():
+    from live_dic import synthetic
+
+    _say("render 512x512 speckle pair")
+    return synthetic.small_affine_pair()
+
+
+@pytest.mark.parametrize(
+    "model,size,stride,weighting,interp",
+    [
+        ("affine", 21, 8, "uniform", "bspline"),
+        ("affine", 16, 16, "uniform", "bspline"),
+        ("translation", 21, 8, "uniform", "bspline"),
+        ("quadratic", 25, 4, "uniform", "bspline"),
+        ("affine", 21, 8, "gaussian", "bspline"),
+        ("affine", 21, 8, "uniform", "linear"),
+        ("affine", 21, 8, "uniform", "cubic"),
+    ],
+)
+def test_reference_preparation_matches_oracle(small_pair, model, size, stride, weighting, interp):
+    from live_dic import ICGNWGPU
+
+    ref, _, _ = small_pair
+    _say("prep", model, size, stride, weighting, interp)
+    with ICGNWGPU(motion_model=model, subset_size=size, subset_stride=stride, spatial_weighting=weighting,
+                  interpolation=interp) as eng:
+        eng.set_reference(ref)
+        plane, meta, inv = eng.export_reference(0)
+    e_plane, e_inv, e_meta = prep.reference_arrays(ref, interpolation=interp, motion_model=model, subset_size=size,
+                                                   subset_stride=stride, spatial_weighting=weighting)
+    P = 12 if model == "quadratic" else 6
+    np.testing.assert_allclose(plane[..., :3], e_plane[..., :3], rtol=2e-6, atol=2e-5)
+    np.testing.assert_array_equal(meta[:, 1], e_meta[:, 1])
+    scale = np.maximum(np.abs(e_meta).max(axis=0), 1e-30)
+    np.testing.assert_allclose(meta / scale, e_meta / scale, rtol=0, atol=2e-6)
+    valid = e_meta[:, 1] > 0.5
+    iscale = np.abs(e_inv[valid]).max(axis=1, keepdims=True)
+    np.testing.assert_allclose(inv[valid] / iscale, e_inv[valid] / iscale, rtol=0, atol=2e-5)
+    assert P * P == inv.shape[1]
+
+
+def _oracle_pipeline(ref, **kw):
+    return oracle.Pipeline(**kw).set_reference(ref)
+
+
+@pytest.mark.parametrize("model", ["affine", "translation", "quadratic"])
+def test_single_level_solver_matches_oracle(small_pair, model):
+    from live_dic import ICGNWGPU
+
+    ref, tgt, _ = small_pair
+    size, stride = (25, 8) if model == "quadratic" else (21, 8)
+    _say("solve 512", model, size, stride)
+    with ICGNWGPU(motion_model=model, subset_size=size, subset_stride=stride) as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-4)
+        got = eng.register(tgt)
+        raw = np.concatenate([got.parameters, got.status[..., None], got.iterations[..., None]], axis=-1)
+    pipe = _oracle_pipeline(ref, pyramid_levels=1, iterations_per_level=(20,), tolerance_per_level=(1e-4,),
+                            motion_model=model, subset_size=size, subset_stride=stride)
+    want = pipe.register(tgt)
+    P = 12 if model == "quadratic" else 6
+    stats = compare_fields(raw, want[..., : P + 2], P)
+    print(stats)
+    assert stats["status_equal"] >= 0.999
+    assert stats["disp_within_1e3"] >= 0.999
+
+
+def test_small_affine_accuracy_against_analytic_field(small_pair):
+    from live_dic import ICGNWGPU, synthetic
+
+    ref, tgt, fld = small_pair
+    _say("accuracy 512 affine S21/D8")
+    with ICGNWGPU(subset_size=21, subset_stride=8) as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-4)
+        res = eng.register(tgt)
+    cx, cy = synthetic.subset_centres(512, 512, 21, 8)
+    u, v = fld.displacement(cx, cy)
+    m = res.status == 1
+    err = np.hypot(res.parameters[..., 0] - u, res.parameters[..., 3] - v)[m]
+    rmse = float(np.sqrt(np.mean(err**2)))
+    print("accepted", m.sum(), "rmse", rmse)
+    # Border subsets touch the 2-pixel sampling margin; the interior must converge.
+    assert m[1:-1, 1:-1].mean() > 0.97
+    assert rmse <= 0.01
+
+
+def test_uhd_golden_statuses_and_oracle():
+    from live_dic import ICGNWGPUFullPyramid
+
+    ref = load_gray("uhd_reference.png")
+    tgt = load_gray("uhd_frame_040.png")
+    goldens = np.load(oracle_golden_path(), allow_pickle=False)
+    for model in ("affine", "quadratic"):
+        _say("uhd golden", model, "l1")
+        with ICGNWGPUFullPyramid(pyramid_levels=1, motion_model=model, outlier_policy="median") as eng:
+            eng.set_reference(ref)
+            res = eng.register(tgt)
+        P = 12 if model == "quadratic" else 6
+        raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+        key = f"{model}-l1-none-zero-f1"
+        golden = np.concatenate([goldens[key + "-parameters"], goldens[key + "-status"][..., None],
+                                 goldens[key + "-iterations"][..., None]], axis=-1)
+        stats = compare_fields(raw, golden, P)
+        print("vs WGSL golden", stats)
+        assert stats["status_equal"] >= 0.999
+        want = _oracle_pipeline(ref, pyramid_levels=1, motion_model=model, outlier_policy="median").register(tgt)
+        stats = compare_fields(raw, want[..., : P + 2], P)
+        print("vs oracle", stats)
+        assert stats["status_equal"] >= 0.999
+        # The quadratic solve at the default 5e-3 tolerance is sensitive to summation
+        # order: the oracle itself with 64 instead of 16 lanes keeps |du| <= 1e-3 on
+        # 99.895 % of accepted subsets (and 99.904 % with 32 lanes).
+        assert stats["disp_within_1e3"] >= (0.999 if model == "affine" else 0.998)
+
+
+def oracle_golden_path():
+    from helpers import GOLDEN
+
+    return GOLDEN / "uhd_goldens.npz"
+
+
+def test_two_level_pyramid_matches_oracle(small_pair):
+    from live_dic import ICGNWGPUFullPyramid
+
+    ref, tgt, _ = small_pair
+    _say("pyramid 2 levels 512 affine 21/8")
+    with ICGNWGPUFullPyramid(pyramid_levels=2, subset_size=21, subset_stride=8) as eng:
+        eng.set_reference(ref)
+        res = eng.register(tgt)
+    raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+    want = _oracle_pipeline(ref, pyramid_levels=2, subset_size=21, subset_stride=8).register(tgt)
+    stats = compare_fields(raw, want, 6)
+    print(stats)
+    assert stats["status_equal"] >= 0.999
+    assert stats["disp_within_1e3"] >= 0.999
+
+
+def test_register_stream_preserves_order(small_pair):
+    from live_dic import ICGNWGPUFullPyramid
+
+    ref, tgt, _ = small_pair
+    _say("stream depth 3")
+    frames = [ref, tgt, ref, tgt, tgt]
+    with ICGNWGPUFullPyramid(pyramid_levels=2, subset_size=21, subset_stride=8) as eng:
+        eng.set_reference(ref)
+        single = [eng.register(f).parameters.copy() for f in frames]
+        streamed = [r.parameters.copy() for r in eng.register_stream(frames, pipeline_depth=3)]
+    for a, b in zip(single, streamed):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_final_zncc_matches_oracle(small_pair):
+    from live_dic import ICGNWGPU
+
+    ref, tgt, _ = small_pair
+    _say("final zncc 512")
+    with ICGNWGPU(subset_size=21, subset_stride=8) as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-4)
+        res = eng.register(tgt, include_zncc=True)
+        plane, _, _ = eng.export_reference(0)
+        coeff = eng.export_coefficients(0)
+    state = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+    gh, gw = res.status.shape
+    want = oracle.affine_zncc(512, 512, gw, gh, 21, 8, 0, plane, state, coeff, tgt).reshape(gh, gw)
+    got = res.zncc
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.abs(got[ok] - want[ok]).max() <= 2e-5
+    assert np.nanmin(got) > 0.9
