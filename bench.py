@@ -80,7 +80,19 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=10)
     parser.add_argument("--drain", type=int, default=32, help="frames queued between host waits")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--calibrate-copy", action="store_true",
+                        help="only copy 256 MiB with one plain device copy kernel (PMC byte-counter calibration)")
     args = parser.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": src.numel(), "read_bytes": src.numel(), "written_bytes": dst.numel()}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for the "
               "sharding design); run with --gpus 1", file=sys.stderr)

@@ -18,6 +18,7 @@ namespace ldic {
 constexpr int kWavesPerBlock = 4;
 constexpr float kSixth = 1.0f / 6.0f;
 
+// Cubic B-spline weights, icgn.wgsl:230-240 (the division by 6 becomes a product).
 __device__ __forceinline__ void bspline_weights(float t, float w[4]) {
     float t2 = t * t, t3 = t2 * t, inv = 1.0f - t;
     w[0] = inv * inv * inv * kSixth;
@@ -40,15 +41,18 @@ struct Tile {
     bool valid;
 };
 
-template <int INTERP>
+// TP > 0: compile-time row pitch, so the 16 taps are immediate offsets from one
+// per-pixel LDS address; TP == 0: the pitch is the staged box width.
+template <int INTERP, int TP>
 __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, float px, float py) {
+    const int pitch = TP > 0 ? TP : t.w;
     float fx = floorf(px), fy = floorf(py);
     int ix = (int)fx, iy = (int)fy;
     float tx = px - fx, ty = py - fy;
     if (INTERP == kLinear) {
-        const float* b = tile + (iy - t.y0) * t.w + (ix - t.x0);
+        const float* b = tile + (iy - t.y0) * pitch + (ix - t.x0);
         float top = b[0] * (1.0f - tx) + b[1] * tx;
-        float bottom = b[t.w] * (1.0f - tx) + b[t.w + 1] * tx;
+        float bottom = b[pitch] * (1.0f - tx) + b[pitch + 1] * tx;
         return top * (1.0f - ty) + bottom * ty;
     }
     float wx[4], wy[4];
@@ -59,15 +63,15 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
         bspline_weights(tx, wx);
         bspline_weights(ty, wy);
     }
-    const float* b = tile + (iy - 1 - t.y0) * t.w + (ix - 1 - t.x0);
-    float v = 0.0f;
+    const float* b = tile + (iy - 1 - t.y0) * pitch + (ix - 1 - t.x0);
+    float v;
 #pragma unroll
     for (int yy = 0; yy < 4; ++yy) {
-        const float* r = b + yy * t.w;
-        float row = 0.0f;
+        const float* r = b + yy * pitch;
+        float row = wx[0] * r[0];
 #pragma unroll
-        for (int xx = 0; xx < 4; ++xx) row += wx[xx] * r[xx];
-        v += wy[yy] * row;
+        for (int xx = 1; xx < 4; ++xx) row += wx[xx] * r[xx];
+        v = yy == 0 ? wy[0] * row : v + wy[yy] * row;
     }
     return v;
 }
@@ -291,15 +295,16 @@ __device__ __forceinline__ float increment_change(const float* d, float he) {
 }
 
 // Stage the target footprint [x0, x0+w) x [y0, y0+h) of the wave's LDS slice.
-template <int INTERP>
+template <int INTERP, int TP>
 __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, const Tile& t, int lane) {
     const int n = t.w * t.h;
+    const int pitch = TP > 0 ? TP : t.w;
     const float inv_w = 1.0f / (float)t.w;
     for (int i = lane; i < n; i += kWave) {
         int r = (int)(((float)i + 0.5f) * inv_w);
         int c = i - r * t.w;
         size_t g = (size_t)(t.y0 + r) * a.width + (t.x0 + c);
-        tile[i] = INTERP == kBspline ? a.coeff[g] : (float)a.target[g];
+        tile[r * pitch + c] = INTERP == kBspline ? a.coeff[g] : (float)a.target[g];
     }
     wave_lds_sync();
 }
@@ -307,7 +312,7 @@ __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, cons
 // PPL > 0: the reference tile of each lane lives in registers (PPL pixels per
 // lane, pixel p = lane + 64 k).  PPL == 0: any subset width, reference values
 // are re-read from the plane every iteration.
-template <int MODEL, int INTERP, int PPL, bool WEIGHTED>
+template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     extern __shared__ float lds[];
     constexpr int P = MODEL == kQuadratic ? 12 : 6;
@@ -408,22 +413,26 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
             if (t.valid && rx0 >= t.x0 && ry0 >= t.y0 && rx1 < t.x0 + t.w && ry1 < t.y0 + t.h) {
                 use_tile = true;
             } else {
+                // A fixed pitch bounds the box width; its height by the slice size.
+                auto fits = [&](const Tile& b) {
+                    return TP > 0 ? (b.w <= TP && b.h * TP <= a.tile_cap) : (b.w * b.h <= a.tile_cap);
+                };
                 const int m = 2;
                 Tile nt;
                 nt.x0 = max(0, rx0 - m);
                 nt.y0 = max(0, ry0 - m);
                 nt.w = min(a.width - 1, rx1 + m) - nt.x0 + 1;
                 nt.h = min(a.height - 1, ry1 + m) - nt.y0 + 1;
-                if (nt.w * nt.h > a.tile_cap) {
+                if (!fits(nt)) {
                     nt.x0 = rx0;
                     nt.y0 = ry0;
                     nt.w = rx1 - rx0 + 1;
                     nt.h = ry1 - ry0 + 1;
                 }
-                if (nt.w * nt.h <= a.tile_cap) {
+                if (fits(nt)) {
                     nt.valid = true;
                     t = nt;
-                    stage_tile<INTERP>(a, tile, t, lane);
+                    stage_tile<INTERP, TP>(a, tile, t, lane);
                     use_tile = true;
                 }
             }
@@ -432,11 +441,13 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
         float sc = 0.0f, sc2 = 0.0f, src = 0.0f;
         float sj[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         float sq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        auto accumulate = [&](float lx, float ly, float f, float gxv, float gyv, float iw) {
+        // Sample position (centre - 0.5 + l) + d(l): the first sum is an exact pixel
+        // coordinate, so the position is rounded once, as in icgn.wgsl:741-743.
+        auto accumulate = [&](auto sampler, float lx, float ly, float f, float gxv, float gyv, float iw) {
             float dx, dy;
             displacement<MODEL>(q, lx, ly, dx, dy);
-            float px = cx - 0.5f + lx + dx, py = cy - 0.5f + ly + dy;
-            float cur = use_tile ? tile_sample<INTERP>(tile, t, px, py) : global_sample<INTERP>(a, px, py);
+            float px = (cx - 0.5f + lx) + dx, py = (cy - 0.5f + ly) + dy;
+            float cur = sampler(px, py);
             float wc = iw * cur;
             float gxg = gxv * wc, gyg = gyv * wc;
             sc += wc;
@@ -460,21 +471,31 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
                 }
             }
         };
-        if (PPL > 0) {
+        auto sweep = [&](auto sampler) {
+            if (PPL > 0) {
+                // Slots below `full` hold a pixel in every lane (a wave-uniform test);
+                // only slot `full` is partial.
+                const int full = SS / kWave, rem = SS - full * kWave;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                if (lane + kWave * k < SS)
-                    accumulate(plx[k], ply[k], rf[k], rgx[k], rgy[k], WEIGHTED ? pw[k] : 1.0f);
+                for (int k = 0; k < NP; ++k) {
+                    if (k < full || (k == full && lane < rem))
+                        accumulate(sampler, plx[k], ply[k], rf[k], rgx[k], rgy[k], WEIGHTED ? pw[k] : 1.0f);
+                }
+            } else {
+                for (int p = lane; p < SS; p += kWave) {
+                    int row = p / S, col = p - row * S;
+                    float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
+                    float lx = (float)col - he, ly = (float)row - he;
+                    float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
+                    accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw);
+                }
             }
-        } else {
-            for (int p = lane; p < SS; p += kWave) {
-                int row = p / S, col = p - row * S;
-                float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
-                float lx = (float)col - he, ly = (float)row - he;
-                float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
-                accumulate(lx, ly, r.x - ref_mean, r.y, r.z, iw);
-            }
-        }
+        };
+        // The sampling path is chosen once per iteration (wave-uniform).
+        if (use_tile)
+            sweep([&](float px, float py) { return tile_sample<INTERP, TP>(tile, t, px, py); });
+        else
+            sweep([&](float px, float py) { return global_sample<INTERP>(a, px, py); });
 
         sc = wave_sum(sc);
         sc2 = wave_sum(sc2);
@@ -571,12 +592,28 @@ int solver_pixels_per_lane(int S) {
     return 0;
 }
 
+// Tile pitch of each register-cached variant (0: pitch = staged width).
+static int solver_pitch(int ppl) {
+    switch (ppl) {
+        case 4: return 32;
+        case 8: return 48;
+        case 16: return 48;
+        case 27: return 64;
+        default: return 0;
+    }
+}
+
 size_t solver_lds_bytes(int S, int* tile_cap) {
     // Footprint of a moderately strained subset plus the interpolation halo
     // and the re-staging margin; larger footprints fall back to L2 gathers.
-    int side = S + 12;
-    int cap = side * side;
-    if (cap > 4096) cap = 4096;  // 64 KB of LDS per four-wave block
+    int pitch = solver_pitch(solver_pixels_per_lane(S));
+    int cap;
+    if (pitch > 0) {
+        cap = pitch * (S + 17);
+    } else {
+        cap = (S + 12) * (S + 12);
+        if (cap > 4096) cap = 4096;  // 64 KB of LDS per four-wave block
+    }
     *tile_cap = cap;
     return (size_t)cap * sizeof(float) * kWavesPerBlock;
 }
@@ -590,11 +627,11 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     if (cap != a.tile_cap) return hipErrorInvalidValue;
     dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
     switch (ppl) {
-        case 4: k_solve<MODEL, INTERP, 4, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
-        case 8: k_solve<MODEL, INTERP, 8, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
-        case 16: k_solve<MODEL, INTERP, 16, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
-        case 27: k_solve<MODEL, INTERP, 27, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
-        default: k_solve<MODEL, INTERP, 0, WEIGHTED><<<grid, 256, lds, s>>>(a); break;
+        case 4: k_solve<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, 256, lds, s>>>(a); break;
+        case 8: k_solve<MODEL, INTERP, 8, WEIGHTED, 48><<<grid, 256, lds, s>>>(a); break;
+        case 16: k_solve<MODEL, INTERP, 16, WEIGHTED, 48><<<grid, 256, lds, s>>>(a); break;
+        case 27: k_solve<MODEL, INTERP, 27, WEIGHTED, 64><<<grid, 256, lds, s>>>(a); break;
+        default: k_solve<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, 256, lds, s>>>(a); break;
     }
     return hipGetLastError();
 }

@@ -215,3 +215,34 @@ def test_final_zncc_matches_oracle(small_pair):
     ok = ~np.isnan(want)
     assert np.abs(got[ok] - want[ok]).max() <= 2e-5
     assert np.nanmin(got) > 0.9
+
+
+def _uhd_mask():
+    """ROI of the reference's golden capture (benchmarks/qualify_binding_budget.py:30-32)."""
+    mask = np.zeros((2160, 3840), dtype=bool)
+    mask[160:2000, 160:3680] = True
+    mask[880:1200, 1800:2040] = False
+    return mask
+
+
+@pytest.mark.parametrize("key,model,levels", [("affine-l1-full-zero", "affine", 1),
+                                                ("translation-l2-full-zero", "translation", 2)])
+def test_uhd_roi_goldens(key, model, levels):
+    from live_dic import ICGNWGPUFullPyramid
+
+    ref = load_gray("uhd_reference.png")
+    tgt = load_gray("uhd_frame_040.png") if model != "translation" else np.roll(ref, (-1, 2), (0, 1))
+    goldens = np.load(oracle_golden_path(), allow_pickle=False)
+    _say("uhd roi golden", key)
+    with ICGNWGPUFullPyramid(pyramid_levels=levels, motion_model=model, outlier_policy="median") as eng:
+        eng.set_reference(ref, roi_mask=_uhd_mask())
+        res = eng.register(tgt)
+    raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+    golden = np.concatenate([goldens[f"{key}-f1-parameters"], goldens[f"{key}-f1-status"][..., None],
+                             goldens[f"{key}-f1-iterations"][..., None]], axis=-1)
+    stats = compare_fields(raw, golden, 6)
+    print("vs WGSL golden", stats)
+    assert np.array_equal(res.status == -3, golden[..., 6] == -3)
+    assert stats["status_equal"] >= 0.999
+    if model == "translation":
+        assert stats["disp_within_1e3"] >= 0.99
