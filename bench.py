@@ -30,6 +30,10 @@ sys.path.insert(0, str(ROOT / "live-dic_amd"))
 
 METRIC = "4K frames/s + IC-GN subsets/s; learned vs ZNCC seed at ≤0.01 px RMS error"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes of one fine-level k_solve launch of this workload, measured with rocprofv3
+# FETCH_SIZE/WRITE_SIZE and scaled by a 256 MiB copy calibration (reads 2.0x FETCH_SIZE,
+# writes 1.0x WRITE_SIZE): profiles/k_solve_affine_4k_s31_d8_pmc.md.
+MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 117_755 + 3_986) * 1024
 SUBSET, STRIDE = 31, 8
 WIDTH, HEIGHT = 3840, 2160
 FRAMES = 60
@@ -189,7 +193,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": MEASURED_FINE_SOLVE_TRAFFIC_BYTES,
             "kernel": "k_solve<affine, bspline> (finest level)",
             "kernel_avg_ms": avg_solve_ms,
             "algorithmic_bytes_per_launch": traffic_bytes,
