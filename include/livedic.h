@@ -101,6 +101,19 @@ int ldic_set_level_config(struct ldic_engine* engine, int32_t level, const ldic_
 int ldic_register(struct ldic_engine* engine, const uint8_t* target, int64_t row_stride, float* out_state,
                   ldic_timing* timing);
 
+/* Consecutive-pair registration: the reference is replaced by `reference` and
+ * prepared on the device in the same queued sequence as the target's solve.
+ * The reference does this only in its browser path (web/core/reference-gpu.js:
+ * 90-117 driving shaders/reference.wgsl, then engine.registerPair); the native
+ * supervisor would call set_reference then register.  The dimensions, options,
+ * ROI tables and level configurations of the last ldic_set_reference are kept.
+ * out_reference_valid (fine grid, nullable) receives the pair's validity. */
+int ldic_register_pair(struct ldic_engine* engine, const uint8_t* reference, int64_t reference_stride,
+                       const uint8_t* target, int64_t target_stride, float* out_state,
+                       uint8_t* out_reference_valid, ldic_timing* timing);
+int ldic_register_pair_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_reference,
+                                const uint8_t* device_target, int64_t row_stride);
+
 /* register_stream slots (full_pyramid_backend.py:1834-1902): submit copies the
  * target into the slot's pinned staging and queues its work; retire waits and
  * copies the result out, in submission order chosen by the caller. */

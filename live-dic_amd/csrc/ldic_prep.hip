@@ -69,33 +69,73 @@ __device__ __forceinline__ PixelRef pixel(const PrepArgs& a, int p, int ox, int 
     return r;
 }
 
-// Accumulate upper-triangle entries [E0, E1) of H into LDS matrix `h` (P x P).
-template <int P, int E0, int E1>
-__device__ void accumulate_hessian(const PrepArgs& a, int lane, int ox, int oy, double half, double* h, int pitch) {
+// Moments of one subset.  One pixel pass accumulates the Hessian entries
+// [E0, E1) of the row-major upper triangle and, in the FIRST pass, also the
+// energy, J^T f and sum J (and stores the tile-centred intensity when D == S).
+// Totals are wave sums; the Hessian goes to the LDS matrix `h` with row pitch
+// `pitch`, the other totals to `first`.
+struct FirstMoments {
+    double energy;
+    double jtf[12];
+    double sj[12];
+};
+
+template <int P, int E0, int E1, bool FIRST>
+__device__ void moment_pass(const PrepArgs& a, int lane, int ox, int oy, double half, double mean, double* h,
+                            int pitch, FirstMoments* first) {
+#pragma clang fp contract(off)
     constexpr int NE = E1 - E0;
-    double acc[NE];
+    constexpr int N = P == 2 ? 2 : P;          // size of the symmetric system
+    constexpr int JP = P == 2 ? 6 : P;         // Jacobian components evaluated
+    double acc[NE > 0 ? NE : 1];
 #pragma unroll
     for (int e = 0; e < NE; ++e) acc[e] = 0.0;
+    double energy = 0.0, jtf[JP], sj[JP];
+#pragma unroll
+    for (int k = 0; k < JP; ++k) jtf[k] = sj[k] = 0.0;
     const int SS = a.S * a.S;
     for (int p = lane; p < SS; p += kWave) {
-        PixelRef r = pixel(a, p, ox, oy, half, 0.0);
+        PixelRef r = pixel(a, p, ox, oy, half, mean);
         double J[12];
-        jacobian<P == 2 ? 6 : P>(r.gx, r.gy, r.lx, r.ly, J);
+        jacobian<JP>(r.gx, r.gy, r.lx, r.ly, J);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const int i = tri_row(E0 + e, P == 2 ? 2 : P), j = tri_col(E0 + e, P == 2 ? 2 : P);
+            const int i = tri_row(E0 + e, N), j = tri_col(E0 + e, N);
             // Translation uses the (gx, gy) pair only: indices 0 and 3.
             const int ii = P == 2 ? 3 * i : i, jj = P == 2 ? 3 * j : j;
             acc[e] += J[ii] * J[jj] * r.w;
+        }
+        if (FIRST) {
+            energy += r.f * r.f * r.w;
+#pragma unroll
+            for (int k = 0; k < JP; ++k) {
+                if (P == 2 && k != 0 && k != 3) continue;
+                jtf[k] += J[k] * r.f * r.w;
+                sj[k] += J[k] * r.w;
+            }
+            if (a.D == a.S) {
+                int row = p / a.S, col = p - row * a.S;
+                a.reference[(size_t)(oy + row) * a.width + ox + col].x = (float)r.f;
+            }
         }
     }
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
         double v = wave_sum(acc[e]);
-        const int i = tri_row(E0 + e, P == 2 ? 2 : P), j = tri_col(E0 + e, P == 2 ? 2 : P);
+        const int i = tri_row(E0 + e, N), j = tri_col(E0 + e, N);
         if (lane == 0) {
             h[i * pitch + j] = v;
             h[j * pitch + i] = v;
+        }
+    }
+    if (FIRST) {
+        first->energy = wave_sum(energy);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) first->jtf[k] = first->sj[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < JP; ++k) {
+            first->jtf[k] = wave_sum(jtf[k]);
+            first->sj[k] = wave_sum(sj[k]);
         }
     }
 }
@@ -174,52 +214,23 @@ __global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
     }
     const double mean = wave_sum(s0) / a.weight_sum;
 
-    // Pass 1: energy, J^T f and sum J.
-    double energy = 0.0, jtf[P], sj[P];
-#pragma unroll
-    for (int k = 0; k < P; ++k) jtf[k] = sj[k] = 0.0;
-    for (int p = lane; p < SS; p += kWave) {
-        PixelRef r = pixel(a, p, ox, oy, half, mean);
-        double J[P];
-        jacobian<P>(r.gx, r.gy, r.lx, r.ly, J);
-        energy += r.f * r.f * r.w;
-        if (MODEL == kTranslation) {
-            jtf[0] += J[0] * r.f * r.w;
-            jtf[3] += J[3] * r.f * r.w;
-            sj[0] += J[0] * r.w;
-            sj[3] += J[3] * r.w;
-        } else {
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                jtf[k] += J[k] * r.f * r.w;
-                sj[k] += J[k] * r.w;
-            }
-        }
-        if (a.D == a.S) {
-            // Non-overlapping tiles: store the tile-centred intensity.
-            int row = p / a.S, col = p - row * a.S;
-            a.reference[(size_t)(oy + row) * a.width + ox + col].x = (float)r.f;
-        }
-    }
-    energy = wave_sum(energy);
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-        jtf[k] = wave_sum(jtf[k]);
-        sj[k] = wave_sum(sj[k]);
-    }
-
-    // Pass 2: Hessian, accumulated in chunks that bound register use, written
-    // straight into the augmented P x 2P layout used by the inversion.
+    // Pass 1 (and 2, 3 for quadratic): energy, J^T f, sum J and the Hessian,
+    // in chunks that bound register use, written straight into the augmented
+    // P x 2P layout used by the inversion.
     constexpr int C = 2 * P;
+    FirstMoments fm;
     if (MODEL == kQuadratic) {
-        accumulate_hessian<12, 0, 33>(a, lane, ox, oy, half, mat, C);
-        accumulate_hessian<12, 33, 63>(a, lane, ox, oy, half, mat, C);
-        accumulate_hessian<12, 63, 78>(a, lane, ox, oy, half, mat, C);
+        moment_pass<12, 0, 20, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
+        moment_pass<12, 20, 50, false>(a, lane, ox, oy, half, mean, mat, C, nullptr);
+        moment_pass<12, 50, 78, false>(a, lane, ox, oy, half, mean, mat, C, nullptr);
     } else if (MODEL == kAffine) {
-        accumulate_hessian<6, 0, 21>(a, lane, ox, oy, half, mat, C);
+        moment_pass<6, 0, 21, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
     } else {
-        accumulate_hessian<2, 0, 3>(a, lane, ox, oy, half, mat, C);
+        moment_pass<2, 0, 3, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
     }
+    const double energy = fm.energy;
+    const double* jtf = fm.jtf;
+    const double* sj = fm.sj;
     wave_lds_sync();
 
     bool valid;

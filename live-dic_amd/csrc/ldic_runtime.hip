@@ -121,6 +121,8 @@ struct Slot {
     int drained_frames = 0;
     SlotLevel lv[LDIC_MAX_LEVELS];
     uint8_t* pinned_in = nullptr;
+    uint8_t* pinned_ref = nullptr;
+    uint8_t* pinned_valid = nullptr;
     float* pinned_out = nullptr;
     float* zncc = nullptr;
     float* pinned_zncc = nullptr;
@@ -147,8 +149,11 @@ struct ldic_engine {
     size_t solve_lds = 0;
     bool ready = false;
     double gauss7[7];
-    std::vector<double> weights;  // S*S integration weights (Gaussian weighting)
     double weight_sum = 0.0;
+    // Reference-preparation scratch (float64 planes of the finest level) and the
+    // integration weights, kept for consecutive-pair registration.
+    double *prep_g = nullptr, *prep_a = nullptr, *prep_b = nullptr, *prep_gx = nullptr, *prep_gy = nullptr;
+    double* weights = nullptr;
 };
 
 namespace {
@@ -196,6 +201,8 @@ int create_slot(ldic_engine* e, Slot** out) {
     }
     CHECK(s->arena.alloc(&s->zncc, (size_t)e->lv[0].count()));
     CHECK(s->arena.alloc_pinned(&s->pinned_in, (size_t)e->lv[0].width * e->lv[0].height));
+    CHECK(s->arena.alloc_pinned(&s->pinned_ref, (size_t)e->lv[0].width * e->lv[0].height));
+    CHECK(s->arena.alloc_pinned(&s->pinned_valid, (size_t)e->lv[0].count()));
     CHECK(s->arena.alloc_pinned(&s->pinned_out, (size_t)output_floats(e)));
     CHECK(s->arena.alloc_pinned(&s->pinned_zncc, (size_t)e->lv[0].count()));
     *out = s;
@@ -443,9 +450,10 @@ int validate_config(const ldic_level_config* c) {
     return LDIC_OK;
 }
 
-int prepare_level(ldic_engine* e, int k, double* G, double* A, double* B, double* GX, double* GY, double* weights,
-                  hipStream_t st) {
+int prepare_level(ldic_engine* e, int k, hipStream_t st) {
     Level& L = e->lv[k];
+    double *G = e->prep_g, *A = e->prep_a, *B = e->prep_b, *GX = e->prep_gx, *GY = e->prep_gy;
+    double* weights = e->weights;
     const int w = L.width, h = L.height;
     const bool bspline = e->opt.interpolation == kBspline;
     if (bspline) {
@@ -478,6 +486,15 @@ int prepare_level(ldic_engine* e, int k, double* G, double* A, double* B, double
     p.inverse = L.inverse;
     p.valid = L.valid;
     HIP_CHECK(launch_subset_prep(p, st));
+    return LDIC_OK;
+}
+
+// Reference pyramid and per-level preparation, queued on `st`.  The finest
+// image is already in lv[0].image.
+int enqueue_reference(ldic_engine* e, hipStream_t st) {
+    for (int k = 1; k < e->levels; ++k)
+        HIP_CHECK(launch_reduce(e->lv[k - 1].image, e->lv[k - 1].width, e->lv[k - 1].height, e->lv[k].image, st));
+    for (int k = 0; k < e->levels; ++k) CHECK(prepare_level(e, k, st));
     return LDIC_OK;
 }
 
@@ -542,6 +559,7 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
     }
     // Integration weights of one subset (Gaussian mode).
     double* d_weights = nullptr;
+    e->weights = nullptr;
     if (e->opt.spatial_weighting == 1) {
         std::vector<double> wts((size_t)S * S);
         double half = 0.5 * (S - 1), sigma = e->opt.gaussian_sigma;
@@ -555,28 +573,23 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
         e->weight_sum = sum;
         CHECK(e->ref_arena.alloc(&d_weights, wts.size()));
         HIP_CHECK(hipMemcpy(d_weights, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice));
+        e->weights = d_weights;
     } else {
         e->weight_sum = (double)S * S;
     }
     // Upload the reference, build its pyramid and prepare every level.
     hipStream_t st = e->prep_stream;
     const Level& L0 = e->lv[0];
+    size_t px0 = (size_t)L0.width * L0.height;
+    CHECK(e->ref_arena.alloc(&e->prep_g, px0));
+    CHECK(e->ref_arena.alloc(&e->prep_a, px0));
+    CHECK(e->ref_arena.alloc(&e->prep_b, px0));
+    CHECK(e->ref_arena.alloc(&e->prep_gx, px0));
+    CHECK(e->ref_arena.alloc(&e->prep_gy, px0));
     HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, image, (size_t)row_stride, L0.width, L0.height,
                                hipMemcpyHostToDevice, st));
-    for (int k = 1; k < e->levels; ++k)
-        HIP_CHECK(launch_reduce(e->lv[k - 1].image, e->lv[k - 1].width, e->lv[k - 1].height, e->lv[k].image, st));
-    Arena scratch;
-    size_t px0 = (size_t)L0.width * L0.height;
-    double *G, *A, *B, *GX, *GY;
-    int rc = LDIC_OK;
-    if ((rc = scratch.alloc(&G, px0)) || (rc = scratch.alloc(&A, px0)) || (rc = scratch.alloc(&B, px0)) ||
-        (rc = scratch.alloc(&GX, px0)) || (rc = scratch.alloc(&GY, px0))) {
-        scratch.release();
-        return rc;
-    }
-    for (int k = 0; k < e->levels && rc == LDIC_OK; ++k) rc = prepare_level(e, k, G, A, B, GX, GY, d_weights, st);
+    int rc = enqueue_reference(e, st);
     hipError_t sync = hipStreamSynchronize(st);
-    scratch.release();
     if (rc != LDIC_OK) return rc;
     if (sync != hipSuccess) return set_error(LDIC_ERR_DEVICE, "reference preparation failed: %s", hipGetErrorString(sync));
     if (out_valid) HIP_CHECK(hipMemcpy(out_valid, L0.valid, (size_t)L0.count(), hipMemcpyDeviceToHost));
@@ -682,6 +695,54 @@ int ldic_register(ldic_engine* e, const uint8_t* target, int64_t row_stride, flo
     Slot* s = e->slots[0];
     CHECK(submit_frame(e, s, target, row_stride));
     return finish(e, s, out_state, t, wall);
+}
+
+int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t reference_stride, const uint8_t* target,
+                       int64_t target_stride, float* out_state, uint8_t* out_reference_valid, ldic_timing* t) {
+    double wall = now_ms();
+    CHECK(check_ready(e));
+    if (!reference || !target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "reference, target and out_state are required");
+    const Level& L0 = e->lv[0];
+    if (reference_stride < L0.width || target_stride < L0.width)
+        return set_error(LDIC_ERR_ARGUMENT, "row strides are smaller than the width");
+    HIP_CHECK(hipSetDevice(e->device));
+    Slot* s = e->slots[0];
+    if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
+    for (int y = 0; y < L0.height; ++y)
+        std::memcpy(s->pinned_ref + (size_t)y * L0.width, reference + (size_t)y * reference_stride, (size_t)L0.width);
+    HIP_CHECK(hipMemcpyAsync(e->lv[0].image, s->pinned_ref, (size_t)L0.width * L0.height, hipMemcpyHostToDevice,
+                             s->stream));
+    CHECK(enqueue_reference(e, s->stream));
+    CHECK(submit_frame(e, s, target, target_stride));
+    if (out_reference_valid)
+        HIP_CHECK(hipMemcpyAsync(s->pinned_valid, L0.valid, (size_t)L0.count(), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    CHECK(finish(e, s, out_state, t, wall));
+    if (out_reference_valid) std::memcpy(out_reference_valid, s->pinned_valid, (size_t)L0.count());
+    return LDIC_OK;
+}
+
+int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* device_reference,
+                                const uint8_t* device_target, int64_t row_stride) {
+    CHECK(check_slot(e, slot));
+    if (!device_reference || !device_target) return set_error(LDIC_ERR_ARGUMENT, "device pointers are required");
+    Slot* s = e->slots[slot];
+    if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    HIP_CHECK(hipSetDevice(e->device));
+    const Level& L0 = e->lv[0];
+    if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, device_reference, (size_t)row_stride, L0.width, L0.height,
+                               hipMemcpyDeviceToDevice, s->stream));
+    CHECK(enqueue_reference(e, s->stream));
+    HIP_CHECK(hipMemcpy2DAsync(s->lv[0].target, L0.width, device_target, (size_t)row_stride, L0.width, L0.height,
+                               hipMemcpyDeviceToDevice, s->stream));
+    CHECK(enqueue_frame(e, s));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->upload_ms = 0.0;
+    s->pending = true;
+    s->resident = true;
+    s->submitted_at = now_ms();
+    return LDIC_OK;
 }
 
 int ldic_submit(ldic_engine* e, int32_t slot, const uint8_t* target, int64_t row_stride) {

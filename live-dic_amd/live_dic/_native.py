@@ -92,6 +92,8 @@ SIGNATURES = {
     "ldic_level_shape": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
     "ldic_set_level_config": (ctypes.c_int, [_P, _I32, ctypes.POINTER(LevelConfig)]),
     "ldic_register": (ctypes.c_int, [_P, _P, _I64, _P, ctypes.POINTER(Timing)]),
+    "ldic_register_pair": (ctypes.c_int, [_P, _P, _I64, _P, _I64, _P, _P, ctypes.POINTER(Timing)]),
+    "ldic_register_pair_resident": (ctypes.c_int, [_P, _I32, _P, _P, _I64]),
     "ldic_set_slots": (ctypes.c_int, [_P, _I32]),
     "ldic_submit": (ctypes.c_int, [_P, _I32, _P, _I64]),
     "ldic_retire": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(Timing)]),

@@ -145,6 +145,14 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         result.timing["pyramidLevels"] = float(self.pyramid_levels)
         return result
 
+    @_exclusive
+    def register_pair(self, reference: ImageInput, target: ImageInput) -> RegistrationResult:
+        """Consecutive-pair mode: the reference pyramid and every level's preparation run on the
+        GPU in the same queued sequence as the target's coarse-to-fine solve."""
+        result = self._register_pair(reference, target, self._config)
+        result.timing["pyramidLevels"] = float(self.pyramid_levels)
+        return result
+
     @_exclusive_stream
     def register_stream(self, targets: Iterable[ImageInput], *,
                         pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES) -> Iterator[RegistrationResult]:

@@ -339,6 +339,25 @@ class _NativeEngine:
         while pending:
             yield retire()
 
+    def _register_pair(self, reference: ImageInput, target: ImageInput, config: dict) -> RegistrationResult:
+        """Replace the reference (same geometry and ROI) and register one target."""
+        if not self._reference_ready:
+            raise ICGNError("Call set_reference() before register_pair(); it fixes the geometry and ROI.")
+        started = time.perf_counter()
+        ref = self._prepare_target(reference)
+        frame = self._prepare_target(target)
+        prep_ms = (time.perf_counter() - started) * 1e3
+        raw = self._state_buffer()
+        out = (ctypes.c_char * len(raw)).from_buffer(raw)
+        valid = np.zeros((self.grid_height, self.grid_width), np.uint8)
+        t = _native.Timing()
+        _native.check(self._lib.ldic_register_pair(self._handle, ref.ctypes.data, ref.strides[0], frame.ctypes.data,
+                                                   frame.strides[0], ctypes.addressof(out), valid.ctypes.data,
+                                                   ctypes.byref(t)))
+        self.reference_valid = valid.astype(bool)
+        self.reference_valid.setflags(write=False)
+        return self._result(raw, self._timing_dict(t, started, prep_ms), config)
+
     def _set_level_configs(self, configs: list[dict]) -> None:
         for k, c in enumerate(configs):
             cfg = _native.LevelConfig(c["iterations"], c["tolerance"], c["znccMin"])
@@ -528,6 +547,12 @@ class ICGNWGPU(_NativeEngine):
             raise ICGNError("Call set_reference() before register_stream().")
         config = self._resolve_config(iterations, tolerance, zncc_min)
         yield from self._stream(targets, pipeline_depth, config)
+
+    @_exclusive
+    def register_pair(self, reference: ImageInput, target: ImageInput) -> RegistrationResult:
+        """Consecutive-pair mode: prepare `reference` on the GPU and register `target` against it,
+        keeping the geometry, ROI and solver controls of the last set_reference()."""
+        return self._register_pair(reference, target, self._config)
 
     def register_many(self, targets: Iterable[ImageInput], *, pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES,
                       iterations: int | None = None, tolerance: float | None = None,

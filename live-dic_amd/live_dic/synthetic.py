@@ -195,3 +195,64 @@ def subset_centres(width: int, height: int, size: int, stride: int) -> tuple[np.
     gh = 1 + (height - size) // stride
     gy, gx = np.mgrid[:gh, :gw].astype(np.float64)
     return gx * stride + 0.5 * (size - 1), gy * stride + 0.5 * (size - 1)
+
+
+def quadratic_plate_field(width: int = 3840, height: int = 2160) -> Field:
+    """u_x = 1e-6 (x - W/2)^2 + 0.001 y, u_y = -8e-7 (y - H/2)^2 - 0.0005 x (about the image centre)."""
+    cx, cy = width / 2, height / 2
+    return Field(origin=(cx, cy),
+                 poly=(0.001 * cy, 0.0, 0.001, -0.0005 * cx, -0.0005, 0.0, 2e-6, 0.0, 0.0, 0.0, 0.0, -1.6e-6))
+
+
+def plate_with_holes(width: int = 3840, height: int = 2160, seed: int = SEED, margin: int = 96,
+                     holes: int = 3, radius: tuple[int, int] = (200, 400)) -> np.ndarray:
+    """Rectangle minus `holes` disjoint circular holes of seeded radii and centres."""
+    rng = np.random.default_rng(seed + 7)
+    mask = np.zeros((height, width), dtype=bool)
+    mask[margin:height - margin, margin:width - margin] = True
+    yy, xx = np.mgrid[:height, :width]
+    placed: list[tuple[float, float, float]] = []
+    scale = min(width / 3840, height / 2160)
+    rmin, rmax = radius[0] * scale, radius[1] * scale
+    while len(placed) < holes:
+        r = rng.uniform(rmin, rmax)
+        cx = rng.uniform(margin + r, width - margin - r)
+        cy = rng.uniform(margin + r, height - margin - r)
+        if all((cx - px) ** 2 + (cy - py) ** 2 > (r + pr + 40) ** 2 for px, py, pr in placed):
+            placed.append((cx, cy, r))
+            mask &= (xx - cx) ** 2 + (yy - cy) ** 2 > r * r
+    return mask
+
+
+class QuadraticPairs:
+    """Consecutive-pair sequence: frame k shows the quadratic field scaled by s_k = k / (frames - 1)."""
+
+    def __init__(self, frames: int = 30, seed: int = SEED, width: int = 3840, height: int = 2160):
+        self.width, self.height, self.frames, self.seed = width, height, frames, seed
+        count = int(round(BLOB_DENSITY * width * height))
+        self.texture = SpeckleTexture(width, height, blobs(width, height, count, seed + 3))
+        self.field = quadratic_plate_field(width, height)
+
+    def scale(self, k: int) -> float:
+        return k / max(1, self.frames - 1)
+
+    def render(self, k: int) -> np.ndarray:
+        s = self.scale(k)
+        return self.texture.render(None if s == 0 else self.field.scaled(s), noise_sigma=1.0,
+                                   noise_seed=self.seed * 7919 + k)
+
+    def pair_truth(self, k: int, X: np.ndarray, Y: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """Displacement from frame k to frame k+1 at frame-k pixel coordinates (X, Y)."""
+        s0, s1 = self.scale(k), self.scale(k + 1)
+        f0 = self.field.scaled(s0)
+        mx, my = X.astype(np.float64).copy(), Y.astype(np.float64).copy()
+        for _ in range(30):  # material point M with M + u_k(M) = (X, Y)
+            u, v = f0.displacement(mx, my)
+            g = f0.gradient(mx, my)
+            rx, ry = mx + u - X, my + v - Y
+            j11, j12, j21, j22 = 1 + g[..., 0], g[..., 1], g[..., 2], 1 + g[..., 3]
+            det = j11 * j22 - j12 * j21
+            mx -= (j22 * rx - j12 * ry) / det
+            my -= (-j21 * rx + j11 * ry) / det
+        u, v = self.field.displacement(mx, my)
+        return (s1 - s0) * u, (s1 - s0) * v

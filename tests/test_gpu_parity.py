@@ -246,3 +246,35 @@ def test_uhd_roi_goldens(key, model, levels):
     assert stats["status_equal"] >= 0.999
     if model == "translation":
         assert stats["disp_within_1e3"] >= 0.99
+
+
+def test_consecutive_pair_mode_matches_set_reference_then_register():
+    """register_pair prepares the new reference on the GPU inside the frame's sequence; it must give
+    exactly what set_reference + register give, and track the analytic quadratic pair field."""
+    from live_dic import ICGNWGPUFullPyramid, synthetic
+
+    seq = synthetic.QuadraticPairs(frames=30, width=1280, height=720)
+    mask = synthetic.plate_with_holes(1280, 720, margin=40)
+    frames = [seq.render(k) for k in (10, 11, 12)]
+    _say("consecutive pairs quadratic 1280x720 S25/D4 ROI")
+    with ICGNWGPUFullPyramid(pyramid_levels=1, iterations_per_level=(30,), tolerance_per_level=(1e-3,),
+                             motion_model="quadratic", subset_size=25, subset_stride=4) as eng:
+        eng.set_reference(frames[0], roi_mask=mask)
+        first = eng.register(frames[1])
+        paired = eng.register_pair(frames[1], frames[2])
+        again = eng.register_pair(frames[0], frames[1])
+        eng.set_reference(frames[1], roi_mask=mask)
+        direct = eng.register(frames[2])
+    np.testing.assert_array_equal(again.parameters, first.parameters)
+    np.testing.assert_array_equal(again.status, first.status)
+    np.testing.assert_array_equal(paired.parameters, direct.parameters)
+    np.testing.assert_array_equal(paired.status, direct.status)
+    cx, cy = synthetic.subset_centres(1280, 720, 25, 4)
+    u, v = seq.pair_truth(11, cx, cy)
+    ok = paired.status == 1
+    err = np.hypot(paired.parameters[..., 0] - u, paired.parameters[..., 3] - v)[ok]
+    rmse = float(np.sqrt(np.mean(err**2)))
+    print("selected", int(paired.roi.selected.sum()), "accepted", int(ok.sum()), "rmse", rmse)
+    assert ok.sum() > 0.9 * paired.roi.selected.sum()
+    assert np.all(paired.status[~paired.roi.selected] == -3)
+    assert rmse <= 0.01
