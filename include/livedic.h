@@ -128,6 +128,48 @@ int ldic_register_seeded(struct ldic_engine* engine, const uint8_t* target, int6
                          const float* seed_state, const uint8_t* active, int32_t include_zncc, float* out_state,
                          float* out_zncc, ldic_timing* timing);
 
+/* ---- Seeding (no counterpart in the reference).  The reference seeds the
+ * coarsest pyramid level with zeros, a temporal prediction, a Fourier estimate
+ * or an external CPM-HOG matcher (full_pyramid_backend.py:1693-1716, seeds
+ * written to the coarsest level's seed_state with its active_mask); its
+ * nearest ZNCC formulation scores a few candidate translations
+ * (shaders/spatial_seed.wgsl:65-190).  This engine adds an exhaustive integer
+ * ZNCC search on the uint8 level planes (exact integer moments, float64
+ * quotient) in two places: as the coarsest-level initialiser, and as the
+ * fallback of seeded registration. ---- */
+enum { LDIC_INIT_ZERO = 0, LDIC_INIT_ZNCC = 1 };
+
+typedef struct ldic_initialization {
+    int32_t kind;            /* LDIC_INIT_ZERO or LDIC_INIT_ZNCC (coarsest level searched over +-search_radius) */
+    int32_t search_radius;   /* coarsest-level pixels, 0..64 */
+    float fallback_zncc;     /* seeded registration: subsets that do not converge or whose final affine ZNCC is
+                                below this are searched again around their seed and re-solved; <= -1 disables */
+    int32_t fallback_radius; /* finest-level pixels around the seed, 0..64 */
+} ldic_initialization;
+
+/* Takes effect for the next queued frame; rejected while a slot is pending.
+ * The search needs (2R+S)(ceil((2R+1)/4)+ceil(S/4))*4 + S*ceil(S/4)*4 +
+ * 8(2R+1)(2R+S-1) bytes of LDS, at most 64 KiB: search large displacements
+ * on a coarser level. */
+int ldic_set_initialization(struct ldic_engine* engine, const ldic_initialization* init);
+
+/* Seeded registration of a resident frame: seeds (packed state layout) and
+ * active flags (uint32) are device pointers, e.g. the output of the learned
+ * seed network.  Finish with ldic_slot_wait / ldic_slot_output. */
+int ldic_register_seeded_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_target,
+                                  int64_t row_stride, const float* device_seed_state, const uint32_t* device_active);
+
+/* The last search of a slot's level: shift (grid*2, level pixels) and ZNCC
+ * score (grid; -2 where no shift had a valid window, NaN where skipped). */
+int ldic_export_search(struct ldic_engine* engine, int32_t slot, int32_t level, float* shift, float* score);
+
+/* Stand-alone search on host planes (parity tests): every integer shift in
+ * [-radius, radius]^2 around the prior translation (packed state, components 0
+ * and 3; NULL for none) of every subset with mask != 0 (NULL: all). */
+int ldic_op_zncc_search(const uint8_t* reference, const uint8_t* target, int32_t width, int32_t height,
+                        int32_t subset_size, int32_t subset_stride, int32_t radius, const float* prior,
+                        int32_t prior_stride, const uint8_t* mask, float* shift, float* score);
+
 /* Resident frames for throughput measurement: the target is already in device
  * memory (row pitch row_stride bytes).  Queues the same per-frame work on the
  * slot's stream without host copies.  Further resident frames may be queued on

@@ -23,6 +23,7 @@ SOURCES = (
     "ldic_post.hip",
     "ldic_prep.hip",
     "ldic_render.hip",
+    "ldic_seed.hip",
     "ldic_runtime.hip",
 )
 ARCH = os.environ.get("LDIC_OFFLOAD_ARCH", "gfx950")

@@ -60,6 +60,20 @@ struct ZnccArgs {
     float* out;
 };
 
+struct SearchArgs {
+    int width, height, grid_w, grid_h, S, D, R;
+    const uint8_t* reference;  // uint8 planes of one level, row pitch = width
+    const uint8_t* target;
+    const float* prior;        // optional packed state: search around (prior[0], prior[3])
+    int prior_stride;
+    const uint8_t* mask;       // optional: search only where non-zero
+    float* shift;              // grid * 2: best shift with sub-pixel refinement
+    float* score;              // grid: its ZNCC, -2 when no shift is valid, NaN when skipped
+    float* seed;               // optional packed state: translation seed (zeros elsewhere)
+    int seed_stride;
+    uint32_t* active;          // optional: 1 where searched, 0 where skipped
+};
+
 struct PrepArgs {
     int width, height, grid_w, grid_h, S, D, model, weighted;
     int direct;                 // 1 for the linear and Keys interpolants
@@ -85,6 +99,13 @@ hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a,
 hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s);
 hipError_t launch_median(const MedianArgs& a, hipStream_t s);
 hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);
+
+// Exhaustive integer ZNCC search; LDS bytes per workgroup for (S, R).
+size_t zncc_search_lds_bytes(int S, int R);
+hipError_t launch_zncc_search(const SearchArgs& a, hipStream_t s);
+// mask[i] = selected[i] && (status != 1 || zncc[i] < zncc_min); zncc nullable.
+hipError_t launch_retry_mask(int count, int model, const float* state, const float* zncc, float zncc_min,
+                             const uint8_t* selected, uint8_t* mask, hipStream_t s);
 
 // Reference preparation (float64).
 hipError_t launch_gaussian(const uint8_t* src, int w, int h, const double* w7, double* tmp, double* out,

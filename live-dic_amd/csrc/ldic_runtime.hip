@@ -105,6 +105,11 @@ struct SlotLevel {
     float* filtered = nullptr;
     float* seed = nullptr;
     uint32_t* active = nullptr;
+    float* shift = nullptr;         // ZNCC search: best shift per subset
+    float* score = nullptr;         // ZNCC search: its score
+    uint8_t* mask = nullptr;        // subsets to re-solve from a ZNCC seed
+    float* retry_seed = nullptr;
+    uint32_t* retry_active = nullptr;
 };
 
 constexpr int kEventRing = 256;
@@ -154,9 +159,14 @@ struct ldic_engine {
     // integration weights, kept for consecutive-pair registration.
     double *prep_g = nullptr, *prep_a = nullptr, *prep_b = nullptr, *prep_gx = nullptr, *prep_gy = nullptr;
     double* weights = nullptr;
+    ldic_initialization init{LDIC_INIT_ZERO, 0, -2.0f, 0};
 };
 
 namespace {
+
+constexpr int kMaxSearchRadius = 64;
+constexpr int kMaxSearchSubset = 255;   // keeps the uint32 dot-product sums exact
+constexpr size_t kMaxSearchLds = 65536;
 
 int output_floats(const ldic_engine* e) { return e->lv[0].count() * state_stride(e->opt.motion_model); }
 
@@ -198,6 +208,11 @@ int create_slot(ldic_engine* e, Slot** out) {
         CHECK(s->arena.alloc(&sl.filtered, n * stride));
         CHECK(s->arena.alloc(&sl.seed, n * stride));
         CHECK(s->arena.alloc(&sl.active, n));
+        CHECK(s->arena.alloc(&sl.shift, n * 2));
+        CHECK(s->arena.alloc(&sl.score, n));
+        CHECK(s->arena.alloc(&sl.mask, n));
+        CHECK(s->arena.alloc(&sl.retry_seed, n * stride));
+        CHECK(s->arena.alloc(&sl.retry_active, n));
     }
     CHECK(s->arena.alloc(&s->zncc, (size_t)e->lv[0].count()));
     CHECK(s->arena.alloc_pinned(&s->pinned_in, (size_t)e->lv[0].width * e->lv[0].height));
@@ -248,6 +263,24 @@ SolveArgs solve_args(const ldic_engine* e, int k, const SlotLevel& sl, const flo
     a.active = active;
     a.state = sl.state;
     return a;
+}
+
+SearchArgs search_args(const ldic_engine* e, int k, const SlotLevel& sl, int radius) {
+    const Level& L = e->lv[k];
+    SearchArgs q{};
+    q.width = L.width;
+    q.height = L.height;
+    q.grid_w = L.gw;
+    q.grid_h = L.gh;
+    q.S = e->opt.subset_size;
+    q.D = e->opt.subset_stride;
+    q.R = radius;
+    q.reference = L.image;
+    q.target = sl.target;
+    q.shift = sl.shift;
+    q.score = sl.score;
+    q.seed_stride = state_stride(e->opt.motion_model);
+    return q;
 }
 
 int launch_median_level(ldic_engine* e, int k, SlotLevel& sl, bool seed_available, hipStream_t st) {
@@ -323,6 +356,17 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
             seed = sl.seed;
             active = sl.active;
         }
+        if (k == L - 1 && e->init.kind == LDIC_INIT_ZNCC) {
+            // Translation seeds of the coarsest level from the exhaustive
+            // ZNCC search; ROI-excluded subsets stay disabled.
+            SearchArgs q = search_args(e, k, sl, e->init.search_radius);
+            q.mask = lev.roi ? lev.selected : nullptr;
+            q.seed = sl.seed;
+            q.active = sl.active;
+            HIP_CHECK(launch_zncc_search(q, st));
+            seed = sl.seed;
+            active = sl.active;
+        }
         if (e->opt.interpolation == kBspline)
             HIP_CHECK(launch_prefilter(sl.target, lev.width, lev.height, e->fir, sl.coeff, st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
@@ -330,6 +374,89 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
                                solve_args(e, k, sl, seed, active, 0), st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
         CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
+    }
+    return LDIC_OK;
+}
+
+ZnccArgs zncc_args(const ldic_engine* e, const SlotLevel& sl, const float* state, float* out) {
+    const Level& L = e->lv[0];
+    ZnccArgs z{};
+    z.width = L.width;
+    z.height = L.height;
+    z.grid_w = L.gw;
+    z.grid_h = L.gh;
+    z.S = e->opt.subset_size;
+    z.D = e->opt.subset_stride;
+    z.interpolation = e->opt.interpolation;
+    z.reference = L.reference;
+    z.state = state;
+    z.coeff = sl.coeff;
+    z.target = sl.target;
+    z.out = out;
+    return z;
+}
+
+// Seeded finest-level registration (solve_seeded, icgn.wgsl:1001-1008) from
+// the seeds in lv[0].seed / lv[0].active, with the optional ZNCC fallback:
+// subsets that did not converge, or whose final affine ZNCC is below
+// init.fallback_zncc, are searched again around their seed and re-solved from
+// the ZNCC seed while every other subset keeps its state (retry_seeded,
+// icgn.wgsl:1010-1017).  include_zncc leaves the final ZNCC of the output in
+// slot->zncc.
+int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
+    hipStream_t st = s->stream;
+    const Level& L = e->lv[0];
+    SlotLevel& sl = s->lv[0];
+    const int model = e->opt.motion_model;
+    if (e->opt.interpolation == kBspline) HIP_CHECK(launch_prefilter(sl.target, L.width, L.height, e->fir, sl.coeff, st));
+    if (s->ring_count == kEventRing) CHECK(drain_ring(s));
+    const int ring = (s->ring_first + s->ring_count) % kEventRing;
+    s->ring_count += 1;
+    HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
+    HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting,
+                           solve_args(e, 0, sl, sl.seed, sl.active, 0), st));
+    if (e->init.fallback_zncc > -1.0f) {
+        const float* z = nullptr;
+        if (model == kAffine) {
+            HIP_CHECK(launch_affine_zncc(zncc_args(e, sl, sl.state, s->zncc), st));
+            z = s->zncc;
+        }
+        HIP_CHECK(launch_retry_mask(L.count(), model, sl.state, z, e->init.fallback_zncc, L.roi ? L.selected : nullptr,
+                                    sl.mask, st));
+        SearchArgs q = search_args(e, 0, sl, e->init.fallback_radius);
+        q.prior = sl.seed;
+        q.prior_stride = state_stride(model);
+        q.mask = sl.mask;
+        q.seed = sl.retry_seed;
+        q.active = sl.retry_active;
+        HIP_CHECK(launch_zncc_search(q, st));
+        HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting,
+                               solve_args(e, 0, sl, sl.retry_seed, sl.retry_active, 1), st));
+    }
+    HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
+    CHECK(launch_median_level(e, 0, sl, false, st));
+    if (include_zncc) HIP_CHECK(launch_affine_zncc(zncc_args(e, sl, level_output(e, sl), s->zncc), st));
+    return LDIC_OK;
+}
+
+int validate_initialization(const ldic_engine* e, const ldic_initialization* in) {
+    if (!in) return set_error(LDIC_ERR_ARGUMENT, "initialization is NULL");
+    const int S = e->opt.subset_size;
+    if (in->kind != LDIC_INIT_ZERO && in->kind != LDIC_INIT_ZNCC)
+        return set_error(LDIC_ERR_ARGUMENT, "initialization kind must be LDIC_INIT_ZERO or LDIC_INIT_ZNCC");
+    const bool search = in->kind == LDIC_INIT_ZNCC, fallback = in->fallback_zncc > -1.0f;
+    if (!std::isfinite(in->fallback_zncc) || in->fallback_zncc > 1.0f)
+        return set_error(LDIC_ERR_ARGUMENT, "fallback_zncc must be finite and at most 1 (<= -1 disables it)");
+    if ((search || fallback) && S > kMaxSearchSubset)
+        return set_error(LDIC_ERR_ARGUMENT, "the ZNCC search supports subset sizes up to %d", kMaxSearchSubset);
+    const int radii[2] = {search ? in->search_radius : 0, fallback ? in->fallback_radius : 0};
+    for (int r : radii) {
+        if (r < 0 || r > kMaxSearchRadius)
+            return set_error(LDIC_ERR_ARGUMENT, "search radius %d is outside 0..%d", r, kMaxSearchRadius);
+        if (zncc_search_lds_bytes(S, r) > kMaxSearchLds)
+            return set_error(LDIC_ERR_ARGUMENT,
+                             "a ZNCC search of radius %d with subset %d needs %zu bytes of LDS (limit %zu); "
+                             "search on a coarser pyramid level", r, S, zncc_search_lds_bytes(S, r), kMaxSearchLds);
     }
     return LDIC_OK;
 }
@@ -781,33 +908,10 @@ int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stri
     std::vector<uint32_t> act((size_t)n);
     for (int i = 0; i < n; ++i) act[i] = active[i] ? 1u : 0u;
     HIP_CHECK(hipMemcpyAsync(sl.active, act.data(), sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, s->stream));
-    if (e->opt.interpolation == kBspline) HIP_CHECK(launch_prefilter(sl.target, L.width, L.height, e->fir, sl.coeff, s->stream));
-    if (s->ring_count == kEventRing) CHECK(drain_ring(s));
-    const int ring = (s->ring_first + s->ring_count) % kEventRing;
-    s->ring_count += 1;
-    HIP_CHECK(hipEventRecord(s->ev_solve0[ring], s->stream));
-    HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting,
-                           solve_args(e, 0, sl, sl.seed, sl.active, 0), s->stream));
-    HIP_CHECK(hipEventRecord(s->ev_solve1[ring], s->stream));
-    CHECK(launch_median_level(e, 0, sl, false, s->stream));
+    CHECK(enqueue_seeded(e, s, include_zncc != 0));
     const float* out = level_output(e, sl);
-    if (include_zncc) {
-        ZnccArgs z{};
-        z.width = L.width;
-        z.height = L.height;
-        z.grid_w = L.gw;
-        z.grid_h = L.gh;
-        z.S = e->opt.subset_size;
-        z.D = e->opt.subset_stride;
-        z.interpolation = e->opt.interpolation;
-        z.reference = L.reference;
-        z.state = out;
-        z.coeff = sl.coeff;
-        z.target = sl.target;
-        z.out = s->zncc;
-        HIP_CHECK(launch_affine_zncc(z, s->stream));
+    if (include_zncc)
         HIP_CHECK(hipMemcpyAsync(s->pinned_zncc, s->zncc, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
-    }
     HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)n * stride, hipMemcpyDeviceToHost, s->stream));
     HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
     s->pending = true;
@@ -835,6 +939,53 @@ int ldic_register_resident(ldic_engine* e, int32_t slot, const uint8_t* device_t
     s->pending = true;
     s->resident = true;
     s->submitted_at = now_ms();
+    return LDIC_OK;
+}
+
+int ldic_register_seeded_resident(ldic_engine* e, int32_t slot, const uint8_t* device_target, int64_t row_stride,
+                                  const float* device_seed_state, const uint32_t* device_active) {
+    CHECK(check_slot(e, slot));
+    if (!device_target || !device_seed_state || !device_active)
+        return set_error(LDIC_ERR_ARGUMENT, "device target, seed_state and active are required");
+    Slot* s = e->slots[slot];
+    if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    HIP_CHECK(hipSetDevice(e->device));
+    const Level& L = e->lv[0];
+    SlotLevel& sl = s->lv[0];
+    const size_t n = (size_t)L.count();
+    s->upload_ms = 0.0;
+    if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    HIP_CHECK(hipMemcpy2DAsync(sl.target, L.width, device_target, (size_t)row_stride, L.width, L.height,
+                               hipMemcpyDeviceToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(sl.seed, device_seed_state, sizeof(float) * n * state_stride(e->opt.motion_model),
+                             hipMemcpyDeviceToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(sl.active, device_active, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s->stream));
+    CHECK(enqueue_seeded(e, s, false));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->pending = true;
+    s->resident = true;
+    s->submitted_at = now_ms();
+    return LDIC_OK;
+}
+
+int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
+    if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
+    CHECK(validate_initialization(e, init));
+    for (Slot* s : e->slots)
+        if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
+    e->init = *init;
+    return LDIC_OK;
+}
+
+int ldic_export_search(ldic_engine* e, int32_t slot, int32_t level, float* shift, float* score) {
+    CHECK(check_slot(e, slot));
+    if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
+    Slot* s = e->slots[slot];
+    HIP_CHECK(hipSetDevice(e->device));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)e->lv[level].count();
+    if (shift) HIP_CHECK(hipMemcpy(shift, s->lv[level].shift, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+    if (score) HIP_CHECK(hipMemcpy(score, s->lv[level].score, sizeof(float) * n, hipMemcpyDeviceToHost));
     return LDIC_OK;
 }
 
@@ -911,6 +1062,57 @@ int ldic_op_reduce(const uint8_t* src, int32_t w, int32_t h, uint8_t* dst) {
     a.release();
     if (rc != LDIC_OK) return rc;
     if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "reduction failed: %s", hipGetErrorString(he));
+    return LDIC_OK;
+}
+
+int ldic_op_zncc_search(const uint8_t* reference, const uint8_t* target, int32_t width, int32_t height,
+                        int32_t subset_size, int32_t subset_stride, int32_t radius, const float* prior,
+                        int32_t prior_stride, const uint8_t* mask, float* shift, float* score) {
+    if (!reference || !target || !shift || !score) return set_error(LDIC_ERR_ARGUMENT, "images and outputs are required");
+    if (subset_size < 4 || subset_size > kMaxSearchSubset || width < subset_size || height < subset_size)
+        return set_error(LDIC_ERR_ARGUMENT, "subset_size must be 4..%d and fit the image", kMaxSearchSubset);
+    if (subset_stride < 1) return set_error(LDIC_ERR_ARGUMENT, "subset_stride must be positive");
+    if (radius < 0 || radius > kMaxSearchRadius || zncc_search_lds_bytes(subset_size, radius) > kMaxSearchLds)
+        return set_error(LDIC_ERR_ARGUMENT, "radius %d is outside the supported range for subset %d", radius, subset_size);
+    if (prior && prior_stride < 4) return set_error(LDIC_ERR_ARGUMENT, "prior_stride must be at least 4");
+    SearchArgs q{};
+    q.width = width;
+    q.height = height;
+    q.grid_w = 1 + (width - subset_size) / subset_stride;
+    q.grid_h = 1 + (height - subset_size) / subset_stride;
+    q.S = subset_size;
+    q.D = subset_stride;
+    q.R = radius;
+    q.prior_stride = prior_stride;
+    const size_t px = (size_t)width * height, n = (size_t)q.grid_w * q.grid_h;
+    Arena a;
+    uint8_t *dr = nullptr, *dt = nullptr, *dm = nullptr;
+    float *dp = nullptr, *ds = nullptr, *dc = nullptr;
+    int rc = a.alloc(&dr, px);
+    if (rc == LDIC_OK) rc = a.alloc(&dt, px);
+    if (rc == LDIC_OK) rc = a.alloc(&ds, 2 * n);
+    if (rc == LDIC_OK) rc = a.alloc(&dc, n);
+    if (rc == LDIC_OK && prior) rc = a.alloc(&dp, n * prior_stride);
+    if (rc == LDIC_OK && mask) rc = a.alloc(&dm, n);
+    hipError_t he = hipSuccess;
+    if (rc == LDIC_OK) {
+        he = hipMemcpy(dr, reference, px, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(dt, target, px, hipMemcpyHostToDevice);
+        if (he == hipSuccess && prior) he = hipMemcpy(dp, prior, sizeof(float) * n * prior_stride, hipMemcpyHostToDevice);
+        if (he == hipSuccess && mask) he = hipMemcpy(dm, mask, n, hipMemcpyHostToDevice);
+        q.reference = dr;
+        q.target = dt;
+        q.prior = dp;
+        q.mask = dm;
+        q.shift = ds;
+        q.score = dc;
+        if (he == hipSuccess) he = launch_zncc_search(q, nullptr);
+        if (he == hipSuccess) he = hipMemcpy(shift, ds, sizeof(float) * 2 * n, hipMemcpyDeviceToHost);
+        if (he == hipSuccess) he = hipMemcpy(score, dc, sizeof(float) * n, hipMemcpyDeviceToHost);
+    }
+    a.release();
+    if (rc != LDIC_OK) return rc;
+    if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "ZNCC search failed: %s", hipGetErrorString(he));
     return LDIC_OK;
 }
 

@@ -78,6 +78,19 @@ class Timing(ctypes.Structure):
     ]
 
 
+class Initialization(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("search_radius", ctypes.c_int32),
+        ("fallback_zncc", ctypes.c_float),
+        ("fallback_radius", ctypes.c_int32),
+    ]
+
+
+INIT_ZERO = 0
+INIT_ZNCC = 1
+
+
 # Every symbol include/livedic.h declares, with its argument types.
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -99,6 +112,10 @@ SIGNATURES = {
     "ldic_retire": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(Timing)]),
     "ldic_register_seeded": (ctypes.c_int, [_P, _P, _I64, _P, _P, _I32, _P, _P, ctypes.POINTER(Timing)]),
     "ldic_register_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
+    "ldic_set_initialization": (ctypes.c_int, [_P, ctypes.POINTER(Initialization)]),
+    "ldic_register_seeded_resident": (ctypes.c_int, [_P, _I32, _P, _I64, _P, _P]),
+    "ldic_export_search": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
+    "ldic_op_zncc_search": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P, _P, _P]),
     "ldic_slot_wait": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Timing)]),
     "ldic_slot_output": (ctypes.c_int, [_P, _I32, _P]),
     "ldic_export_reference": (ctypes.c_int, [_P, _I32, _P, _P, _P]),

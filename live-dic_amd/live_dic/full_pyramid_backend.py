@@ -14,6 +14,7 @@ from collections.abc import Iterable, Iterator
 
 import numpy as np
 
+from . import _native
 from .engine import ICGNError, ImageInput, ReferenceInfo, RegistrationResult
 from .hip_backend import (
     DEFAULT_IN_FLIGHT_FRAMES,
@@ -53,9 +54,11 @@ class ICGNWGPUFullPyramid(_NativeEngine):
     """GPU-resident image pyramid with a coarse-to-fine IC-GN solve.
 
     ``iterations_per_level`` and ``tolerance_per_level`` are ordered from the
-    coarsest level to the finest.  Only ``initialization="zero"`` and the
-    fixed pyramid policy are built; the reference's opt-in temporal, Fourier
-    and matching initialisers are not.
+    coarsest level to the finest.  Initialisers: ``"zero"`` (the reference
+    default) and ``"zncc"``, an addition: the coarsest level is seeded by an
+    exhaustive integer ZNCC search covering ``seed_search_radius`` pixels of the
+    finest level.  The reference's temporal, Fourier and CPM-HOG matching
+    initialisers and the adaptive pyramid policy are not built.
     """
 
     def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
@@ -67,7 +70,7 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                  median_filter_scope: str = "rejected", subset_size: int = DEFAULT_SUBSET_SIZE,
                  subset_stride: int | None = None, spatial_weighting: str = "uniform",
                  gaussian_sigma: float | None = None, initialization: str = "zero",
-                 pyramid_policy: str = "fixed", **unsupported) -> None:
+                 pyramid_policy: str = "fixed", seed_search_radius: int = 32, **unsupported) -> None:
         if unsupported:
             raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
         if not isinstance(pyramid_policy, str) or pyramid_policy not in ("fixed", "adaptive"):
@@ -75,11 +78,21 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if pyramid_policy != "fixed":
             raise ValueError("adaptive pyramid requires temporal initialization, which is not built")
         if not isinstance(initialization, str) or initialization not in (
-                "zero", "temporal", "fourier", "wgpu-matching"):
-            raise ValueError("initialization must be 'zero', 'temporal', 'fourier' or 'wgpu-matching'")
-        if initialization != "zero":
-            raise ValueError(f"initialization={initialization!r} is not built; use 'zero'")
+                "zero", "zncc", "temporal", "fourier", "wgpu-matching"):
+            raise ValueError("initialization must be 'zero', 'zncc', 'temporal', 'fourier' or 'wgpu-matching'")
+        if initialization not in ("zero", "zncc"):
+            raise ValueError(f"initialization={initialization!r} is not built; use 'zero' or 'zncc'")
         self.pyramid_levels = _level_count(pyramid_levels)
+        try:
+            radius = operator.index(seed_search_radius)
+        except TypeError as error:
+            raise ValueError("seed_search_radius must be a non-negative integer") from error
+        if isinstance(seed_search_radius, bool) or radius < 0:
+            raise ValueError("seed_search_radius must be a non-negative integer")
+        self.seed_search_radius = radius
+        # Coarsest-level radius: the finest-level range scaled down, plus one
+        # pixel for the rounding of the scaled displacement.
+        self.coarse_search_radius = -(-radius // (1 << (self.pyramid_levels - 1))) + 1
         self.pyramid_policy = pyramid_policy
         self.initialization = initialization
         L = self.pyramid_levels
@@ -98,6 +111,8 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                          subset_size=subset_size, subset_stride=subset_stride,
                          spatial_weighting=spatial_weighting, gaussian_sigma=gaussian_sigma)
         self._zncc_min = 0.8
+        if initialization == "zncc":
+            self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius)
 
     def set_reference(self, image: ImageInput, *, iterations_per_level: tuple[int, ...] | None = None,
                       tolerance_per_level: tuple[float, ...] | None = None, zncc_min: float = 0.8,
@@ -126,6 +141,9 @@ class ICGNWGPUFullPyramid(_NativeEngine):
             "iterationsPerLevel": list(iterations),
             "tolerancePerLevel": list(tolerances),
         }
+        if self.initialization == "zncc":
+            self._pyramid_config.update(initialization="zncc", seedSearchRadius=self.seed_search_radius,
+                                        coarseSearchRadius=self.coarse_search_radius)
         return self._reference_info(started, build_ms, {
             **configs[0],
             **({"roi": {**self.roi.summary(), "selectedPerLevel": self._roi_level_counts}} if self.roi else {}),

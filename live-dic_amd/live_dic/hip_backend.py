@@ -391,6 +391,21 @@ class _NativeEngine:
         _native.check(self._lib.ldic_export_level_state(self._handle, slot, level, out.ctypes.data))
         return out
 
+    def _set_initialization(self, kind: int, search_radius: int = 0, fallback_zncc: float | None = None,
+                            fallback_radius: int = 0) -> None:
+        init = _native.Initialization(kind, int(search_radius), -2.0 if fallback_zncc is None else float(fallback_zncc),
+                                      int(fallback_radius))
+        _native.check(self._lib.ldic_set_initialization(self._handle, ctypes.byref(init)))
+        self._initialization = init
+
+    def export_search(self, level: int, slot: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """Shift (gh, gw, 2) and ZNCC score (gh, gw) of the slot's last ZNCC search on one level."""
+        _, _, gw, gh = self._level_shapes[level]
+        shift = np.empty((gh, gw, 2), np.float32)
+        score = np.empty((gh, gw), np.float32)
+        _native.check(self._lib.ldic_export_search(self._handle, slot, level, shift.ctypes.data, score.ctypes.data))
+        return shift, score
+
     def close(self) -> None:
         """Release device resources permanently; repeated calls are harmless."""
         if self._closed:
@@ -516,8 +531,14 @@ class ICGNWGPU(_NativeEngine):
     @_exclusive
     def register_seeded(self, target: ImageInput, parameters, active, *, iterations: int | None = None,
                         tolerance: float | None = None, zncc_min: float | None = None,
-                        include_zncc: bool = False) -> RegistrationResult:
-        """Solve active affine subsets from supplied initial states (wgpu_backend.py:2773-2819)."""
+                        include_zncc: bool = False, fallback_zncc: float | None = None,
+                        fallback_radius: int = 8) -> RegistrationResult:
+        """Solve active affine subsets from supplied initial states (wgpu_backend.py:2773-2819).
+
+        ``fallback_zncc`` (an addition; None keeps the reference behaviour) re-solves every
+        selected subset that did not converge or whose final ZNCC is below it from the best
+        integer ZNCC shift within ``fallback_radius`` pixels of its seed.
+        """
         if not self._reference_ready:
             raise ICGNError("Call set_reference() before register_seeded().")
         if self.motion_model != "affine":
@@ -533,11 +554,18 @@ class ICGNWGPU(_NativeEngine):
             raise ValueError(f"active must be a Boolean array with shape {shape}")
         if not np.isfinite(values[mask]).all():
             raise ValueError("active parameters must be finite")
+        if fallback_zncc is not None and not (-1.0 < float(fallback_zncc) <= 1.0):
+            raise ValueError("fallback_zncc must be in (-1, 1] or None")
         started = time.perf_counter()
         frame = self._prepare_target(target)
         config = self._resolve_config(iterations, tolerance, zncc_min)
         prep_ms = (time.perf_counter() - started) * 1e3
-        return self._seeded(frame, values, mask, include_zncc, started, prep_ms, config)
+        self._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius if fallback_zncc is not None else 0)
+        try:
+            return self._seeded(frame, values, mask, include_zncc, started, prep_ms, config)
+        finally:
+            if fallback_zncc is not None:
+                self._set_initialization(_native.INIT_ZERO)
 
     @_exclusive_stream
     def register_stream(self, targets: Iterable[ImageInput], *, pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES,

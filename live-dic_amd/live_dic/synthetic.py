@@ -153,6 +153,16 @@ def small_affine_pair(seed: int = SEED) -> tuple[np.ndarray, np.ndarray, Field]:
     return ref, tgt, fld
 
 
+def affine_field(width: int, height: int, tx: float, ty: float, theta: float = 0.0, exx: float = 0.0,
+                 eyy: float = 0.0) -> Field:
+    """u(X) = t + (R(theta) diag(1 + exx, 1 + eyy) - I)(X - c) about the image centre c."""
+    c, s_ = math.cos(theta), math.sin(theta)
+    a00, a01 = c * (1 + exx), -s_ * (1 + eyy)
+    a10, a11 = s_ * (1 + exx), c * (1 + eyy)
+    return Field(origin=((width - 1) / 2, (height - 1) / 2),
+                 poly=(tx, a00 - 1.0, a01, ty, a10, a11 - 1.0) + (0.0,) * 6)
+
+
 def uhd_sequence_field(width: int = 3840, height: int = 2160) -> Field:
     """u_x = 2 + 0.002 x + 1.5 sin(2 pi y / H), u_y = -1 + 0.001 y + 0.8 cos(2 pi x / W)."""
     return Field(origin=(0.0, 0.0), poly=(2.0, 0.002, 0.0, -1.0, 0.0, 0.001) + (0.0,) * 6,

@@ -266,3 +266,24 @@ class Pipeline:
 
 def cpu_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def zncc_search(ref, tgt, size, stride, radius, prior=None, mask=None):
+    """Exhaustive integer ZNCC search (``ora_zncc_search``).
+
+    ``prior`` is a packed state array (translation at components 0 and 3) or
+    None; ``mask`` selects the subsets to search.  Returns (gh, gw, 2) shifts
+    and (gh, gw) scores.
+    """
+    r = np.ascontiguousarray(ref, dtype=np.uint8)
+    t = np.ascontiguousarray(tgt, dtype=np.uint8)
+    h, w = r.shape
+    gw, gh = prep.grid_shape(w, h, size, stride)
+    shifts = np.zeros((gh * gw, 2), np.float32)
+    scores = np.zeros(gh * gw, np.float32)
+    pr = None if prior is None else np.ascontiguousarray(prior, dtype=np.float32).reshape(gh * gw, -1)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+    lib().ora_zncc_search(_p(r), _p(t), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(gw), ctypes.c_int(gh),
+                          ctypes.c_int(size), ctypes.c_int(stride), ctypes.c_int(radius), _p(pr),
+                          ctypes.c_int(0 if pr is None else pr.shape[1]), _p(mk), _p(shifts), _p(scores))
+    return shifts.reshape(gh, gw, 2), scores.reshape(gh, gw)

@@ -980,3 +980,87 @@ void ora_affine_zncc(int w, int h, int gw, int gh, int S, int D, int interpolati
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Integer-shift ZNCC seed search.  No reference counterpart: the nearest is  */
+/* the candidate scoring of shaders/spatial_seed.wgsl:65-190 (ZNCC of a few   */
+/* candidate translations).  Here every shift (dx, dy) in [-R, R]^2 around    */
+/* the prior displacement (rounded half-to-even) is scored on the raw uint8   */
+/* planes from exact integer moments, n = S^2:                               */
+/*   num = n*Sfg - Sf*Sg,  vf = n*Sff - Sf^2,  vg = n*Sgg - Sg^2  (int64)      */
+/*   zncc = (float)(num / sqrt((double)vf * (double)vg))                     */
+/* A window leaving the image, or vf == 0 or vg == 0, scores -2.  The peak is */
+/* the largest float score, ties to the lower row-major index (dy outer).    */
+/* A parabola through the peak and its two neighbours per axis gives the     */
+/* sub-pixel offset, 0 on the search border or next to an invalid shift.     */
+/* mask (nullable): subsets with mask == 0 are skipped (shift 0, score NaN). */
+/* ------------------------------------------------------------------------- */
+void ora_zncc_search(const uint8_t* ref, const uint8_t* tgt, int w, int h, int gw, int gh, int S, int D, int R,
+                     const float* prior, int prior_stride, const uint8_t* mask, float* out_shift, float* out_score) {
+    const int K = 2 * R + 1;
+    const int64_t n = (int64_t)S * S;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < gw * gh; ++i) {
+        if (mask && !mask[i]) {
+            out_shift[2 * i] = out_shift[2 * i + 1] = 0.0f;
+            out_score[i] = NAN;
+            continue;
+        }
+        const int ox = (i % gw) * D, oy = (i / gw) * D;
+        int px = 0, py = 0;
+        if (prior) {
+            /* A prior that is not finite or lies beyond the image counts as 0. */
+            float fx = prior[(size_t)i * prior_stride], fy = prior[(size_t)i * prior_stride + 3];
+            px = fabsf(fx) <= (float)w ? (int)rintf(fx) : 0;
+            py = fabsf(fy) <= (float)h ? (int)rintf(fy) : 0;
+        }
+        int64_t sf = 0, sff = 0;
+        for (int r = 0; r < S; ++r)
+            for (int c = 0; c < S; ++c) {
+                int64_t f = ref[(size_t)(oy + r) * w + ox + c];
+                sf += f;
+                sff += f * f;
+            }
+        const int64_t vf = n * sff - sf * sf;
+        float* scores = (float*)malloc(sizeof(float) * K * K);
+        int best = 0;
+        for (int s = 0; s < K * K; ++s) {
+            const int x0 = ox + px - R + s % K, y0 = oy + py - R + s / K;
+            float score = -2.0f;
+            if (x0 >= 0 && y0 >= 0 && x0 + S <= w && y0 + S <= h && vf > 0) {
+                int64_t sg = 0, sgg = 0, sfg = 0;
+                for (int r = 0; r < S; ++r)
+                    for (int c = 0; c < S; ++c) {
+                        int64_t f = ref[(size_t)(oy + r) * w + ox + c];
+                        int64_t g = tgt[(size_t)(y0 + r) * w + x0 + c];
+                        sg += g;
+                        sgg += g * g;
+                        sfg += f * g;
+                    }
+                const int64_t vg = n * sgg - sg * sg;
+                if (vg > 0) score = (float)((double)(n * sfg - sf * sg) / sqrt((double)vf * (double)vg));
+            }
+            scores[s] = score;
+            if (score > scores[best]) best = s;
+        }
+        const int bx = best % K, by = best / K;
+        float subx = 0.0f, suby = 0.0f;
+        const float c0 = scores[best];
+        if (c0 > -1.5f) {
+            if (bx > 0 && bx < K - 1) {
+                const float l = scores[best - 1], r = scores[best + 1];
+                const float den = (l + r) - 2.0f * c0;
+                if (l > -1.5f && r > -1.5f && den < 0.0f) subx = (0.5f * (l - r)) / den;
+            }
+            if (by > 0 && by < K - 1) {
+                const float u = scores[best - K], d = scores[best + K];
+                const float den = (u + d) - 2.0f * c0;
+                if (u > -1.5f && d > -1.5f && den < 0.0f) suby = (0.5f * (u - d)) / den;
+            }
+        }
+        out_shift[2 * i] = (float)(bx - R + px) + subx;
+        out_shift[2 * i + 1] = (float)(by - R + py) + suby;
+        out_score[i] = c0;
+        free(scores);
+    }
+}
