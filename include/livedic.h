@@ -227,6 +227,114 @@ int ldic_render_speckle(const ldic_speckle* speckle, const ldic_field* field, in
                         float noise_sigma, uint64_t noise_seed, uint8_t* device_out, float* device_float_out,
                         void* stream);
 
+/* Batched rendering: `count` images of width x height into device_out (image b
+ * at b*width*height).  device_fields (nullable: all undeformed) holds one field
+ * per image (enabled == 0 for an undeformed one); device_offsets (nullable)
+ * holds per-image texture offsets (x, y) so that crops of one large texture
+ * can be rendered.  Image b draws its noise from noise_seed advanced by b. */
+int ldic_render_speckle_batch(const ldic_speckle* speckle, const ldic_field* device_fields,
+                              const int32_t* device_offsets, int32_t count, int32_t width, int32_t height,
+                              float noise_sigma, uint64_t noise_seed, uint8_t* device_out, void* stream);
+
+/* reduce_fused (full_pyramid.wgsl:215-249) of `count` device images of
+ * width x height (image b at b*width*height) into (width/2) x (height/2)
+ * images, queued on `stream`. */
+int ldic_reduce_device(const uint8_t* device_src, int32_t width, int32_t height, int32_t count, uint8_t* device_dst,
+                       void* stream);
+
+/* The HIP stream of a slot, so that device work of the caller (the learned
+ * seed network) is queued in order with the slot's registrations. */
+int ldic_slot_stream(struct ldic_engine* engine, int32_t slot, void** stream);
+
+/* ---- Learned seed network (no counterpart in the reference; the reference's
+ * nearest initialisers are listed above ldic_set_initialization).  Kernels of
+ * a small correlation CNN, forward and training: bfloat16 activations and
+ * weights (uint16_t storage, NHWC with a channel stride and offset so that a
+ * concatenation is a set of channel slices of one buffer), float32
+ * accumulation on the matrix cores, float32 gradients.  All pointers are
+ * device pointers; work is queued on `stream`. ---- */
+
+/* 3x3 convolution, padding 1, stride 1 or 2.  Weights are (out_channels, KP)
+ * with KP = 9*in_channels rounded up to 16, column k = tap*in_channels + ci,
+ * tap = 3*ky + kx; the padding columns are zero. */
+typedef struct ldic_nn_conv {
+    int32_t batch, height, width;
+    int32_t in_channels, in_stride, in_offset;
+    int32_t out_height, out_width;   /* (height-1)/stride+1, (width-1)/stride+1 */
+    int32_t out_channels, out_stride, out_offset;
+    int32_t stride;
+} ldic_nn_conv;
+
+/* y = act(conv(x) + bias): bfloat16 into y, or float32 into y_f32 when y_f32
+ * is not NULL; leaky != 0 applies the leaky ReLU (slope 0.1). */
+int ldic_nn_conv_forward(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
+                         int32_t leaky, uint16_t* y, float* y_f32, void* stream);
+/* Gradients from dy (float32, layout of the output, taken after the
+ * activation's backward): dx (nullable; float32 with its own channel stride
+ * and offset, overwritten or accumulated) needs the transposed weights; dw
+ * and db (nullable) are accumulated. */
+int ldic_nn_conv_backward(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w_transposed, const float* dy,
+                          float* dx, int32_t dx_stride, int32_t dx_offset, int32_t accumulate, float* dw, float* db,
+                          void* stream);
+/* w_transposed is (in_channels, KPT): column tap*out_channels + co holds
+ * w[co][tap*in_channels + ci]; KPT = 9*out_channels rounded up to 16. */
+int ldic_nn_transpose_weights(int32_t in_channels, int32_t out_channels, const uint16_t* w, uint16_t* w_transposed,
+                              void* stream);
+/* dy *= (y > 0 ? 1 : 0.1) over `count` elements of identical layout. */
+int ldic_nn_leaky_backward(size_t count, const uint16_t* y, float* dy, void* stream);
+
+/* Local correlation out[p][d] = scale * sum_c f1[p][c] f2[p+d][c] over the
+ * (2 radius + 1)^2 offsets d (dy outer), zero outside the image. */
+typedef struct ldic_nn_corr {
+    int32_t batch, height, width, channels, radius;
+    int32_t f1_stride, f1_offset, f2_stride, f2_offset, out_stride, out_offset;
+    float scale;
+} ldic_nn_corr;
+int ldic_nn_corr_forward(const ldic_nn_corr* corr, const uint16_t* f1, const uint16_t* f2, uint16_t* out,
+                         void* stream);
+/* Accumulates the gradients of f1 and f2 (either nullable) from dc. */
+int ldic_nn_corr_backward(const ldic_nn_corr* corr, const uint16_t* f1, const uint16_t* f2, const float* dc,
+                          int32_t dc_stride, int32_t dc_offset, float* df1, int32_t df1_stride, int32_t df1_offset,
+                          float* df2, int32_t df2_stride, int32_t df2_offset, void* stream);
+
+/* Bilinear warp of f (dense NHWC) by a float32 flow (pixels of f), zero
+ * outside; the backward accumulates into d_f only (the flow is a constant). */
+int ldic_nn_warp_forward(int32_t batch, int32_t height, int32_t width, int32_t channels, const uint16_t* f,
+                         const float* flow, uint16_t* out, void* stream);
+int ldic_nn_warp_backward(int32_t batch, int32_t height, int32_t width, int32_t channels, const float* flow,
+                          const float* d_out, float* d_f, void* stream);
+/* fine(x, y) = 2 * bilinear(coarse)(x/2, y/2); optional bfloat16 copy into a
+ * channel slice. */
+int ldic_nn_upsample_flow(int32_t batch, int32_t coarse_height, int32_t coarse_width, int32_t fine_height,
+                          int32_t fine_width, const float* coarse, float* fine, uint16_t* fine_bf16,
+                          int32_t bf16_stride, int32_t bf16_offset, void* stream);
+int ldic_nn_copy_slice(size_t pixels, int32_t channels, const uint16_t* src, int32_t src_stride, int32_t src_offset,
+                       uint16_t* dst, int32_t dst_stride, int32_t dst_offset, void* stream);
+int ldic_nn_add_slice(size_t pixels, int32_t channels, const float* src, int32_t src_stride, int32_t src_offset,
+                      float* dst, int32_t dst_stride, int32_t dst_offset, void* stream);
+/* uint8 -> bfloat16 as (v - 128) / 64; float32 -> bfloat16. */
+int ldic_nn_images_to_bf16(size_t count, const uint8_t* src, uint16_t* dst, void* stream);
+int ldic_nn_to_bf16(size_t count, const float* src, uint16_t* dst, void* stream);
+/* Huber loss (delta 1) of `count` flow vectors: grad = weight/count *
+ * clamp(flow - truth, -1, 1); the loss is added into *loss. */
+int ldic_nn_flow_loss(int32_t count, const float* flow, const float* truth, float weight, float* grad, float* loss,
+                      void* stream);
+/* Adam with bias correction (step >= 1); param_bf16 (nullable) follows. */
+int ldic_nn_adam(size_t count, float* param, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                 float eps, int32_t step, uint16_t* param_bf16, void* stream);
+
+/* Seeds of a subset grid from a flow field (flow pixel j at finest-level
+ * coordinate j * scale): translation at the subset centre, optional
+ * gradients from central differences; active = selected (nullable: all). */
+typedef struct ldic_nn_seed {
+    int32_t grid_width, grid_height, subset_size, subset_stride;
+    int32_t flow_width, flow_height;
+    float scale;
+    int32_t state_stride, gradients;
+    const uint8_t* selected;
+} ldic_nn_seed;
+int ldic_nn_seed_from_flow(const ldic_nn_seed* seed, const float* flow, float* state, uint32_t* active, void* stream);
+
 int ldic_device_alloc(size_t bytes, void** out);
 int ldic_device_free(void* pointer);
 int ldic_copy_to_device(void* device_dst, const void* host_src, size_t bytes);

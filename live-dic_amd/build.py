@@ -24,6 +24,7 @@ SOURCES = (
     "ldic_prep.hip",
     "ldic_render.hip",
     "ldic_seed.hip",
+    "ldic_seednet.hip",
     "ldic_runtime.hip",
 )
 ARCH = os.environ.get("LDIC_OFFLOAD_ARCH", "gfx950")

@@ -22,6 +22,8 @@ struct RenderArgs {
     float sigma, background, noise_sigma;
     unsigned long long seed;
     ldic_field field;
+    const ldic_field* fields;   // optional per-image fields (device), batch index = blockIdx.z
+    const int32_t* offsets;     // optional per-image texture offsets (device, x and y)
     uint8_t* out;
     float* out_float;           // optional noiseless float image
 };
@@ -64,12 +66,16 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     int x = blockIdx.x * 16 + (threadIdx.x & 15);
     int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= a.width || y >= a.height) return;
-    double X = x, Y = y;
-    if (a.field.enabled) {
+    const int b = blockIdx.z;
+    const ldic_field& field = a.fields ? a.fields[b] : a.field;
+    // Texture coordinates of this pixel.
+    const double xt = x + (a.offsets ? a.offsets[2 * b] : 0), yt = y + (a.offsets ? a.offsets[2 * b + 1] : 0);
+    double X = xt, Y = yt;
+    if (field.enabled) {
         for (int it = 0; it < 20; ++it) {
             double u, v, ux, uy, vx, vy;
-            field_eval(a.field, X, Y, u, v, ux, uy, vx, vy);
-            double rx = X + u - x, ry = Y + v - y;
+            field_eval(field, X, Y, u, v, ux, uy, vx, vy);
+            double rx = X + u - xt, ry = Y + v - yt;
             double j11 = 1.0 + ux, j12 = uy, j21 = vx, j22 = 1.0 + vy;
             double det = j11 * j22 - j12 * j21;
             X -= (j22 * rx - j12 * ry) / det;
@@ -96,14 +102,15 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     size_t idx = (size_t)y * a.width + x;
     if (a.out_float) a.out_float[idx] = (float)value;
     if (a.noise_sigma > 0.0f) {
-        unsigned long long h1 = splitmix64(a.seed ^ (idx * 2ull + 1ull));
+        const unsigned long long seed = a.seed + 0x9E3779B97F4A7C15ull * (unsigned long long)b;
+        unsigned long long h1 = splitmix64(seed ^ (idx * 2ull + 1ull));
         unsigned long long h2 = splitmix64(h1 ^ 0xD1B54A32D192ED03ull);
         double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         double u2 = ((double)(h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         value += a.noise_sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
     }
     value = rint(value);
-    a.out[idx] = (uint8_t)fmin(fmax(value, 0.0), 255.0);
+    a.out[(size_t)b * a.width * a.height + idx] = (uint8_t)fmin(fmax(value, 0.0), 255.0);
 }
 
 }  // namespace ldic
@@ -130,6 +137,33 @@ extern "C" int ldic_render_speckle(const ldic_speckle* sp, const ldic_field* fie
     a.out = device_out;
     a.out_float = device_float_out;
     dim3 grid((width + 15) / 16, (height + 15) / 16);
+    ldic::k_render<<<grid, 256, 0, (hipStream_t)stream>>>(a);
+    return hipGetLastError() == hipSuccess ? LDIC_OK : LDIC_ERR_DEVICE;
+}
+
+extern "C" int ldic_render_speckle_batch(const ldic_speckle* sp, const ldic_field* device_fields,
+                                         const int32_t* device_offsets, int32_t count, int32_t width, int32_t height,
+                                         float noise_sigma, uint64_t noise_seed, uint8_t* device_out, void* stream) {
+    if (!sp || !device_out || width < 1 || height < 1 || count < 1 || count > 65535 || sp->cell < 1)
+        return LDIC_ERR_ARGUMENT;
+    ldic::RenderArgs a{};
+    a.width = width;
+    a.height = height;
+    a.cells_x = sp->cells_x;
+    a.cells_y = sp->cells_y;
+    a.cell = sp->cell;
+    a.cell_origin = sp->cell_origin;
+    a.blobs = reinterpret_cast<const float4*>(sp->device_blobs);
+    a.cell_start = sp->device_cell_start;
+    a.sigma = sp->sigma;
+    a.background = sp->background;
+    a.noise_sigma = noise_sigma;
+    a.seed = noise_seed;
+    a.field.enabled = 0;
+    a.fields = device_fields;
+    a.offsets = device_offsets;
+    a.out = device_out;
+    dim3 grid((width + 15) / 16, (height + 15) / 16, count);
     ldic::k_render<<<grid, 256, 0, (hipStream_t)stream>>>(a);
     return hipGetLastError() == hipSuccess ? LDIC_OK : LDIC_ERR_DEVICE;
 }

@@ -1065,6 +1065,23 @@ int ldic_op_reduce(const uint8_t* src, int32_t w, int32_t h, uint8_t* dst) {
     return LDIC_OK;
 }
 
+int ldic_reduce_device(const uint8_t* device_src, int32_t width, int32_t height, int32_t count, uint8_t* device_dst,
+                       void* stream) {
+    if (!device_src || !device_dst || width < 2 || height < 2 || count < 1)
+        return set_error(LDIC_ERR_ARGUMENT, "invalid reduction arguments");
+    const size_t in = (size_t)width * height, out = (size_t)(width / 2) * (height / 2);
+    for (int b = 0; b < count; ++b)
+        HIP_CHECK(launch_reduce(device_src + b * in, width, height, device_dst + b * out, (hipStream_t)stream));
+    return LDIC_OK;
+}
+
+int ldic_slot_stream(ldic_engine* e, int32_t slot, void** stream) {
+    CHECK(check_slot(e, slot));
+    if (!stream) return set_error(LDIC_ERR_ARGUMENT, "stream is NULL");
+    *stream = e->slots[slot]->stream;
+    return LDIC_OK;
+}
+
 int ldic_op_zncc_search(const uint8_t* reference, const uint8_t* target, int32_t width, int32_t height,
                         int32_t subset_size, int32_t subset_stride, int32_t radius, const float* prior,
                         int32_t prior_stride, const uint8_t* mask, float* shift, float* score) {

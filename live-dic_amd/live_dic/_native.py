@@ -87,6 +87,25 @@ class Initialization(ctypes.Structure):
     ]
 
 
+class NNConv(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "batch", "height", "width", "in_channels", "in_stride", "in_offset", "out_height", "out_width",
+        "out_channels", "out_stride", "out_offset", "stride")]
+
+
+class NNCorr(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "batch", "height", "width", "channels", "radius", "f1_stride", "f1_offset", "f2_stride", "f2_offset",
+        "out_stride", "out_offset")] + [("scale", ctypes.c_float)]
+
+
+class NNSeed(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "grid_width", "grid_height", "subset_size", "subset_stride", "flow_width", "flow_height")] + [
+        ("scale", ctypes.c_float), ("state_stride", ctypes.c_int32), ("gradients", ctypes.c_int32),
+        ("selected", ctypes.c_void_p)]
+
+
 INIT_ZERO = 0
 INIT_ZNCC = 1
 
@@ -125,6 +144,29 @@ SIGNATURES = {
     "ldic_op_prefilter": (ctypes.c_int, [_P, _I32, _I32, _P]),
     "ldic_render_speckle": (ctypes.c_int, [ctypes.POINTER(Speckle), ctypes.POINTER(Field), _I32, _I32,
                                            ctypes.c_float, ctypes.c_uint64, _P, _P, _P]),
+    "ldic_render_speckle_batch": (ctypes.c_int, [ctypes.POINTER(Speckle), _P, _P, _I32, _I32, _I32,
+                                                 ctypes.c_float, ctypes.c_uint64, _P, _P]),
+    "ldic_reduce_device": (ctypes.c_int, [_P, _I32, _I32, _I32, _P, _P]),
+    "ldic_slot_stream": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
+    "ldic_nn_conv_forward": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _I32, _P, _P, _P]),
+    "ldic_nn_conv_backward": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _P, _I32, _I32, _I32, _P, _P,
+                                             _P]),
+    "ldic_nn_transpose_weights": (ctypes.c_int, [_I32, _I32, _P, _P, _P]),
+    "ldic_nn_leaky_backward": (ctypes.c_int, [ctypes.c_size_t, _P, _P, _P]),
+    "ldic_nn_corr_forward": (ctypes.c_int, [ctypes.POINTER(NNCorr), _P, _P, _P, _P]),
+    "ldic_nn_corr_backward": (ctypes.c_int, [ctypes.POINTER(NNCorr), _P, _P, _P, _I32, _I32, _P, _I32, _I32, _P,
+                                             _I32, _I32, _P]),
+    "ldic_nn_warp_forward": (ctypes.c_int, [_I32, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "ldic_nn_warp_backward": (ctypes.c_int, [_I32, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "ldic_nn_upsample_flow": (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _P]),
+    "ldic_nn_copy_slice": (ctypes.c_int, [ctypes.c_size_t, _I32, _P, _I32, _I32, _P, _I32, _I32, _P]),
+    "ldic_nn_add_slice": (ctypes.c_int, [ctypes.c_size_t, _I32, _P, _I32, _I32, _P, _I32, _I32, _P]),
+    "ldic_nn_images_to_bf16": (ctypes.c_int, [ctypes.c_size_t, _P, _P, _P]),
+    "ldic_nn_to_bf16": (ctypes.c_int, [ctypes.c_size_t, _P, _P, _P]),
+    "ldic_nn_flow_loss": (ctypes.c_int, [_I32, _P, _P, ctypes.c_float, _P, _P, _P]),
+    "ldic_nn_adam": (ctypes.c_int, [ctypes.c_size_t, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, _I32, _P, _P]),
+    "ldic_nn_seed_from_flow": (ctypes.c_int, [ctypes.POINTER(NNSeed), _P, _P, _P, _P]),
     "ldic_device_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     "ldic_device_free": (ctypes.c_int, [_P]),
     "ldic_copy_to_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
