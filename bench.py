@@ -77,6 +77,182 @@ def cpu_baseline(seq, frame_index: int, budget_s: float = 25.0) -> dict:
     }
 
 
+# Learned seed vs ZNCC seed at large displacement: 3840x2160 pairs, translation U[-64, 64] px,
+# rotation U[-10, 10] deg, normal strains U[-3 %, 3 %] about the frame centre, noise 2 grey.
+LARGE_PAIRS = 6
+LARGE_LEVELS = 4                       # ZNCC path: 1/8-resolution search covers +-512 px
+LARGE_SEARCH_RADIUS = 504              # finest-level pixels (coarsest-level radius 64)
+LARGE_ITERATIONS = (12, 8, 8, 16)
+
+
+def _large_targets(count: int, seed: int = 5150):
+    """One reference and `count` device-resident targets of the large-displacement workload,
+    each under its own random affine field."""
+    from live_dic import _native, seednet, synthetic
+
+    rng = np.random.default_rng(seed)
+    tex = synthetic.SpeckleTexture(WIDTH, HEIGHT, synthetic.blobs(
+        WIDTH, HEIGHT, int(round(synthetic.BLOB_DENSITY * WIDTH * HEIGHT)), seed))
+    ref = _native.DeviceBuffer(WIDTH * HEIGHT)
+    tex.render_device(ref, None, 2.0, 1)
+    host = np.empty((HEIGHT, WIDTH), np.uint8)
+    ref.download(host)
+    targets = []
+    for k in range(count):
+        fld = seednet.random_affine(rng)
+        tgt = _native.DeviceBuffer(WIDTH * HEIGHT)
+        tex.render_device(tgt, fld, 2.0, 2 * k + 2)
+        targets.append((tgt, fld))
+    return ref, host, targets
+
+
+def _accuracy(states, targets) -> dict:
+    """Over subsets whose deformed window stays in the frame: accepted fraction, and over the
+    accepted ones the RMSE, median and 99th percentile of the displacement error (px) and the
+    fraction within 0.01 px."""
+    from live_dic import synthetic
+
+    cx, cy = synthetic.subset_centres(WIDTH, HEIGHT, SUBSET, STRIDE)
+    acc, errs = [], []
+    for st, (_, fld) in zip(states, targets):
+        u, v = fld.displacement(cx, cy)
+        inside = (cx + u > SUBSET) & (cx + u < WIDTH - SUBSET) & (cy + v > SUBSET) & (cy + v < HEIGHT - SUBSET)
+        ok = (st[..., 6] > 0.5) & inside
+        acc.append(ok.sum() / max(inside.sum(), 1))
+        errs.append(np.hypot(st[..., 0] - u, st[..., 3] - v)[ok])
+    e = np.concatenate(errs)
+    return {"accepted_fraction": float(np.mean(acc)), "rmse_px": float(np.sqrt(np.mean(e * e))),
+            "median_px": float(np.median(e)), "p99_px": float(np.percentile(e, 99)),
+            "within_0.01px": float(np.mean(e <= 0.01)), "over_1px": int(np.sum(e > 1.0))}
+
+
+def learned_vs_zncc(reps: int = 3) -> dict:
+    """Throughput and accuracy of the two seeding paths on the same fixed reference and targets."""
+    from live_dic import ICGNWGPU, ICGNWGPUFullPyramid, _native, seednet
+
+    if not seednet.WEIGHTS_PATH.exists():
+        return {"skipped": f"no trained weights at {seednet.WEIGHTS_PATH.relative_to(ROOT)}"}
+    lib = _native.library()
+    ref, host, targets = _large_targets(LARGE_PAIRS)
+    result = {"workload": f"3840x2160 reference and {LARGE_PAIRS} targets, each t U[-64,64] px, rotation "
+                          "U[-10,10] deg, strains U[-3%,3%] about the centre, noise 2; affine IC-GN subset 31 "
+                          "step 8; reference prepared once; targets resident in HBM"}
+
+    def timed(submit, handle, out_shape):
+        states = []
+        for tgt, _ in targets:                          # warm-up and accuracy
+            submit(tgt)
+            _native.check(lib.ldic_slot_wait(handle, 0, None))
+            st = np.empty(out_shape, np.float32)
+            _native.check(lib.ldic_slot_output(handle, 0, st.ctypes.data))
+            states.append(st)
+        _native.check(lib.ldic_device_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for tgt, _ in targets:
+                submit(tgt)
+        _native.check(lib.ldic_slot_wait(handle, 0, None))
+        _native.check(lib.ldic_device_synchronize())
+        rate = reps * len(targets) / (time.perf_counter() - t0)
+        return {"frames_per_s": rate, **_accuracy(states, targets)}
+
+    zeng = ICGNWGPUFullPyramid(pyramid_levels=LARGE_LEVELS, iterations_per_level=LARGE_ITERATIONS,
+                               subset_size=SUBSET, subset_stride=STRIDE, initialization="zncc",
+                               seed_search_radius=LARGE_SEARCH_RADIUS)
+    info = zeng.set_reference(host)
+    gw, gh = info.grid
+    zncc = timed(lambda t: _native.check(lib.ldic_register_resident(zeng._handle, 0, t.address, WIDTH)),
+                 zeng._handle, (gh, gw, 8))
+    zeng.close()
+
+    leng = ICGNWGPU(subset_size=SUBSET, subset_stride=STRIDE)
+    leng.set_reference(host, iterations=20, tolerance=1e-3)
+    net = seednet.SeedNet()
+    net.load(seednet.WEIGHTS_PATH)
+    seeder = seednet.LearnedSeeder(leng, net)
+    seeder.set_reference(ref.address)
+    learned = timed(lambda t: seeder.submit(t.address), leng._handle, (gh, gw, 8))
+    leng.close()
+    zncc["path"] = f"{LARGE_LEVELS}-level pyramid {list(LARGE_ITERATIONS)}, coarsest-level ZNCC search " \
+                   f"+-{LARGE_SEARCH_RADIUS // 8 + 1} px"
+    learned.update({"path": "seed network (BF16 MFMA) + finest-level IC-GN (20 iterations), ZNCC fallback below 0.9",
+                    "network_parameters": net.arch.parameter_count(),
+                    "network_gmac_per_frame": net.arch.macs(HEIGHT >> 2, WIDTH >> 2) / 1e9,
+                    "network_forward_ms": seeder.tiles.get("forward_ms")})
+    result.update({"zncc": zncc, "learned": learned, "speedup": learned["frames_per_s"] / zncc["frames_per_s"]})
+    return result
+
+
+# Batched heterogeneous workload: 16 pairs of mixed sizes, subset sizes, steps and warp models,
+# random affine fields (t U[-5, 5] px, strain U[-1 %, 1 %], rotation U[-1, 1] deg).
+BATCH_SIZES = [(640, 480)] * 4 + [(1280, 720)] * 4 + [(1920, 1080)] * 4 + [(3840, 2160)] * 4
+BATCH_LEVELS = {640: 2, 1280: 3, 1920: 3, 3840: 4}
+
+
+def batched_heterogeneous(frames: int = 20) -> dict:
+    """Aggregate subsets/s of the 16-pair batch replayed as one HIP graph per frame, and the
+    p50/p99 latency of a frame (every pair of a frame completes with it)."""
+    from live_dic import ICGNWGPUFullPyramid, _native, seednet, synthetic
+    from live_dic.batch import BatchRegistration
+
+    rng = np.random.default_rng(4242)
+    engines, targets, fields, configs = [], [], [], []
+    for i, (w, h) in enumerate(BATCH_SIZES):
+        S = (15, 21, 31, 41)[i % 4]
+        D = min((4, 8, 16)[i % 3], S)
+        model = "quadratic" if i % 2 else "affine"
+        fld = seednet.random_affine(rng, w, h, translation=5.0, rotation_deg=1.0, strain=0.01)
+        tex = synthetic.SpeckleTexture(w, h, synthetic.blobs(w, h, int(round(synthetic.BLOB_DENSITY * w * h)), 77 + i))
+        ref = tex.render(None, noise_sigma=1.0, noise_seed=2 * i + 1)
+        tgt = _native.DeviceBuffer(w * h)
+        tex.render_device(tgt, fld, 1.0, 2 * i + 2)
+        eng = ICGNWGPUFullPyramid(pyramid_levels=BATCH_LEVELS[w], subset_size=S, subset_stride=D, motion_model=model)
+        eng.set_reference(ref)
+        engines.append(eng)
+        targets.append(tgt)
+        fields.append(fld)
+        configs.append(f"{w}x{h} S{S} D{D} {model} L{BATCH_LEVELS[w]}")
+    subsets = sum(e.grid_width * e.grid_height for e in engines)
+    with BatchRegistration(engines, [t.address for t in targets]) as batch:
+        table = batch.tune()
+        batch.launch(use_graph=True)
+        batch.wait()
+        lat = []
+        for _ in range(frames):
+            batch.launch(use_graph=True)
+            lat.append(batch.wait())
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            batch.launch(use_graph=True)
+        batch.wait()
+        per_frame = (time.perf_counter() - t0) / frames
+        outs = batch.outputs()
+    medians, accepted = [], []
+    for e, st, fld in zip(engines, outs, fields):
+        w, h = e._level_shapes[0][:2]
+        cx, cy = synthetic.subset_centres(w, h, e.subset_size, e.subset_stride)
+        u, v = fld.displacement(cx, cy)
+        P = 12 if e.motion_model == "quadratic" else 6
+        ok = st[..., P] > 0.5
+        accepted.append(float(ok.mean()))
+        medians.append(float(np.median(np.hypot(st[..., 0] - u, st[..., 3] - v)[ok])))
+    for e in engines:
+        e.close()
+    return {
+        "workload": "16 pairs: 4x each of 640x480, 1280x720, 1920x1080, 3840x2160; subsets 15/21/31/41, steps "
+                    "4/8/16, affine and quadratic; random affine fields; one HIP graph per frame",
+        "pairs": configs,
+        "subsets_per_frame": subsets,
+        "subsets_per_s": subsets / per_frame,
+        "frame_ms": per_frame * 1e3,
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "accepted_fraction_min": min(accepted),
+        "median_error_px_max": max(medians),
+        "persistent_launches": table,
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -84,6 +260,8 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=10)
     parser.add_argument("--drain", type=int, default=32, help="frames queued between host waits")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-learned", action="store_true", help="skip the learned-vs-ZNCC seeding comparison")
+    parser.add_argument("--no-batch", action="store_true", help="skip the batched heterogeneous workload")
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="only copy 256 MiB with one plain device copy kernel (PMC byte-counter calibration)")
     args = parser.parse_args()
@@ -200,6 +378,12 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_learned:
+        print("[bench] learned vs ZNCC seeding at large displacement", file=sys.stderr, flush=True)
+        line["learned_vs_zncc"] = learned_vs_zncc()
+    if not args.no_batch:
+        print("[bench] batched heterogeneous pairs", file=sys.stderr, flush=True)
+        line["batched_heterogeneous"] = batched_heterogeneous()
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(seq, 30)

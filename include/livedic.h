@@ -159,6 +159,14 @@ int ldic_set_initialization(struct ldic_engine* engine, const ldic_initializatio
 int ldic_register_seeded_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_target,
                                   int64_t row_stride, const float* device_seed_state, const uint32_t* device_active);
 
+/* Queues the preparation of a new reference (device plane) on a slot's stream,
+ * in order with that slot's resident registrations: the per-pair reference of
+ * seeded consecutive-pair registration.  Same preparation as
+ * ldic_set_reference (full_pyramid_backend.py:1135-1368); dimensions, options,
+ * ROI tables and level configurations of the last ldic_set_reference are kept. */
+int ldic_prepare_reference_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_reference,
+                                    int64_t row_stride);
+
 /* The last search of a slot's level: shift (grid*2, level pixels) and ZNCC
  * score (grid; -2 where no shift had a valid window, NaN where skipped). */
 int ldic_export_search(struct ldic_engine* engine, int32_t slot, int32_t level, float* shift, float* score);
@@ -169,6 +177,32 @@ int ldic_export_search(struct ldic_engine* engine, int32_t slot, int32_t level, 
 int ldic_op_zncc_search(const uint8_t* reference, const uint8_t* target, int32_t width, int32_t height,
                         int32_t subset_size, int32_t subset_stride, int32_t radius, const float* prior,
                         int32_t prior_stride, const uint8_t* mask, float* shift, float* score);
+
+/* ---- Batched heterogeneous registration (no counterpart in the reference,
+ * which registers one pair per engine).  Each engine, prepared with
+ * ldic_set_reference, contributes one pair: its own size, subset size, step,
+ * warp model and pyramid depth.  A frame of the batch runs every pair's
+ * sequence (full_pyramid_backend.py:1436-1569) on one stream, levels aligned on
+ * the finest and processed coarse to fine; the solves of all pairs that share
+ * a solver variant run in one persistent launch whose grid size is picked by
+ * timing candidates on the device.  The targets are device planes (row pitch =
+ * width) read at every launch; results stay in each engine's slot 0 and are
+ * read with ldic_slot_output after ldic_batch_wait.  The engines must not be
+ * used otherwise while the batch exists. ---- */
+typedef struct ldic_batch ldic_batch;
+int ldic_batch_create(struct ldic_engine* const* engines, int32_t count, const uint8_t* const* device_targets,
+                      ldic_batch** out_batch);
+/* Runs one frame eagerly, times the grid candidates of every persistent solve,
+ * keeps the fastest and captures one frame into a HIP graph. */
+int ldic_batch_tune(ldic_batch* batch);
+/* Queues one frame: the graph (use_graph != 0, after ldic_batch_tune) or the
+ * same launches issued one by one. */
+int ldic_batch_launch(ldic_batch* batch, int32_t use_graph);
+/* Waits for the queued frames; gpu_ms (nullable) is the device time of the last one. */
+int ldic_batch_wait(ldic_batch* batch, double* gpu_ms);
+/* JSON description of the persistent launches and the autotuner's timings. */
+int ldic_batch_describe(ldic_batch* batch, char* buffer, int32_t capacity);
+void ldic_batch_destroy(ldic_batch* batch);
 
 /* Resident frames for throughput measurement: the target is already in device
  * memory (row pitch row_stride bytes).  Queues the same per-frame work on the
@@ -263,6 +297,7 @@ typedef struct ldic_nn_conv {
     int32_t out_height, out_width;   /* (height-1)/stride+1, (width-1)/stride+1 */
     int32_t out_channels, out_stride, out_offset;
     int32_t stride;
+    int32_t out_channel_tile;        /* forward: output channels per workgroup, 32, 64 or 128; 0 picks */
 } ldic_nn_conv;
 
 /* y = act(conv(x) + bias): bfloat16 into y, or float32 into y_f32 when y_f32

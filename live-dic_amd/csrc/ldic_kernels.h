@@ -9,6 +9,8 @@ enum Model : int { kAffine = 0, kTranslation = 1, kQuadratic = 2 };
 enum Interp : int { kBspline = 0, kLinear = 1, kKeys = 2 };
 enum Policy : int { kFlag = 0, kReplace = 1, kDisabled = 2, kMedian = 3 };
 
+constexpr int kWavesPerBlock = 4;   // subsets (waves) per solver workgroup
+
 inline int param_count(int model) { return model == kQuadratic ? 12 : 6; }
 inline int state_stride(int model) { return model == kQuadratic ? 16 : 8; }
 inline int metadata_stride(int model) { return model == kQuadratic ? 28 : 16; }
@@ -96,6 +98,20 @@ size_t solver_lds_bytes(int S, int* tile_cap);
 hipError_t launch_reduce(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
 hipError_t launch_prefilter(const uint8_t* src, int w, int h, const float* fir31, float* coeff, hipStream_t s);
 hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a, hipStream_t s);
+
+// One persistent launch over the subset grids of several pairs that share
+// model, interpolant, register bucket (ppl) and weighting.  pairs and
+// block_start (npairs + 1 prefix sums of ceil(grid / 4)) are device arrays;
+// wave_stride is the LDS floats per wave (the largest tile_cap of the pairs).
+struct BatchLaunch {
+    int model, interp, weighted, ppl;
+    const SolveArgs* pairs;
+    const int* block_start;
+    int npairs, total, grid;
+    size_t lds;
+    int wave_stride;
+};
+hipError_t launch_solve_batched(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s);
 hipError_t launch_median(const MedianArgs& a, hipStream_t s);
 hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);

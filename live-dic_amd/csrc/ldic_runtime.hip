@@ -7,6 +7,7 @@
 // outlier buffers plus pinned host staging.  A frame performs no allocation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -968,6 +969,26 @@ int ldic_register_seeded_resident(ldic_engine* e, int32_t slot, const uint8_t* d
     return LDIC_OK;
 }
 
+int ldic_prepare_reference_resident(ldic_engine* e, int32_t slot, const uint8_t* device_reference,
+                                    int64_t row_stride) {
+    CHECK(check_slot(e, slot));
+    if (!device_reference) return set_error(LDIC_ERR_ARGUMENT, "device_reference is NULL");
+    Slot* s = e->slots[slot];
+    if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
+    HIP_CHECK(hipSetDevice(e->device));
+    const Level& L0 = e->lv[0];
+    if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, device_reference, (size_t)row_stride, L0.width, L0.height,
+                               hipMemcpyDeviceToDevice, s->stream));
+    CHECK(enqueue_reference(e, s->stream));
+    HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
+    s->pending = true;
+    s->resident = true;
+    s->submitted_at = now_ms();
+    return LDIC_OK;
+}
+
 int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
     if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
     CHECK(validate_initialization(e, init));
@@ -1154,6 +1175,334 @@ int ldic_op_prefilter(const uint8_t* src, int32_t w, int32_t h, float* coeff) {
     if (rc != LDIC_OK) return rc;
     if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "prefilter failed: %s", hipGetErrorString(he));
     return LDIC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched heterogeneous registration (no counterpart in the reference, which
+// registers one pair per engine).  Several engines (one image pair each: own
+// size, subset size, step, warp model and pyramid depth) share one frame
+// sequence on one stream: levels are aligned on the finest one and processed
+// coarse to fine; at each level the solves of all pairs with the same solver
+// variant run in one persistent launch (k_solve_batched), whose grid size is
+// chosen by timing candidates on the device (the autotuner).  The whole frame
+// is then captured once into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct ldic_batch {
+    struct Group {
+        BatchLaunch launch{};
+        std::vector<int> members;
+        std::vector<int> candidates;   // grid sizes tried by the autotuner
+        std::vector<float> timings;
+    };
+    std::vector<ldic_engine*> engines;
+    std::vector<const uint8_t*> targets;
+    int device = 0;
+    int max_levels = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<std::vector<Group>> steps;  // steps[j]: groups at finest-aligned level j
+    Arena arena;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool launched = false;
+};
+
+namespace {
+
+int batch_pre_solve(ldic_batch* b, int p, int k, const float** seed, const uint32_t** active) {
+    ldic_engine* e = b->engines[p];
+    Slot* s = e->slots[0];
+    SlotLevel& sl = s->lv[k];
+    const Level& lev = e->lv[k];
+    hipStream_t st = b->stream;
+    *seed = nullptr;
+    *active = nullptr;
+    if (k < e->levels - 1) {
+        const Level& src = e->lv[k + 1];
+        SeedArgs sa{};
+        sa.src_grid_w = src.gw;
+        sa.src_grid_h = src.gh;
+        sa.dst_grid_w = lev.gw;
+        sa.dst_count = lev.count();
+        sa.src_scale = (float)(1 << (k + 1));
+        sa.dst_scale = (float)(1 << k);
+        sa.S = (float)e->opt.subset_size;
+        sa.D = (float)e->opt.subset_stride;
+        sa.model = e->opt.motion_model;
+        sa.track_availability = 0;
+        sa.src_state = level_output(e, s->lv[k + 1]);
+        sa.permissions = lev.seed_links;
+        sa.dst_seed = sl.seed;
+        sa.dst_active = sl.active;
+        HIP_CHECK(launch_transfer_seed(sa, st));
+        *seed = sl.seed;
+        *active = sl.active;
+    } else if (e->init.kind == LDIC_INIT_ZNCC) {
+        SearchArgs q = search_args(e, k, sl, e->init.search_radius);
+        q.mask = lev.roi ? lev.selected : nullptr;
+        q.seed = sl.seed;
+        q.active = sl.active;
+        HIP_CHECK(launch_zncc_search(q, st));
+        *seed = sl.seed;
+        *active = sl.active;
+    }
+    if (e->opt.interpolation == kBspline) HIP_CHECK(launch_prefilter(sl.target, lev.width, lev.height, e->fir, sl.coeff, st));
+    return LDIC_OK;
+}
+
+// One frame of the batch, queued on its stream (also what the graph captures).
+int batch_enqueue(ldic_batch* b) {
+    hipStream_t st = b->stream;
+    for (size_t p = 0; p < b->engines.size(); ++p) {
+        ldic_engine* e = b->engines[p];
+        const Level& L0 = e->lv[0];
+        Slot* s = e->slots[0];
+        HIP_CHECK(hipMemcpy2DAsync(s->lv[0].target, L0.width, b->targets[p], L0.width, L0.width, L0.height,
+                                   hipMemcpyDeviceToDevice, st));
+        for (int k = 1; k < e->levels; ++k)
+            HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, st));
+    }
+    for (int j = b->max_levels - 1; j >= 0; --j) {
+        for (size_t p = 0; p < b->engines.size(); ++p) {
+            if (b->engines[p]->levels <= j) continue;
+            const float* seed;
+            const uint32_t* active;
+            CHECK(batch_pre_solve(b, (int)p, j, &seed, &active));
+        }
+        for (const ldic_batch::Group& g : b->steps[j]) HIP_CHECK(launch_solve_batched(g.launch, st));
+        for (size_t p = 0; p < b->engines.size(); ++p) {
+            ldic_engine* e = b->engines[p];
+            if (e->levels <= j) continue;
+            CHECK(launch_median_level(e, j, e->slots[0]->lv[j], e->opt.outlier_policy == kMedian && j > 0, st));
+        }
+    }
+    return LDIC_OK;
+}
+
+int batch_build(ldic_batch* b) {
+    b->max_levels = 0;
+    for (ldic_engine* e : b->engines) b->max_levels = std::max(b->max_levels, e->levels);
+    b->steps.assign(b->max_levels, {});
+    for (int j = 0; j < b->max_levels; ++j) {
+        // Group the pairs present at level j by solver variant.
+        std::vector<ldic_batch::Group>& groups = b->steps[j];
+        for (size_t p = 0; p < b->engines.size(); ++p) {
+            ldic_engine* e = b->engines[p];
+            if (e->levels <= j) continue;
+            const int ppl = solver_pixels_per_lane(e->opt.subset_size);
+            ldic_batch::Group* g = nullptr;
+            for (auto& cand : groups) {
+                const BatchLaunch& l = cand.launch;
+                if (l.model == e->opt.motion_model && l.interp == e->opt.interpolation &&
+                    l.weighted == e->opt.spatial_weighting && l.ppl == ppl)
+                    g = &cand;
+            }
+            if (!g) {
+                groups.emplace_back();
+                g = &groups.back();
+                g->launch.model = e->opt.motion_model;
+                g->launch.interp = e->opt.interpolation;
+                g->launch.weighted = e->opt.spatial_weighting;
+                g->launch.ppl = ppl;
+            }
+            g->members.push_back((int)p);
+        }
+        for (ldic_batch::Group& g : groups) {
+            std::vector<SolveArgs> args;
+            std::vector<int> start(1, 0);
+            int cap = 0;
+            for (int p : g.members) {
+                ldic_engine* e = b->engines[p];
+                SlotLevel& sl = e->slots[0]->lv[j];
+                const bool seeded = j < e->levels - 1 || e->init.kind == LDIC_INIT_ZNCC;
+                args.push_back(solve_args(e, j, sl, seeded ? sl.seed : nullptr, seeded ? sl.active : nullptr, 0));
+                start.push_back(start.back() + (e->lv[j].count() + kWavesPerBlock - 1) / kWavesPerBlock);
+                cap = std::max(cap, e->tile_cap);
+            }
+            SolveArgs* d_args = nullptr;
+            int* d_start = nullptr;
+            CHECK(b->arena.alloc(&d_args, args.size()));
+            CHECK(b->arena.alloc(&d_start, start.size()));
+            HIP_CHECK(hipMemcpy(d_args, args.data(), sizeof(SolveArgs) * args.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+            BatchLaunch& l = g.launch;
+            l.pairs = d_args;
+            l.block_start = d_start;
+            l.npairs = (int)args.size();
+            l.total = start.back();
+            l.wave_stride = cap;
+            l.lds = (size_t)cap * sizeof(float) * kWavesPerBlock;
+            l.grid = l.total;
+            g.candidates.clear();
+            const int cus = b->engines[0]->prop.multiProcessorCount;
+            for (int per : {1, 2, 3, 4, 8})
+                if (cus * per < l.total) g.candidates.push_back(cus * per);
+            g.candidates.push_back(l.total);
+        }
+    }
+    return LDIC_OK;
+}
+
+// Times every grid candidate of every group on the device (each solve is
+// idempotent for fixed seeds, so the frame's own buffers are reused) and keeps
+// the fastest.
+int batch_tune(ldic_batch* b) {
+    CHECK(batch_enqueue(b));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (int j = b->max_levels - 1; j >= 0; --j) {
+        for (ldic_batch::Group& g : b->steps[j]) {
+            g.timings.clear();
+            float best = 1e30f;
+            int best_grid = g.launch.total;
+            for (int grid : g.candidates) {
+                BatchLaunch l = g.launch;
+                l.grid = grid;
+                HIP_CHECK(launch_solve_batched(l, b->stream));  // warm
+                HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+                HIP_CHECK(launch_solve_batched(l, b->stream));
+                HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+                HIP_CHECK(hipEventSynchronize(b->ev1));
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                g.timings.push_back(ms);
+                if (ms < best) {
+                    best = ms;
+                    best_grid = grid;
+                }
+            }
+            g.launch.grid = best_grid;
+        }
+    }
+    return LDIC_OK;
+}
+
+void batch_release_graph(ldic_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t* const* device_targets,
+                      ldic_batch** out) {
+    if (!out || !engines || !device_targets || count < 1) return set_error(LDIC_ERR_ARGUMENT, "invalid batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i) {
+        CHECK(check_ready(engines[i]));
+        if (!device_targets[i]) return set_error(LDIC_ERR_ARGUMENT, "device target %d is NULL", i);
+        if (engines[i]->device != engines[0]->device)
+            return set_error(LDIC_ERR_ARGUMENT, "all engines of a batch must use one device");
+        for (int k = 0; k < i; ++k)
+            if (engines[k] == engines[i]) return set_error(LDIC_ERR_ARGUMENT, "engine %d appears twice", i);
+    }
+    ldic_batch* b = new ldic_batch();
+    b->engines.assign(engines, engines + count);
+    b->targets.assign(device_targets, device_targets + count);
+    b->device = engines[0]->device;
+    auto fail = [&](int rc) {
+        ldic_batch_destroy(b);
+        return rc;
+    };
+    hipError_t he = hipSetDevice(b->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreate(&b->ev0);
+    if (he == hipSuccess) he = hipEventCreate(&b->ev1);
+    if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "batch set-up failed: %s", hipGetErrorString(he)));
+    for (ldic_engine* e : b->engines)
+        if (e->slots[0]->pending) return fail(set_error(LDIC_ERR_STATE, "an engine's slot 0 holds a pending frame"));
+    int rc = batch_build(b);
+    if (rc != LDIC_OK) return fail(rc);
+    *out = b;
+    return LDIC_OK;
+}
+
+int ldic_batch_tune(ldic_batch* b) {
+    if (!b) return set_error(LDIC_ERR_ARGUMENT, "batch is NULL");
+    HIP_CHECK(hipSetDevice(b->device));
+    batch_release_graph(b);
+    CHECK(batch_tune(b));
+    // Capture one frame with the chosen grids.
+    HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b);
+    hipGraph_t graph = nullptr;
+    hipError_t he = hipStreamEndCapture(b->stream, &graph);
+    if (rc != LDIC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(he));
+    b->graph = graph;
+    HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return LDIC_OK;
+}
+
+int ldic_batch_launch(ldic_batch* b, int32_t use_graph) {
+    if (!b) return set_error(LDIC_ERR_ARGUMENT, "batch is NULL");
+    if (use_graph && !b->exec) return set_error(LDIC_ERR_STATE, "call ldic_batch_tune before launching the graph");
+    HIP_CHECK(hipSetDevice(b->device));
+    HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+    if (use_graph) {
+        HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        CHECK(batch_enqueue(b));
+    }
+    HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+    b->launched = true;
+    return LDIC_OK;
+}
+
+int ldic_batch_wait(ldic_batch* b, double* gpu_ms) {
+    if (!b) return set_error(LDIC_ERR_ARGUMENT, "batch is NULL");
+    HIP_CHECK(hipSetDevice(b->device));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (gpu_ms) {
+        float ms = 0.0f;
+        if (b->launched) HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        *gpu_ms = ms;
+    }
+    return LDIC_OK;
+}
+
+int ldic_batch_describe(ldic_batch* b, char* buffer, int32_t capacity) {
+    if (!b || !buffer || capacity <= 0) return set_error(LDIC_ERR_ARGUMENT, "invalid buffer");
+    std::string text = "[";
+    for (int j = b->max_levels - 1; j >= 0; --j)
+        for (const ldic_batch::Group& g : b->steps[j]) {
+            char item[512];
+            std::string times;
+            for (size_t i = 0; i < g.timings.size(); ++i) {
+                char t[48];
+                std::snprintf(t, sizeof t, "%s[%d,%.4f]", i ? "," : "", g.candidates[i], g.timings[i]);
+                times += t;
+            }
+            std::snprintf(item, sizeof item,
+                          "%s{\"level\":%d,\"model\":%d,\"ppl\":%d,\"pairs\":%d,\"blocks\":%d,\"grid\":%d,"
+                          "\"candidates_ms\":[%s]}",
+                          text.size() > 1 ? "," : "", j, g.launch.model, g.launch.ppl, g.launch.npairs, g.launch.total,
+                          g.launch.grid, times.c_str());
+            text += item;
+        }
+    text += "]";
+    std::snprintf(buffer, (size_t)capacity, "%s", text.c_str());
+    return (int)text.size() < capacity ? LDIC_OK : set_error(LDIC_ERR_ARGUMENT, "buffer too small (%zu)", text.size());
+}
+
+void ldic_batch_destroy(ldic_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_release_graph(b);
+    b->arena.release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 }  // extern "C"

@@ -79,40 +79,66 @@ __device__ __forceinline__ void pixel_of(int m, int Ho, int Wo, int& n, int& oy,
     ox = r - oy * Wo;
 }
 
-// Forward: each wave owns 16 output pixels and up to 128 output channels.
-template <bool VEC>
+// Forward: a workgroup owns 64 output pixels (16 per wave) and up to 128
+// output channels.  The weights are staged through LDS in chunks of kChunk
+// GEMM columns, shared by the four waves; each wave gathers its own im2col
+// rows straight into the MFMA A registers (all four k-steps of a chunk are
+// issued before the first MFMA that needs them).
+constexpr int kChunk = 64;
+constexpr int kChunkPitch = kChunk + 4;   // bf16 elements per LDS row (breaks bank aliasing)
+
+template <bool VEC, int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t* X, const uint16_t* Wt,
                                                   const float* bias, int leaky, uint16_t* Y, float* Yf) {
+    __shared__ __align__(16) uint16_t Bs[16 * NT * kChunkPitch];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int M = g.batch * g.out_height * g.out_width;
     const int m0 = (blockIdx.x * 4 + wave) * 16;
-    if (m0 >= M) return;
-    const int co_base = blockIdx.y * 128;
-    const int ntiles = min(8, (g.out_channels - co_base + 15) / 16);
+    const int co_base = blockIdx.y * 16 * NT;
+    const int rows = min(16 * NT, g.out_channels - co_base);
+    const int ntiles = (rows + 15) / 16;
     const int row = lane & 15, quad = lane >> 4;
     const int m = m0 + row;
     int n = 0, oy = 0, ox = 0;
     const bool mvalid = m < M;
     if (mvalid) pixel_of(m, g.out_height, g.out_width, n, oy, ox);
     const int KP = (9 * g.in_channels + 15) & ~15;
-    f32x4 acc[8];
+    f32x4 acc[NT];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < KP; k0 += 16) {
-        const int k = k0 + 4 * quad;
-        const s16x4 a = im2col4<VEC>(g, X, n, oy, ox, k, mvalid);
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < KP; k0 += kChunk) {
+        const int kc = min(kChunk, KP - k0);
+        // Stage W[co_base + r][k0 .. k0 + kc) in 8-byte pieces.
+        __syncthreads();
+        for (int i = threadIdx.x; i < rows * (kc / 4); i += 256) {
+            const int r = i / (kc / 4), c4 = i - r * (kc / 4);
+            *reinterpret_cast<s16x4*>(Bs + r * kChunkPitch + 4 * c4) =
+                *reinterpret_cast<const s16x4*>(Wt + (size_t)(co_base + r) * KP + k0 + 4 * c4);
+        }
+        __syncthreads();
+        if (m0 < M) {
+            s16x4 a[kChunk / 16];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            if (t < ntiles) {
-                const int co = co_base + 16 * t + row;
-                s16x4 b = {0, 0, 0, 0};
-                if (co < g.out_channels) b = *reinterpret_cast<const s16x4*>(Wt + (size_t)co * KP + k);
-                acc[t] = mfma16(a, b, acc[t]);
+            for (int s = 0; s < kChunk / 16; ++s)
+                a[s] = s * 16 < kc ? im2col4<VEC>(g, X, n, oy, ox, k0 + 16 * s + 4 * quad, mvalid) : s16x4{0, 0, 0, 0};
+#pragma unroll 1
+            for (int s = 0; s < kChunk / 16; ++s) {
+                if (s * 16 >= kc) break;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (t < ntiles) {
+                        const int r = 16 * t + row;
+                        s16x4 b = {0, 0, 0, 0};
+                        if (r < rows) b = *reinterpret_cast<const s16x4*>(Bs + r * kChunkPitch + 16 * s + 4 * quad);
+                        acc[t] = mfma16(a[s], b, acc[t]);
+                    }
+                }
             }
         }
     }
+    if (m0 >= M) return;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < NT; ++t) {
         if (t >= ntiles) break;
         const int co = co_base + 16 * t + row;
         if (co >= g.out_channels) continue;
@@ -558,11 +584,30 @@ int ldic_nn_conv_forward(const ldic_nn_conv* g, const uint16_t* x, const uint16_
                          int32_t leaky, uint16_t* y, float* y_f32, void* stream) {
     if (check_conv(g) != LDIC_OK || !x || !w || (!y && !y_f32)) return LDIC_ERR_ARGUMENT;
     const int M = g->batch * g->out_height * g->out_width;
-    dim3 grid(blocks(M, 64), (g->out_channels + 127) / 128);
-    if (vec_input(*g))
-        k_conv_fwd<true><<<grid, 256, 0, (hipStream_t)stream>>>(*g, x, w, bias, leaky, y, y_f32);
-    else
-        k_conv_fwd<false><<<grid, 256, 0, (hipStream_t)stream>>>(*g, x, w, bias, leaky, y, y_f32);
+    int tile = g->out_channel_tile;
+    if (tile == 0) {
+        // Enough workgroups to fill the device: narrower channel tiles for small pixel counts.
+        tile = 128;
+        while (tile > 32 && (size_t)blocks(M, 64) * ((g->out_channels + tile - 1) / tile) < 2048) tile /= 2;
+    }
+    if (tile != 32 && tile != 64 && tile != 128) return LDIC_ERR_ARGUMENT;
+    dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile);
+    hipStream_t st = (hipStream_t)stream;
+    const bool vec = vec_input(*g);
+    switch (tile) {
+        case 32:
+            if (vec) k_conv_fwd<true, 2><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            else k_conv_fwd<false, 2><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            break;
+        case 64:
+            if (vec) k_conv_fwd<true, 4><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            else k_conv_fwd<false, 4><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            break;
+        default:
+            if (vec) k_conv_fwd<true, 8><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            else k_conv_fwd<false, 8><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
+            break;
+    }
     return status(hipGetLastError());
 }
 

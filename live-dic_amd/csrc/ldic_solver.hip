@@ -15,7 +15,6 @@
 
 namespace ldic {
 
-constexpr int kWavesPerBlock = 4;
 constexpr float kSixth = 1.0f / 6.0f;
 
 // Cubic B-spline weights, icgn.wgsl:230-240 (the division by 6 becomes a product).
@@ -313,18 +312,11 @@ __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, cons
 // lane, pixel p = lane + 64 k).  PPL == 0: any subset width, reference values
 // are re-read from the plane every iteration.
 template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-    extern __shared__ float lds[];
+__device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subset, const int lane, float* tile) {
     constexpr int P = MODEL == kQuadratic ? 12 : 6;
     constexpr int M = MODEL == kQuadratic ? 28 : 16;
     constexpr int STRIDE = MODEL == kQuadratic ? 16 : 8;
     constexpr int NP = PPL > 0 ? PPL : 1;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int subset = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-    const int N = a.grid_w * a.grid_h;
-    if (subset >= N) return;
-    float* tile = lds + wave * a.tile_cap;
     const float* meta = a.metadata + (size_t)subset * M;
     const float* inv = a.inverse + (size_t)subset * (P * P);
 
@@ -583,6 +575,39 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     }
 }
 
+template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int subset = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    if (subset >= a.grid_w * a.grid_h) return;
+    solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, subset, lane, lds + wave * a.tile_cap);
+}
+
+// Persistent solve over the subset grids of several image pairs (one variant
+// of model, interpolant, register bucket and weighting).  Work item i is a
+// block of kWavesPerBlock subsets of pair p, block_start[p] <= i <
+// block_start[p + 1]; each workgroup strides over the items, the XCD mapping
+// giving every XCD contiguous runs of them.  Waves never synchronise with each
+// other, so a wave moves to its next item as soon as its subset is done.
+template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
+__global__ __launch_bounds__(256) void k_solve_batched(const SolveArgs* __restrict__ pairs,
+                                                       const int* __restrict__ block_start, int npairs, int total,
+                                                       int wave_stride) {
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* tile = lds + wave * wave_stride;
+    for (int item = xcd_block(blockIdx.x, gridDim.x); item < total; item += gridDim.x) {
+        int p = 0;
+        while (p + 1 < npairs && block_start[p + 1] <= item) ++p;
+        const SolveArgs a = pairs[p];
+        const int subset = (item - block_start[p]) * kWavesPerBlock + wave;
+        if (subset < a.grid_w * a.grid_h) solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, subset, lane, tile);
+    }
+}
+
 int solver_pixels_per_lane(int S) {
     int need = (S * S + kWave - 1) / kWave;
     if (need <= 4) return 4;
@@ -648,6 +673,44 @@ static hipError_t launch_i(int interp, int weighted, const SolveArgs& a, hipStre
         case kBspline: return launch_w<MODEL, kBspline>(weighted, a, s);
         case kLinear: return launch_w<MODEL, kLinear>(weighted, a, s);
         case kKeys: return launch_w<MODEL, kKeys>(weighted, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int MODEL, int INTERP, bool WEIGHTED>
+static hipError_t launch_batched_ppl(const BatchLaunch& b, hipStream_t s) {
+    if (b.total <= 0) return hipSuccess;
+    dim3 grid(b.grid), block(256);
+    switch (b.ppl) {
+        case 4: k_solve_batched<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
+        case 8: k_solve_batched<MODEL, INTERP, 8, WEIGHTED, 48><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
+        case 16: k_solve_batched<MODEL, INTERP, 16, WEIGHTED, 48><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
+        case 27: k_solve_batched<MODEL, INTERP, 27, WEIGHTED, 64><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
+        default: k_solve_batched<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
+    }
+    return hipGetLastError();
+}
+
+template <int MODEL, int INTERP>
+static hipError_t launch_batched_w(const BatchLaunch& b, hipStream_t s) {
+    return b.weighted ? launch_batched_ppl<MODEL, INTERP, true>(b, s) : launch_batched_ppl<MODEL, INTERP, false>(b, s);
+}
+
+template <int MODEL>
+static hipError_t launch_batched_i(const BatchLaunch& b, hipStream_t s) {
+    switch (b.interp) {
+        case kBspline: return launch_batched_w<MODEL, kBspline>(b, s);
+        case kLinear: return launch_batched_w<MODEL, kLinear>(b, s);
+        case kKeys: return launch_batched_w<MODEL, kKeys>(b, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_solve_batched(const BatchLaunch& b, hipStream_t s) {
+    switch (b.model) {
+        case kAffine: return launch_batched_i<kAffine>(b, s);
+        case kTranslation: return launch_batched_i<kTranslation>(b, s);
+        case kQuadratic: return launch_batched_i<kQuadratic>(b, s);
     }
     return hipErrorInvalidValue;
 }

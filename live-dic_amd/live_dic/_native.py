@@ -90,7 +90,7 @@ class Initialization(ctypes.Structure):
 class NNConv(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "batch", "height", "width", "in_channels", "in_stride", "in_offset", "out_height", "out_width",
-        "out_channels", "out_stride", "out_offset", "stride")]
+        "out_channels", "out_stride", "out_offset", "stride", "out_channel_tile")]
 
 
 class NNCorr(ctypes.Structure):
@@ -134,6 +134,13 @@ SIGNATURES = {
     "ldic_set_initialization": (ctypes.c_int, [_P, ctypes.POINTER(Initialization)]),
     "ldic_register_seeded_resident": (ctypes.c_int, [_P, _I32, _P, _I64, _P, _P]),
     "ldic_export_search": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
+    "ldic_prepare_reference_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
+    "ldic_batch_create": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(_P)]),
+    "ldic_batch_tune": (ctypes.c_int, [_P]),
+    "ldic_batch_launch": (ctypes.c_int, [_P, _I32]),
+    "ldic_batch_wait": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "ldic_batch_describe": (ctypes.c_int, [_P, ctypes.c_char_p, _I32]),
+    "ldic_batch_destroy": (None, [_P]),
     "ldic_op_zncc_search": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I32, _P, _P, _P]),
     "ldic_slot_wait": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Timing)]),
     "ldic_slot_output": (ctypes.c_int, [_P, _I32, _P]),
@@ -177,10 +184,23 @@ SIGNATURES = {
 _lib = None
 
 
+def _load_hip_runtime_of_torch() -> None:
+    """PyTorch-ROCm bundles its own HIP runtime under the soname of the system one
+    (libamdhip64.so.7).  Whichever is loaded first serves the whole process, and torch
+    only initialises its device on its own build, so when torch is installed it is
+    imported before liblivedic.so is loaded: the library then runs on torch's runtime
+    and the learned seed network can share the process."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def library() -> ctypes.CDLL:
     """Load liblivedic.so once and declare its signatures."""
     global _lib
     if _lib is None:
+        _load_hip_runtime_of_torch()
         if not LIBRARY_PATH.exists():
             raise ICGNError(
                 f"{LIBRARY_PATH} is missing; build it with `python live-dic_amd/build.py` (hipcc, gfx950)"

@@ -262,7 +262,7 @@ class SeedNet:
     # -- layers ----------------------------------------------------------------------------------
     def _conv(self, p, batch, h, w, x, x_stride, x_offset, y, y_stride, y_f32=None):
         g = _native.NNConv(batch, h, w, p["cin"], x_stride, x_offset, (h - 1) // p["stride"] + 1,
-                           (w - 1) // p["stride"] + 1, p["cout"], y_stride, 0, p["stride"])
+                           (w - 1) // p["stride"] + 1, p["cout"], y_stride, 0, p["stride"], p.get("tile", 0))
         _native.check(self.lib.ldic_nn_conv_forward(ctypes.byref(g), _ptr(x), _ptr(p["w16"]), _ptr(p["b"]),
                                                     1 if p["act"] else 0, None if y is None else _ptr(y),
                                                     None if y_f32 is None else _ptr(y_f32), self.stream_handle))
@@ -399,6 +399,39 @@ class SeedNet:
             _native.check(lib.ldic_nn_adam(p["b"].numel(), _ptr(p["b"]), _ptr(p["db"]), _ptr(p["mb"]), _ptr(p["vb"]),
                                            lr, beta1, beta2, eps, self.step_count, None, s))
             _native.check(lib.ldic_nn_transpose_weights(p["cin"], p["cout"], _ptr(p["w16"]), _ptr(p["wt16"]), s))
+
+    def autotune(self, h: int, w: int, n: int = 1, reps: int = 5) -> dict:
+        """Pick each forward convolution's output-channel tile (32, 64, 128 per workgroup) by timing
+        whole forward passes on device-resident inputs of this shape, one layer at a time."""
+        torch = self.torch
+        imgs = torch.randint(0, 256, (2 * n, h, w), dtype=torch.uint8, device=self.device)
+        ref, tgt = _ptr(imgs), _ptr(imgs) + n * h * w
+
+        def timed():
+            with torch.cuda.stream(self._tstream):
+                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                self.forward(ref, tgt, n=n, h=h, w=w)
+                start.record(self._tstream)
+                for _ in range(reps):
+                    self.forward(ref, tgt, n=n, h=h, w=w)
+                end.record(self._tstream)
+                end.synchronize()
+                return start.elapsed_time(end) / reps
+
+        for p in self.params.values():
+            p["tile"] = 0
+        choice = {}
+        for name, p in self.params.items():
+            best = None
+            for tile in (32, 64, 128):
+                p["tile"] = tile
+                ms = timed()
+                if best is None or ms < best[0]:
+                    best = (ms, tile)
+            p["tile"] = best[1]
+            choice[name] = best[1]
+        choice["forward_ms"] = timed()
+        return choice
 
     def output_scale(self) -> int:
         """Finest-level pixels per pixel of the last stage's flow."""
@@ -549,6 +582,8 @@ class LearnedSeeder:
         self.active = torch.zeros(n, dtype=torch.int32, device=dev)
         self.gradients = gradients
         self.scale = net.output_scale()
+        w, h = self.sizes[-1]
+        self.tiles = net.autotune(h, w)
 
     def _reduce(self, device_image: int, levels) -> None:
         lib = self.engine._lib
@@ -561,6 +596,14 @@ class LearnedSeeder:
     def set_reference(self, device_reference: int) -> None:
         """Reduce the (already prepared) reference once; device_reference holds the finest plane."""
         self._reduce(device_reference, self.ref_levels)
+
+    def submit_pair(self, device_reference: int, device_target: int) -> None:
+        """Consecutive-pair mode: queue the reference's preparation (finest level) and its
+        reduction, then the target's registration, all on the slot stream."""
+        _native.check(self.engine._lib.ldic_prepare_reference_resident(self.engine._handle, self.slot,
+                                                                        device_reference, self.width))
+        self.set_reference(device_reference)
+        self.submit(device_target)
 
     def submit(self, device_target: int, row_stride: int | None = None) -> None:
         """Queue reduction, network, seeds and the seeded solve of one resident target."""
