@@ -165,8 +165,9 @@ def learned_vs_zncc(reps: int = 3) -> dict:
                  zeng._handle, (gh, gw, 8))
     zeng.close()
 
+    # The finest-level solve uses the ZNCC path's finest-level controls (16 iterations, 5e-3).
     leng = ICGNWGPU(subset_size=SUBSET, subset_stride=STRIDE)
-    leng.set_reference(host, iterations=20, tolerance=1e-3)
+    leng.set_reference(host, iterations=LARGE_ITERATIONS[-1], tolerance=5e-3)
     net = seednet.SeedNet()
     net.load(seednet.WEIGHTS_PATH)
     seeder = seednet.LearnedSeeder(leng, net)
@@ -175,7 +176,8 @@ def learned_vs_zncc(reps: int = 3) -> dict:
     leng.close()
     zncc["path"] = f"{LARGE_LEVELS}-level pyramid {list(LARGE_ITERATIONS)}, coarsest-level ZNCC search " \
                    f"+-{LARGE_SEARCH_RADIUS // 8 + 1} px"
-    learned.update({"path": "seed network (BF16 MFMA) + finest-level IC-GN (20 iterations), ZNCC fallback below 0.9",
+    learned.update({"path": "seed network (BF16 MFMA) + finest-level IC-GN (16 iterations, tolerance 5e-3), "
+                            "ZNCC fallback (+-8 px) below 0.9",
                     "network_parameters": net.arch.parameter_count(),
                     "network_gmac_per_frame": net.arch.macs(HEIGHT >> 2, WIDTH >> 2) / 1e9,
                     "network_forward_ms": seeder.tiles.get("forward_ms")})

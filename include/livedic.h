@@ -142,8 +142,9 @@ enum { LDIC_INIT_ZERO = 0, LDIC_INIT_ZNCC = 1 };
 typedef struct ldic_initialization {
     int32_t kind;            /* LDIC_INIT_ZERO or LDIC_INIT_ZNCC (coarsest level searched over +-search_radius) */
     int32_t search_radius;   /* coarsest-level pixels, 0..64 */
-    float fallback_zncc;     /* seeded registration: subsets that do not converge or whose final affine ZNCC is
-                                below this are searched again around their seed and re-solved; <= -1 disables */
+    float fallback_zncc;     /* seeded registration: subsets that do not converge, or whose ZNCC in the last
+                                solver iteration is below this, are searched again around their seed and
+                                re-solved; <= -1 disables */
     int32_t fallback_radius; /* finest-level pixels around the seed, 0..64 */
 } ldic_initialization;
 
@@ -304,6 +305,12 @@ typedef struct ldic_nn_conv {
  * is not NULL; leaky != 0 applies the leaky ReLU (slope 0.1). */
 int ldic_nn_conv_forward(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
                          int32_t leaky, uint16_t* y, float* y_f32, void* stream);
+/* The same with the GEMM columns split over `splits` workgroup slices (for
+ * layers with few output pixels): partial sums go to `workspace` (batch *
+ * out_height * out_width * out_channels floats), then bias and activation. */
+int ldic_nn_conv_forward_split(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
+                               int32_t leaky, uint16_t* y, float* y_f32, float* workspace, int32_t splits,
+                               void* stream);
 /* Gradients from dy (float32, layout of the output, taken after the
  * activation's backward): dx (nullable; float32 with its own channel stride
  * and offset, overwritten or accumulated) needs the transposed weights; dw

@@ -32,6 +32,7 @@ struct SolveArgs {
     const float* seed;      // optional initial state, same layout as the output
     const uint32_t* active; // optional per-subset enable flags
     float* state;
+    float* zncc_out;        // optional: ZNCC of the last iteration (at its pre-update warp), NaN if none
 };
 
 struct SeedArgs {
@@ -71,7 +72,7 @@ struct SearchArgs {
     const uint8_t* mask;       // optional: search only where non-zero
     float* shift;              // grid * 2: best shift with sub-pixel refinement
     float* score;              // grid: its ZNCC, -2 when no shift is valid, NaN when skipped
-    float* seed;               // optional packed state: translation seed (zeros elsewhere)
+    float* seed;               // optional packed state: translation from the search, other terms from the prior
     int seed_stride;
     uint32_t* active;          // optional: 1 where searched, 0 where skipped
 };

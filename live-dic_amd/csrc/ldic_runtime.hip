@@ -399,8 +399,8 @@ ZnccArgs zncc_args(const ldic_engine* e, const SlotLevel& sl, const float* state
 
 // Seeded finest-level registration (solve_seeded, icgn.wgsl:1001-1008) from
 // the seeds in lv[0].seed / lv[0].active, with the optional ZNCC fallback:
-// subsets that did not converge, or whose final affine ZNCC is below
-// init.fallback_zncc, are searched again around their seed and re-solved from
+// subsets that did not converge, or whose ZNCC in the last solver iteration is
+// below init.fallback_zncc, are searched again around their seed and re-solved from
 // the ZNCC seed while every other subset keeps its state (retry_seeded,
 // icgn.wgsl:1010-1017).  include_zncc leaves the final ZNCC of the output in
 // slot->zncc.
@@ -414,16 +414,15 @@ int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
     const int ring = (s->ring_first + s->ring_count) % kEventRing;
     s->ring_count += 1;
     HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
-    HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting,
-                           solve_args(e, 0, sl, sl.seed, sl.active, 0), st));
-    if (e->init.fallback_zncc > -1.0f) {
-        const float* z = nullptr;
-        if (model == kAffine) {
-            HIP_CHECK(launch_affine_zncc(zncc_args(e, sl, sl.state, s->zncc), st));
-            z = s->zncc;
-        }
-        HIP_CHECK(launch_retry_mask(L.count(), model, sl.state, z, e->init.fallback_zncc, L.roi ? L.selected : nullptr,
-                                    sl.mask, st));
+    const bool fallback = e->init.fallback_zncc > -1.0f;
+    SolveArgs first = solve_args(e, 0, sl, sl.seed, sl.active, 0);
+    // The fallback criterion uses the ZNCC the solver evaluates in its last
+    // iteration (at the warp it then updates by less than the tolerance).
+    if (fallback) first.zncc_out = s->zncc;
+    HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting, first, st));
+    if (fallback) {
+        HIP_CHECK(launch_retry_mask(L.count(), model, sl.state, s->zncc, e->init.fallback_zncc,
+                                    L.roi ? L.selected : nullptr, sl.mask, st));
         SearchArgs q = search_args(e, 0, sl, e->init.fallback_radius);
         q.prior = sl.seed;
         q.prior_stride = state_stride(model);

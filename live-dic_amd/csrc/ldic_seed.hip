@@ -266,8 +266,15 @@ __global__ __launch_bounds__(256) void k_zncc_search(SearchArgs a, SearchGeom g)
     a.shift[2 * subset + 1] = sy;
     a.score[subset] = c0;
     if (a.seed) {
+        // Translation from the search (the prior's where no shift was valid);
+        // the deformation terms of the prior (gradients, curvatures) are kept,
+        // zero without a prior.
         float* sd = a.seed + (size_t)subset * a.seed_stride;
-        for (int k = 0; k < a.seed_stride; ++k) sd[k] = 0.0f;
+        const float* pr = a.prior ? a.prior + (size_t)subset * a.prior_stride : nullptr;
+        for (int k = 0; k < a.seed_stride; ++k) {
+            const float v = pr && k < a.prior_stride ? pr[k] : 0.0f;
+            sd[k] = isfinite(v) ? v : 0.0f;
+        }
         if (c0 > -1.5f) {
             sd[0] = sx;
             sd[3] = sy;

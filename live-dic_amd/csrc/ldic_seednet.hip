@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "ldic_device.h"
 #include "livedic.h"
 
@@ -87,9 +89,14 @@ __device__ __forceinline__ void pixel_of(int m, int Ho, int Wo, int& n, int& oy,
 constexpr int kChunk = 64;
 constexpr int kChunkPitch = kChunk + 4;   // bf16 elements per LDS row (breaks bank aliasing)
 
-template <bool VEC, int NT>
+// PARTIAL: blockIdx.z selects a range of kChunk-aligned GEMM columns (split-K
+// for layers with few output pixels); the partial products are added into the
+// float32 workspace Yf (M x out_channels, dense) and k_conv_epilogue applies
+// bias and activation.
+template <bool VEC, int NT, bool PARTIAL>
 __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t* X, const uint16_t* Wt,
-                                                  const float* bias, int leaky, uint16_t* Y, float* Yf) {
+                                                  const float* bias, int leaky, uint16_t* Y, float* Yf,
+                                                  int k_per_split) {
     __shared__ __align__(16) uint16_t Bs[16 * NT * kChunkPitch];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int M = g.batch * g.out_height * g.out_width;
@@ -106,8 +113,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < KP; k0 += kChunk) {
-        const int kc = min(kChunk, KP - k0);
+    const int kbeg = PARTIAL ? blockIdx.z * k_per_split : 0;
+    const int kend = PARTIAL ? min(KP, kbeg + k_per_split) : KP;
+    for (int k0 = kbeg; k0 < kend; k0 += kChunk) {
+        const int kc = min(kChunk, kend - k0);
         // Stage W[co_base + r][k0 .. k0 + kc) in 8-byte pieces.
         __syncthreads();
         for (int i = threadIdx.x; i < rows * (kc / 4); i += 256) {
@@ -142,6 +151,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
         if (t >= ntiles) break;
         const int co = co_base + 16 * t + row;
         if (co >= g.out_channels) continue;
+        if (PARTIAL) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int mm = m0 + 4 * quad + i;
+                if (mm < M) atomicAdd(Yf + (size_t)mm * g.out_channels + co, acc[t][i]);
+            }
+            continue;
+        }
         const float bco = bias ? bias[co] : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -153,6 +170,62 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
             if (Yf) Yf[o] = v;
             else Y[o] = f2bf(v);
         }
+    }
+}
+
+__global__ void k_fill_zero(size_t count, float* p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0.0f;
+}
+
+__global__ void k_conv_epilogue(ldic_nn_conv g, const float* work, const float* bias, int leaky, uint16_t* Y,
+                                float* Yf) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t M = (size_t)g.batch * g.out_height * g.out_width;
+    if (i >= M * g.out_channels) return;
+    const size_t m = i / g.out_channels;
+    const int co = (int)(i - m * g.out_channels);
+    float v = work[i] + (bias ? bias[co] : 0.0f);
+    if (leaky) v = v > 0.0f ? v : kLeaky * v;
+    const size_t o = m * g.out_stride + g.out_offset + co;
+    if (Yf) Yf[o] = v;
+    else Y[o] = f2bf(v);
+}
+
+// Local correlation with one wave per pixel: lane j owns offset d0 + j of
+// each 64-offset pass and reads its target feature vector with 16-byte loads;
+// the reference vector (channels <= 64, one per lane) enters every product as
+// a wave-uniform v_readlane operand, so no cross-lane reduction is needed and
+// the 64 outputs of a pass are one coalesced store.  Requires channels, the
+// strides and offsets to be multiples of 8.
+__global__ __launch_bounds__(256) void k_corr_fwd_wave(ldic_nn_corr c, const uint16_t* F1, const uint16_t* F2,
+                                                       uint16_t* out) {
+    const int lane = threadIdx.x & 63;
+    const size_t p = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const size_t P = (size_t)c.batch * c.height * c.width;
+    if (p >= P) return;
+    const int D = 2 * c.radius + 1, D2 = D * D;
+    const int n = (int)(p / ((size_t)c.height * c.width));
+    const int r = (int)(p - (size_t)n * c.height * c.width);
+    const int y = r / c.width, x = r - y * c.width;
+    const float f1 = lane < c.channels ? bf2f(F1[p * c.f1_stride + c.f1_offset + lane]) : 0.0f;
+    for (int d0 = 0; d0 < D2; d0 += kWave) {
+        const int d = d0 + lane;
+        const int y2 = y + d / D - c.radius, x2 = x + d % D - c.radius;
+        const bool inb = d < D2 && y2 >= 0 && y2 < c.height && x2 >= 0 && x2 < c.width;
+        const uint16_t* row = F2 + (((size_t)n * c.height + (inb ? y2 : 0)) * c.width + (inb ? x2 : 0)) * c.f2_stride +
+                              c.f2_offset;
+        float s = 0.0f;
+        for (int k = 0; k < c.channels; k += 8) {
+            uint4 v = inb ? *reinterpret_cast<const uint4*>(row + k) : make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s += readlane_f32(f1, k + 2 * j) * __uint_as_float(w[j] << 16);
+                s += readlane_f32(f1, k + 2 * j + 1) * __uint_as_float(w[j] & 0xFFFF0000u);
+            }
+        }
+        if (d < D2) out[p * c.out_stride + c.out_offset + d] = f2bf(s * c.scale);
     }
 }
 
@@ -573,6 +646,25 @@ int check_conv(const ldic_nn_conv* g) {
     return LDIC_OK;
 }
 
+template <bool PARTIAL>
+void conv_fwd_launch(int tile, bool vec, dim3 grid, hipStream_t st, const ldic_nn_conv& g, const uint16_t* x,
+                            const uint16_t* w, const float* bias, int leaky, uint16_t* y, float* yf, int kps) {
+    switch (tile) {
+        case 32:
+            if (vec) k_conv_fwd<true, 2, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            else k_conv_fwd<false, 2, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            break;
+        case 64:
+            if (vec) k_conv_fwd<true, 4, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            else k_conv_fwd<false, 4, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            break;
+        default:
+            if (vec) k_conv_fwd<true, 8, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            else k_conv_fwd<false, 8, PARTIAL><<<grid, 256, 0, st>>>(g, x, w, bias, leaky, y, yf, kps);
+            break;
+    }
+}
+
 }  // namespace
 }  // namespace ldic
 
@@ -583,6 +675,14 @@ extern "C" {
 int ldic_nn_conv_forward(const ldic_nn_conv* g, const uint16_t* x, const uint16_t* w, const float* bias,
                          int32_t leaky, uint16_t* y, float* y_f32, void* stream) {
     if (check_conv(g) != LDIC_OK || !x || !w || (!y && !y_f32)) return LDIC_ERR_ARGUMENT;
+    return ldic_nn_conv_forward_split(g, x, w, bias, leaky, y, y_f32, nullptr, 1, stream);
+}
+
+int ldic_nn_conv_forward_split(const ldic_nn_conv* g, const uint16_t* x, const uint16_t* w, const float* bias,
+                               int32_t leaky, uint16_t* y, float* y_f32, float* workspace, int32_t splits,
+                               void* stream) {
+    if (check_conv(g) != LDIC_OK || !x || !w || (!y && !y_f32) || splits < 1 || (splits > 1 && !workspace))
+        return LDIC_ERR_ARGUMENT;
     const int M = g->batch * g->out_height * g->out_width;
     int tile = g->out_channel_tile;
     if (tile == 0) {
@@ -591,23 +691,22 @@ int ldic_nn_conv_forward(const ldic_nn_conv* g, const uint16_t* x, const uint16_
         while (tile > 32 && (size_t)blocks(M, 64) * ((g->out_channels + tile - 1) / tile) < 2048) tile /= 2;
     }
     if (tile != 32 && tile != 64 && tile != 128) return LDIC_ERR_ARGUMENT;
-    dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile);
+    const int KP = (9 * g->in_channels + 15) & ~15;
+    const int chunks = (KP + kChunk - 1) / kChunk;
+    splits = std::min(splits, chunks);
     hipStream_t st = (hipStream_t)stream;
     const bool vec = vec_input(*g);
-    switch (tile) {
-        case 32:
-            if (vec) k_conv_fwd<true, 2><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            else k_conv_fwd<false, 2><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            break;
-        case 64:
-            if (vec) k_conv_fwd<true, 4><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            else k_conv_fwd<false, 4><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            break;
-        default:
-            if (vec) k_conv_fwd<true, 8><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            else k_conv_fwd<false, 8><<<grid, 256, 0, st>>>(*g, x, w, bias, leaky, y, y_f32);
-            break;
+    if (splits == 1) {
+        dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile);
+        conv_fwd_launch<false>(tile, vec, grid, st, *g, x, w, bias, leaky, y, y_f32, KP);
+        return status(hipGetLastError());
     }
+    const size_t count = (size_t)M * g->out_channels;
+    k_fill_zero<<<blocks(count), 256, 0, st>>>(count, workspace);
+    const int kps = (chunks + splits - 1) / splits * kChunk;
+    dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile, (KP + kps - 1) / kps);
+    conv_fwd_launch<true>(tile, vec, grid, st, *g, x, w, nullptr, 0, nullptr, workspace, kps);
+    k_conv_epilogue<<<blocks(count), 256, 0, st>>>(*g, workspace, bias, leaky, y, y_f32);
     return status(hipGetLastError());
 }
 
@@ -663,8 +762,13 @@ int ldic_nn_leaky_backward(size_t count, const uint16_t* y, float* dy, void* str
 int ldic_nn_corr_forward(const ldic_nn_corr* c, const uint16_t* f1, const uint16_t* f2, uint16_t* out, void* stream) {
     if (!c || !f1 || !f2 || !out || c->radius < 0 || c->channels < 1) return LDIC_ERR_ARGUMENT;
     const int D = 2 * c->radius + 1;
-    const size_t total = (size_t)c->batch * c->height * c->width * D * D;
-    if (total) k_corr_fwd<<<blocks(total), 256, 0, (hipStream_t)stream>>>(*c, f1, f2, out);
+    const size_t pixels = (size_t)c->batch * c->height * c->width;
+    if (!pixels) return LDIC_OK;
+    if (c->channels <= kWave && c->channels % 8 == 0 && c->f1_stride % 8 == 0 && c->f1_offset % 8 == 0 &&
+        c->f2_stride % 8 == 0 && c->f2_offset % 8 == 0)
+        k_corr_fwd_wave<<<blocks(pixels, 4), 256, 0, (hipStream_t)stream>>>(*c, f1, f2, out);
+    else
+        k_corr_fwd<<<blocks(pixels * D * D), 256, 0, (hipStream_t)stream>>>(*c, f1, f2, out);
     return status(hipGetLastError());
 }
 

@@ -382,6 +382,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     Tile t;
     t.valid = false;
     t.x0 = t.y0 = t.w = t.h = 0;
+    float last_zncc = __builtin_nanf("");
 
     for (int it = 0; it < 30; ++it) {
         bool inside = warp_inside<MODEL>(q, cx, cy, he, Wf, Hf);
@@ -514,6 +515,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         }
         const float ratio = ref_norm / norm;
         const float zncc = src / (ref_norm * norm);
+        last_zncc = zncc;
         float change;
         if (MODEL == kQuadratic) {
             float rj[12] = {sj[0], sj[1], sj[2], sj[3], sj[4], sj[5], sq[0], sq[1], sq[2], sq[3], sq[4], sq[5]};
@@ -572,6 +574,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         } else {
             out[1] = make_float4(q.L[2], q.L[3], status, iters);
         }
+        if (a.zncc_out) a.zncc_out[subset] = last_zncc;
     }
 }
 

@@ -256,6 +256,12 @@ class SeedNet:
                     b["dwarped"] = torch.zeros((n, hh, ww, c), dtype=f32, device=dev)
             self.st.append(b)
         self.loss = torch.zeros(1, dtype=f32, device=dev)
+        # Split-K workspace: the largest output of any convolution at this shape.
+        largest = max(2 * n * hh_ * ww_ * c for hh_, ww_, c in self.enc_shapes)
+        for si, stage in enumerate(a.stages):
+            hh, ww, _ = self.enc_shapes[stage.level]
+            largest = max(largest, n * hh * ww * max(co for co, _ in stage.head))
+        self.workspace = torch.zeros(largest, dtype=f32, device=dev)
         torch.cuda.synchronize(dev)
         self._shape = key
 
@@ -263,9 +269,11 @@ class SeedNet:
     def _conv(self, p, batch, h, w, x, x_stride, x_offset, y, y_stride, y_f32=None):
         g = _native.NNConv(batch, h, w, p["cin"], x_stride, x_offset, (h - 1) // p["stride"] + 1,
                            (w - 1) // p["stride"] + 1, p["cout"], y_stride, 0, p["stride"], p.get("tile", 0))
-        _native.check(self.lib.ldic_nn_conv_forward(ctypes.byref(g), _ptr(x), _ptr(p["w16"]), _ptr(p["b"]),
-                                                    1 if p["act"] else 0, None if y is None else _ptr(y),
-                                                    None if y_f32 is None else _ptr(y_f32), self.stream_handle))
+        splits = p.get("splits", 1)
+        _native.check(self.lib.ldic_nn_conv_forward_split(
+            ctypes.byref(g), _ptr(x), _ptr(p["w16"]), _ptr(p["b"]), 1 if p["act"] else 0,
+            None if y is None else _ptr(y), None if y_f32 is None else _ptr(y_f32),
+            _ptr(self.workspace) if splits > 1 else None, splits, self.stream_handle))
         return g
 
     def _conv_back(self, p, g, x, dy, dx, dx_stride, accumulate):
@@ -401,8 +409,9 @@ class SeedNet:
             _native.check(lib.ldic_nn_transpose_weights(p["cin"], p["cout"], _ptr(p["w16"]), _ptr(p["wt16"]), s))
 
     def autotune(self, h: int, w: int, n: int = 1, reps: int = 5) -> dict:
-        """Pick each forward convolution's output-channel tile (32, 64, 128 per workgroup) by timing
-        whole forward passes on device-resident inputs of this shape, one layer at a time."""
+        """Pick each forward convolution's output-channel tile (32, 64, 128 per workgroup) and its
+        split of the GEMM columns (1, 2, 4, 8) by timing whole forward passes on device-resident
+        inputs of this shape, one layer at a time."""
         torch = self.torch
         imgs = torch.randint(0, 256, (2 * n, h, w), dtype=torch.uint8, device=self.device)
         ref, tgt = _ptr(imgs), _ptr(imgs) + n * h * w
@@ -419,17 +428,20 @@ class SeedNet:
                 return start.elapsed_time(end) / reps
 
         for p in self.params.values():
-            p["tile"] = 0
+            p["tile"], p["splits"] = 0, 1
         choice = {}
         for name, p in self.params.items():
             best = None
             for tile in (32, 64, 128):
-                p["tile"] = tile
-                ms = timed()
-                if best is None or ms < best[0]:
-                    best = (ms, tile)
-            p["tile"] = best[1]
-            choice[name] = best[1]
+                for splits in (1, 2, 4, 8):
+                    if splits > 1 and 9 * p["cin"] < 128 * splits:
+                        continue
+                    p["tile"], p["splits"] = tile, splits
+                    ms = timed()
+                    if best is None or ms < best[0]:
+                        best = (ms, tile, splits)
+            p["tile"], p["splits"] = best[1], best[2]
+            choice[name] = [best[1], best[2]]
         choice["forward_ms"] = timed()
         return choice
 
