@@ -264,6 +264,8 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-learned", action="store_true", help="skip the learned-vs-ZNCC seeding comparison")
     parser.add_argument("--no-batch", action="store_true", help="skip the batched heterogeneous workload")
+    parser.add_argument("--only", choices=("learned", "batch"),
+                        help="run only this secondary workload and print its JSON object (for profiling)")
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="only copy 256 MiB with one plain device copy kernel (PMC byte-counter calibration)")
     args = parser.parse_args()
@@ -281,6 +283,9 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for the "
               "sharding design); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.only:
+        print(json.dumps(learned_vs_zncc() if args.only == "learned" else batched_heterogeneous()))
+        return 0
 
     from live_dic import ICGNWGPUFullPyramid, _native, synthetic
 
