@@ -67,6 +67,8 @@ typedef struct ldic_timing {
     double readback_ms;  /* wait and copy of the packed state into the caller's buffer */
     double wall_ms;      /* whole call */
     int32_t frames;      /* frames completed by this call (resident slots may queue several) */
+    int32_t temporal_seeded;   /* temporal mode: coarsest subsets seeded from the history (last frame) */
+    int32_t temporal_updates;  /* temporal mode: finest subsets merged into the history (last frame) */
 } ldic_timing;
 
 /* Device selection and kernel set-up: ICGNWGPU.__init__ device/pipeline
@@ -137,16 +139,30 @@ int ldic_register_seeded(struct ldic_engine* engine, const uint8_t* target, int6
  * ZNCC search on the uint8 level planes (exact integer moments, float64
  * quotient) in two places: as the coarsest-level initialiser, and as the
  * fallback of seeded registration. ---- */
-enum { LDIC_INIT_ZERO = 0, LDIC_INIT_ZNCC = 1 };
+enum { LDIC_INIT_ZERO = 0, LDIC_INIT_ZNCC = 1, LDIC_INIT_TEMPORAL = 2 };
 
 typedef struct ldic_initialization {
-    int32_t kind;            /* LDIC_INIT_ZERO or LDIC_INIT_ZNCC (coarsest level searched over +-search_radius) */
+    int32_t kind;            /* LDIC_INIT_ZERO, LDIC_INIT_ZNCC (coarsest level searched over +-search_radius) or
+                                LDIC_INIT_TEMPORAL: the last accepted finest field, projected onto the coarsest
+                                grid, seeds every frame (initialization="temporal",
+                                full_pyramid.wgsl:437-575, full_pyramid_backend.py:433-881) */
     int32_t search_radius;   /* coarsest-level pixels, 0..64 */
     float fallback_zncc;     /* seeded registration: subsets that do not converge, or whose ZNCC in the last
                                 solver iteration is below this, are searched again around their seed and
                                 re-solved; <= -1 disables */
     int32_t fallback_radius; /* finest-level pixels around the seed, 0..64 */
+    uint32_t temporal_max_age; /* LDIC_INIT_TEMPORAL: history expires after more than this many rejected frames;
+                                  0xFFFFFFFE keeps it (temporal_max_age=None) */
 } ldic_initialization;
+
+/* Forget the temporal history (reset_temporal_state, full_pyramid_backend.py:
+ * 660-677).  ldic_set_reference and a change of initialisation also reset it.
+ * Frames must be registered in chronological order in temporal mode. */
+int ldic_reset_temporal(struct ldic_engine* engine);
+/* ROI links of the temporal projection from the finest grid onto the
+ * coarsest (seed_permissions with source scale 1, full_pyramid_backend.py:
+ * 608-620); NULL restores "all links".  Call after ldic_set_reference. */
+int ldic_set_temporal_links(struct ldic_engine* engine, const uint32_t* coarse_links);
 
 /* Takes effect for the next queued frame; rejected while a slot is pending.
  * The search needs (2R+S)(ceil((2R+1)/4)+ceil(S/4))*4 + S*ceil(S/4)*4 +

@@ -44,6 +44,7 @@ struct SeedArgs {
     const uint32_t* permissions;  // optional; 31 when absent
     float* dst_seed;
     uint32_t* dst_active;
+    uint32_t* seeded_count;       // optional: += subsets that received a weighted seed and are selected
 };
 
 struct MedianArgs {
@@ -115,6 +116,12 @@ struct BatchLaunch {
 hipError_t launch_solve_batched(const BatchLaunch& b, hipStream_t s);
 hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s);
 hipError_t launch_median(const MedianArgs& a, hipStream_t s);
+// Temporal history (full_pyramid.wgsl:519-575, full_pyramid_backend.py:660-677).
+hipError_t launch_temporal_reset(int count, int model, float* history, uint32_t* ages, hipStream_t s);
+hipError_t launch_temporal_merge(int count, int model, const float* raw, const float* output, int use_filtered,
+                                 uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
+                                 hipStream_t s);
+hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s);
 hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);
 
 // Exhaustive integer ZNCC search; LDS bytes per workgroup for (S, R).

@@ -94,6 +94,70 @@ __global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
         o[1] = make_float4(acc[4], acc[5], a.track_availability ? marker : 0.0f, 0.0f);
     }
     a.dst_active[di] = (perm >> 4u) & 1u;
+    if (a.seeded_count && wsum > 0.0f && (perm & 16u)) atomicAdd(a.seeded_count, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Temporal history of the finest field (full_pyramid.wgsl:519-575 and the reset
+// of full_pyramid_backend.py:660-677): ages 0xFFFFFFFF mean "never set"; a
+// subset whose final status is accepted (1 or 2) -- and, unless the filtered
+// output is used, whose raw status is accepted too -- is copied into the
+// history with age 0; every other subset ages, saturating below the sentinel,
+// and its history status becomes -1 once the age exceeds max_age.
+// ---------------------------------------------------------------------------
+__global__ void k_temporal_reset(int count, int stride, int P, float* history, uint32_t* ages) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    for (int k = 0; k < stride; ++k) history[(size_t)i * stride + k] = k == P ? -1.0f : 0.0f;
+    ages[i] = 0xFFFFFFFFu;
+}
+
+__global__ void k_temporal_merge(int count, int stride, int P, const float* raw, const float* output,
+                                 int use_filtered, uint32_t max_age, float* history, uint32_t* ages,
+                                 uint32_t* accepted_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float rs = raw[(size_t)i * stride + P], fs = output[(size_t)i * stride + P];
+    const bool raw_ok = rs == 1.0f || rs == 2.0f, final_ok = fs == 1.0f || fs == 2.0f;
+    if (final_ok && (use_filtered || raw_ok)) {
+        const float* src = (use_filtered ? output : raw) + (size_t)i * stride;
+        for (int k = 0; k < stride; ++k) history[(size_t)i * stride + k] = src[k];
+        ages[i] = 0u;
+        if (accepted_count) atomicAdd(accepted_count, 1u);
+    } else {
+        const uint32_t age = ages[i];
+        if (age != 0xFFFFFFFFu) {
+            const uint32_t next = min(age, 0xFFFFFFFDu) + 1u;
+            ages[i] = next;
+            if (next > max_age) history[(size_t)i * stride + P] = -1.0f;
+        }
+    }
+}
+
+__global__ void k_fill_u32(int count, uint32_t value, uint32_t* p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+hipError_t launch_temporal_reset(int count, int model, float* history, uint32_t* ages, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_temporal_reset<<<(count + 255) / 256, 256, 0, s>>>(count, state_stride(model), param_count(model), history, ages);
+    return hipGetLastError();
+}
+
+hipError_t launch_temporal_merge(int count, int model, const float* raw, const float* output, int use_filtered,
+                                 uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
+                                 hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_temporal_merge<<<(count + 255) / 256, 256, 0, s>>>(count, state_stride(model), param_count(model), raw, output,
+                                                         use_filtered, max_age, history, ages, accepted_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_fill_u32<<<(count + 255) / 256, 256, 0, s>>>(count, value, p);
+    return hipGetLastError();
 }
 
 hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s) {

@@ -132,6 +132,8 @@ struct Slot {
     float* pinned_out = nullptr;
     float* zncc = nullptr;
     float* pinned_zncc = nullptr;
+    uint32_t* counts = nullptr;         // temporal: seeded coarsest subsets, merged finest subsets
+    uint32_t* pinned_counts = nullptr;
     bool pending = false;
     bool resident = false;
     double upload_ms = 0.0;
@@ -160,7 +162,15 @@ struct ldic_engine {
     // integration weights, kept for consecutive-pair registration.
     double *prep_g = nullptr, *prep_a = nullptr, *prep_b = nullptr, *prep_gx = nullptr, *prep_gy = nullptr;
     double* weights = nullptr;
-    ldic_initialization init{LDIC_INIT_ZERO, 0, -2.0f, 0};
+    ldic_initialization init{LDIC_INIT_ZERO, 0, -2.0f, 0, 0xFFFFFFFEu};
+    // Temporal history of the finest field and the projection links onto the coarsest grid.
+    float* hist_state = nullptr;
+    uint32_t* hist_ages = nullptr;
+    uint32_t* temporal_links = nullptr;
+    // Recorded after each frame's history merge; the next frame's projection,
+    // on whichever slot, waits for it, so streams keep the chronological chain.
+    hipEvent_t temporal_done = nullptr;
+    bool temporal_pending = false;
 };
 
 namespace {
@@ -216,6 +226,9 @@ int create_slot(ldic_engine* e, Slot** out) {
         CHECK(s->arena.alloc(&sl.retry_active, n));
     }
     CHECK(s->arena.alloc(&s->zncc, (size_t)e->lv[0].count()));
+    CHECK(s->arena.alloc(&s->counts, 4));
+    CHECK(s->arena.alloc_pinned(&s->pinned_counts, 4));
+    std::memset(s->pinned_counts, 0, 4 * sizeof(uint32_t));
     CHECK(s->arena.alloc_pinned(&s->pinned_in, (size_t)e->lv[0].width * e->lv[0].height));
     CHECK(s->arena.alloc_pinned(&s->pinned_ref, (size_t)e->lv[0].width * e->lv[0].height));
     CHECK(s->arena.alloc_pinned(&s->pinned_valid, (size_t)e->lv[0].count()));
@@ -357,6 +370,32 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
             seed = sl.seed;
             active = sl.active;
         }
+        if (k == L - 1 && e->init.kind == LDIC_INIT_TEMPORAL) {
+            // Project the last accepted finest field onto this (coarsest) grid
+            // (project_temporal_seed, full_pyramid.wgsl:437-449).
+            const Level& fine = e->lv[0];
+            if (e->temporal_pending) HIP_CHECK(hipStreamWaitEvent(st, e->temporal_done, 0));
+            HIP_CHECK(launch_fill_u32(4, 0u, s->counts, st));
+            SeedArgs sa{};
+            sa.src_grid_w = fine.gw;
+            sa.src_grid_h = fine.gh;
+            sa.dst_grid_w = lev.gw;
+            sa.dst_count = lev.count();
+            sa.src_scale = 1.0f;
+            sa.dst_scale = (float)(1 << k);
+            sa.S = (float)e->opt.subset_size;
+            sa.D = (float)e->opt.subset_stride;
+            sa.model = e->opt.motion_model;
+            sa.track_availability = 0;
+            sa.src_state = e->hist_state;
+            sa.permissions = e->temporal_links;
+            sa.dst_seed = sl.seed;
+            sa.dst_active = sl.active;
+            sa.seeded_count = s->counts;
+            HIP_CHECK(launch_transfer_seed(sa, st));
+            seed = sl.seed;
+            active = sl.active;
+        }
         if (k == L - 1 && e->init.kind == LDIC_INIT_ZNCC) {
             // Translation seeds of the coarsest level from the exhaustive
             // ZNCC search; ROI-excluded subsets stay disabled.
@@ -375,6 +414,17 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
                                solve_args(e, k, sl, seed, active, 0), st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
         CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
+    }
+    if (e->init.kind == LDIC_INIT_TEMPORAL) {
+        // update_temporal_state (full_pyramid.wgsl:519-565): the filtered output
+        // is merged under the replace policy, the raw state otherwise.
+        SlotLevel& fine = s->lv[0];
+        HIP_CHECK(launch_temporal_merge(e->lv[0].count(), e->opt.motion_model, fine.state, level_output(e, fine),
+                                        e->opt.outlier_policy == kReplace ? 1 : 0, e->init.temporal_max_age,
+                                        e->hist_state, e->hist_ages, s->counts + 1, st));
+        HIP_CHECK(hipMemcpyAsync(s->pinned_counts, s->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipEventRecord(e->temporal_done, st));
+        e->temporal_pending = true;
     }
     return LDIC_OK;
 }
@@ -442,8 +492,10 @@ int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
 int validate_initialization(const ldic_engine* e, const ldic_initialization* in) {
     if (!in) return set_error(LDIC_ERR_ARGUMENT, "initialization is NULL");
     const int S = e->opt.subset_size;
-    if (in->kind != LDIC_INIT_ZERO && in->kind != LDIC_INIT_ZNCC)
-        return set_error(LDIC_ERR_ARGUMENT, "initialization kind must be LDIC_INIT_ZERO or LDIC_INIT_ZNCC");
+    if (in->kind != LDIC_INIT_ZERO && in->kind != LDIC_INIT_ZNCC && in->kind != LDIC_INIT_TEMPORAL)
+        return set_error(LDIC_ERR_ARGUMENT, "initialization kind must be LDIC_INIT_ZERO, _ZNCC or _TEMPORAL");
+    if (in->temporal_max_age > 0xFFFFFFFEu)
+        return set_error(LDIC_ERR_ARGUMENT, "temporal_max_age must be at most 0xFFFFFFFE");
     const bool search = in->kind == LDIC_INIT_ZNCC, fallback = in->fallback_zncc > -1.0f;
     if (!std::isfinite(in->fallback_zncc) || in->fallback_zncc > 1.0f)
         return set_error(LDIC_ERR_ARGUMENT, "fallback_zncc must be finite and at most 1 (<= -1 disables it)");
@@ -503,6 +555,8 @@ int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wal
         t->gpu_ms = gpu;
         t->solve_ms = s->drained_solve_ms;
         t->frames = s->drained_frames;
+        t->temporal_seeded = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[0] : 0;
+        t->temporal_updates = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[1] : 0;
         t->readback_ms = now_ms() - t0;
         t->wall_ms = now_ms() - wall_start;
     }
@@ -713,6 +767,10 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
     CHECK(e->ref_arena.alloc(&e->prep_b, px0));
     CHECK(e->ref_arena.alloc(&e->prep_gx, px0));
     CHECK(e->ref_arena.alloc(&e->prep_gy, px0));
+    CHECK(e->ref_arena.alloc(&e->hist_state, (size_t)L0.count() * state_stride(e->opt.motion_model)));
+    CHECK(e->ref_arena.alloc(&e->hist_ages, (size_t)L0.count()));
+    e->temporal_links = nullptr;
+    HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, st));
     HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, image, (size_t)row_stride, L0.width, L0.height,
                                hipMemcpyHostToDevice, st));
     int rc = enqueue_reference(e, st);
@@ -753,6 +811,7 @@ int ldic_create(const ldic_options* options, ldic_engine** out_engine) {
     hipError_t he = hipSetDevice(e->device);
     if (he == hipSuccess) he = hipGetDeviceProperties(&e->prop, e->device);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->prep_stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->temporal_done, hipEventDisableTiming);
     if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "device set-up failed: %s", hipGetErrorString(he)));
     float taps[31];
     fir_taps(taps);
@@ -772,6 +831,7 @@ void ldic_destroy(ldic_engine* e) {
     e->ref_arena.release();
     e->const_arena.release();
     if (e->prep_stream) (void)hipStreamDestroy(e->prep_stream);
+    if (e->temporal_done) (void)hipEventDestroy(e->temporal_done);
     delete e;
 }
 
@@ -835,6 +895,9 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
     HIP_CHECK(hipSetDevice(e->device));
     Slot* s = e->slots[0];
     if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
+    // A new reference starts a new temporal history.
+    if (e->init.kind == LDIC_INIT_TEMPORAL)
+        HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, s->stream));
     for (int y = 0; y < L0.height; ++y)
         std::memcpy(s->pinned_ref + (size_t)y * L0.width, reference + (size_t)y * reference_stride, (size_t)L0.width);
     HIP_CHECK(hipMemcpyAsync(e->lv[0].image, s->pinned_ref, (size_t)L0.width * L0.height, hipMemcpyHostToDevice,
@@ -858,6 +921,8 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
     HIP_CHECK(hipSetDevice(e->device));
     const Level& L0 = e->lv[0];
     if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
+    if (e->init.kind == LDIC_INIT_TEMPORAL)
+        HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, s->stream));
     HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, device_reference, (size_t)row_stride, L0.width, L0.height,
                                hipMemcpyDeviceToDevice, s->stream));
     CHECK(enqueue_reference(e, s->stream));
@@ -993,7 +1058,35 @@ int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
     CHECK(validate_initialization(e, init));
     for (Slot* s : e->slots)
         if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
+    const bool restart = init->kind == LDIC_INIT_TEMPORAL && e->init.kind != LDIC_INIT_TEMPORAL;
     e->init = *init;
+    if (restart && e->ready) CHECK(ldic_reset_temporal(e));
+    return LDIC_OK;
+}
+
+int ldic_reset_temporal(ldic_engine* e) {
+    CHECK(check_ready(e));
+    for (Slot* s : e->slots)
+        if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
+    HIP_CHECK(hipSetDevice(e->device));
+    HIP_CHECK(launch_temporal_reset(e->lv[0].count(), e->opt.motion_model, e->hist_state, e->hist_ages,
+                                    e->prep_stream));
+    HIP_CHECK(hipStreamSynchronize(e->prep_stream));
+    return LDIC_OK;
+}
+
+int ldic_set_temporal_links(ldic_engine* e, const uint32_t* coarse_links) {
+    CHECK(check_ready(e));
+    HIP_CHECK(hipSetDevice(e->device));
+    if (!coarse_links) {
+        e->temporal_links = nullptr;
+        return LDIC_OK;
+    }
+    const size_t n = (size_t)e->lv[e->levels - 1].count();
+    uint32_t* d = nullptr;
+    CHECK(e->ref_arena.alloc(&d, n));
+    HIP_CHECK(hipMemcpy(d, coarse_links, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    e->temporal_links = d;
     return LDIC_OK;
 }
 
@@ -1397,6 +1490,8 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
         if (!device_targets[i]) return set_error(LDIC_ERR_ARGUMENT, "device target %d is NULL", i);
         if (engines[i]->device != engines[0]->device)
             return set_error(LDIC_ERR_ARGUMENT, "all engines of a batch must use one device");
+        if (engines[i]->init.kind == LDIC_INIT_TEMPORAL)
+            return set_error(LDIC_ERR_ARGUMENT, "temporal initialisation is not supported in a batch");
         for (int k = 0; k < i; ++k)
             if (engines[k] == engines[i]) return set_error(LDIC_ERR_ARGUMENT, "engine %d appears twice", i);
     }

@@ -75,6 +75,8 @@ class Timing(ctypes.Structure):
         ("readback_ms", ctypes.c_double),
         ("wall_ms", ctypes.c_double),
         ("frames", ctypes.c_int32),
+        ("temporal_seeded", ctypes.c_int32),
+        ("temporal_updates", ctypes.c_int32),
     ]
 
 
@@ -84,6 +86,7 @@ class Initialization(ctypes.Structure):
         ("search_radius", ctypes.c_int32),
         ("fallback_zncc", ctypes.c_float),
         ("fallback_radius", ctypes.c_int32),
+        ("temporal_max_age", ctypes.c_uint32),
     ]
 
 
@@ -108,6 +111,7 @@ class NNSeed(ctypes.Structure):
 
 INIT_ZERO = 0
 INIT_ZNCC = 1
+INIT_TEMPORAL = 2
 
 
 # Every symbol include/livedic.h declares, with its argument types.
@@ -134,6 +138,8 @@ SIGNATURES = {
     "ldic_set_initialization": (ctypes.c_int, [_P, ctypes.POINTER(Initialization)]),
     "ldic_register_seeded_resident": (ctypes.c_int, [_P, _I32, _P, _I64, _P, _P]),
     "ldic_export_search": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
+    "ldic_reset_temporal": (ctypes.c_int, [_P]),
+    "ldic_set_temporal_links": (ctypes.c_int, [_P, _P]),
     "ldic_prepare_reference_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
     "ldic_batch_create": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(_P)]),
     "ldic_batch_tune": (ctypes.c_int, [_P]),

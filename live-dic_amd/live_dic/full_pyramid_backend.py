@@ -55,10 +55,14 @@ class ICGNWGPUFullPyramid(_NativeEngine):
 
     ``iterations_per_level`` and ``tolerance_per_level`` are ordered from the
     coarsest level to the finest.  Initialisers: ``"zero"`` (the reference
-    default) and ``"zncc"``, an addition: the coarsest level is seeded by an
-    exhaustive integer ZNCC search covering ``seed_search_radius`` pixels of the
-    finest level.  The reference's temporal, Fourier and CPM-HOG matching
-    initialisers and the adaptive pyramid policy are not built.
+    default); ``"temporal"`` (full_pyramid_backend.py:433-881): the last accepted
+    finest field, projected onto the coarsest grid, seeds every frame; history
+    entries expire after more than ``temporal_max_age`` rejected frames; targets
+    must come in chronological order; and ``"zncc"``, an addition: the coarsest
+    level is seeded by an exhaustive integer ZNCC search covering
+    ``seed_search_radius`` pixels of the finest level.  Not built: the temporal
+    velocity prediction and fine correction, the Fourier and CPM-HOG matching
+    initialisers, and the adaptive pyramid policy.
     """
 
     def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
@@ -70,7 +74,9 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                  median_filter_scope: str = "rejected", subset_size: int = DEFAULT_SUBSET_SIZE,
                  subset_stride: int | None = None, spatial_weighting: str = "uniform",
                  gaussian_sigma: float | None = None, initialization: str = "zero",
-                 pyramid_policy: str = "fixed", seed_search_radius: int = 32, **unsupported) -> None:
+                 temporal_max_age: int | None = None, temporal_fine_correction: bool = False,
+                 temporal_prediction: str = "none", pyramid_policy: str = "fixed", seed_search_radius: int = 32,
+                 **unsupported) -> None:
         if unsupported:
             raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
         if not isinstance(pyramid_policy, str) or pyramid_policy not in ("fixed", "adaptive"):
@@ -80,8 +86,29 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if not isinstance(initialization, str) or initialization not in (
                 "zero", "zncc", "temporal", "fourier", "wgpu-matching"):
             raise ValueError("initialization must be 'zero', 'zncc', 'temporal', 'fourier' or 'wgpu-matching'")
-        if initialization not in ("zero", "zncc"):
-            raise ValueError(f"initialization={initialization!r} is not built; use 'zero' or 'zncc'")
+        if initialization not in ("zero", "zncc", "temporal"):
+            raise ValueError(f"initialization={initialization!r} is not built; use 'zero', 'temporal' or 'zncc'")
+        if not isinstance(temporal_prediction, str) or temporal_prediction not in ("none", "velocity"):
+            raise ValueError("temporal_prediction must be 'none' or 'velocity'")
+        if temporal_prediction != "none" and initialization != "temporal":
+            raise ValueError("temporal_prediction requires temporal initialization")
+        if temporal_prediction != "none":
+            raise ValueError("temporal_prediction='velocity' is not built; use 'none'")
+        if not isinstance(temporal_fine_correction, bool):
+            raise ValueError("temporal_fine_correction must be a boolean")
+        if temporal_fine_correction:
+            raise ValueError("temporal_fine_correction is not built")
+        if temporal_max_age is not None:
+            try:
+                age = operator.index(temporal_max_age)
+            except TypeError as error:
+                raise ValueError("temporal_max_age must be a nonnegative integer or None") from error
+            if isinstance(temporal_max_age, bool) or not 0 <= age <= 0xFFFFFFFE:
+                raise ValueError("temporal_max_age must be between 0 and 4294967294 or None")
+            temporal_max_age = age
+        self.temporal_max_age = temporal_max_age
+        self.temporal_prediction = temporal_prediction
+        self.temporal_fine_correction = temporal_fine_correction
         self.pyramid_levels = _level_count(pyramid_levels)
         try:
             radius = operator.index(seed_search_radius)
@@ -113,6 +140,8 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         self._zncc_min = 0.8
         if initialization == "zncc":
             self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius)
+        elif initialization == "temporal":
+            self._set_initialization(_native.INIT_TEMPORAL, temporal_max_age=temporal_max_age)
 
     def set_reference(self, image: ImageInput, *, iterations_per_level: tuple[int, ...] | None = None,
                       tolerance_per_level: tuple[float, ...] | None = None, zncc_min: float = 0.8,
@@ -131,6 +160,8 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         configs_coarse_first = [_validated_config(i, t, zncc_min) for i, t in zip(iterations, tolerances)]
         configs = configs_coarse_first[::-1]  # finest first for the native engine
         build_ms = self._prepare_reference(image, configs, roi_mask, roi_policy, roi_truncation)
+        if self.initialization == "temporal":
+            self._set_temporal_links()
         self.iterations_per_level = iterations
         self.tolerance_per_level = tolerances
         self._zncc_min = float(zncc_min)
@@ -144,10 +175,30 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if self.initialization == "zncc":
             self._pyramid_config.update(initialization="zncc", seedSearchRadius=self.seed_search_radius,
                                         coarseSearchRadius=self.coarse_search_radius)
+        elif self.initialization == "temporal":
+            self._pyramid_config.update(initialization="temporal", temporalMaxAge=self.temporal_max_age,
+                                        temporalSeedState="filtered" if self.outlier_policy == "replace" else "raw")
         return self._reference_info(started, build_ms, {
             **configs[0],
             **({"roi": {**self.roi.summary(), "selectedPerLevel": self._roi_level_counts}} if self.roi else {}),
             **self._option_config()})
+
+    def _set_temporal_links(self) -> None:
+        """Projection links finest -> coarsest under an effective ROI (full_pyramid_backend.py:608-620)."""
+        from .roi import seed_permissions
+
+        links = None
+        if self.roi is not None and self.roi.effective:
+            links = np.ascontiguousarray(seed_permissions(
+                self.roi, self._roi_plans[-1], source_scale=1, destination_scale=1 << (self.pyramid_levels - 1),
+                size=self.subset_size, stride=self.subset_stride, reference_mask=self._roi_mask), dtype=np.uint32)
+        _native.check(self._lib.ldic_set_temporal_links(self._handle, None if links is None else links.ctypes.data))
+
+    def reset_temporal_state(self) -> None:
+        """Forget the temporal history (full_pyramid_backend.py:660-677); a no-op before
+        set_reference or in other initialisation modes.  Finish consuming a stream first."""
+        if self._reference_ready and self.initialization == "temporal":
+            _native.check(self._lib.ldic_reset_temporal(self._handle))
 
     @_exclusive
     def register(self, target: ImageInput) -> RegistrationResult:

@@ -204,6 +204,8 @@ class _NativeEngine:
         self.width, self.height = width, height
         self.grid_width, self.grid_height = shapes[0][2], shapes[0][3]
         self.roi = plans[0]
+        self._roi_plans = plans
+        self._roi_mask = mask
         self.reference_valid = valid.astype(bool)
         self.reference_valid.setflags(write=False)
         self._roi_level_counts = [int(p.selected.sum()) if p is not None else s[2] * s[3]
@@ -286,6 +288,8 @@ class _NativeEngine:
             "queueSubmissions": 1.0,
             "totalWallMs": total,
             "resultAgeWallMs": total,
+            "temporallySeededCoarseSubsets": float(t.temporal_seeded),
+            "temporalAcceptedUpdates": float(t.temporal_updates),
         }
 
     def _run(self, frame: np.ndarray, started: float, prep_ms: float, config: dict) -> RegistrationResult:
@@ -392,9 +396,10 @@ class _NativeEngine:
         return out
 
     def _set_initialization(self, kind: int, search_radius: int = 0, fallback_zncc: float | None = None,
-                            fallback_radius: int = 0) -> None:
+                            fallback_radius: int = 0, temporal_max_age: int | None = None) -> None:
         init = _native.Initialization(kind, int(search_radius), -2.0 if fallback_zncc is None else float(fallback_zncc),
-                                      int(fallback_radius))
+                                      int(fallback_radius),
+                                      0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age))
         _native.check(self._lib.ldic_set_initialization(self._handle, ctypes.byref(init)))
         self._initialization = init
 

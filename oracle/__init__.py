@@ -121,14 +121,32 @@ def solve(cfg: SolveCfg, reference, metadata, inverse, coeff, target, seed=None,
     return state
 
 
-def transfer_seed(cfg: SeedCfg, source_state, permissions=None):
+def transfer_seed(cfg: SeedCfg, source_state, permissions=None, return_weight=False):
     stride = 16 if cfg.motion_model == 2 else 8
     seed = np.zeros((cfg.destination_grid_count, stride), dtype=np.float32)
     active = np.zeros(cfg.destination_grid_count, dtype=np.uint32)
+    weight = np.zeros(cfg.destination_grid_count, dtype=np.float32)
     src = np.ascontiguousarray(source_state, dtype=np.float32)
     perm = None if permissions is None else np.ascontiguousarray(permissions, dtype=np.uint32)
-    lib().ora_transfer_seed(ctypes.byref(cfg), _p(src), _p(perm), _p(seed), _p(active))
-    return seed, active
+    lib().ora_transfer_seed(ctypes.byref(cfg), _p(src), _p(perm), _p(seed), _p(active), _p(weight))
+    return (seed, active, weight) if return_weight else (seed, active)
+
+
+NEVER = 0xFFFFFFFF
+
+
+def temporal_merge(history, ages, raw, output, use_filtered, max_age):
+    """update_temporal_state (full_pyramid.wgsl:519-565) on float32 (count, stride) arrays and uint64
+    ages; returns the number of merged subsets."""
+    P = 12 if history.shape[1] == 16 else 6
+    rs, fs = raw[:, P], output[:, P]
+    upd = ((fs == 1.0) | (fs == 2.0)) & (bool(use_filtered) | (rs == 1.0) | (rs == 2.0))
+    history[upd] = (output if use_filtered else raw)[upd]
+    ages[upd] = 0
+    aging = ~upd & (ages != NEVER)
+    ages[aging] = np.minimum(ages[aging], 0xFFFFFFFD) + 1
+    history[aging & (ages > max_age), P] = -1.0
+    return int(upd.sum())
 
 
 def median(cfg: MedianCfg, state, metadata, roi_words=None):
@@ -192,7 +210,7 @@ class Pipeline:
                  zncc_min=0.8, motion_model="affine", interpolation="bspline", subset_size=16,
                  subset_stride=None, spatial_weighting="uniform", gaussian_sigma=None,
                  outlier_policy="replace", median_filter_size=3, median_filter_scope="rejected",
-                 lanes=16):
+                 lanes=16, initialization="zero", temporal_max_age=None):
         L = pyramid_levels
         self.levels_count = L
         self.iterations = tuple(iterations_per_level or DEFAULT_ITERATIONS[L])
@@ -209,13 +227,28 @@ class Pipeline:
         self.lanes = lanes
         self.fir = prep.fir_weights()
         self.levels: list[Level] = []
+        # Temporal initialisation (full_pyramid_backend.py:433-881, default options).
+        self.temporal = initialization == "temporal"
+        self.temporal_max_age = 0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age)
+        self.temporal_links = None
+        self.last_counts = (0, 0)
 
-    def set_reference(self, image, rois=None):
+    def set_reference(self, image, rois=None, temporal_links=None):
         images = [np.ascontiguousarray(image, dtype=np.uint8)]
         for _ in range(1, self.levels_count):
             images.append(reduce(images[-1]))
         self.levels = [Level(img, self.opts, None if rois is None else rois[k]) for k, img in enumerate(images)]
+        self.temporal_links = temporal_links
+        self.reset_temporal_state()
         return self
+
+    def reset_temporal_state(self):
+        """reset_temporal_state (full_pyramid_backend.py:660-677)."""
+        fine = self.levels[0]
+        stride = 16 if self.opts["motion_model"] == "quadratic" else 8
+        self.history = np.zeros((fine.grid_width * fine.grid_height, stride), np.float32)
+        self.history[:, 12 if stride == 16 else 6] = -1.0
+        self.ages = np.full(fine.grid_width * fine.grid_height, NEVER, np.uint64)
 
     def _solve_cfg(self, k):
         lv = self.levels[k]
@@ -249,7 +282,18 @@ class Pipeline:
             lv = self.levels[k]
             coeff = prefilter(targets[k], self.fir) if o["interpolation"] == "bspline" else None
             cfg = self._solve_cfg(k)
-            if k == self.levels_count - 1:
+            if k == self.levels_count - 1 and self.temporal:
+                # project_temporal_seed (full_pyramid.wgsl:437-449): finest history -> coarsest grid.
+                fine = self.levels[0]
+                scfg = SeedCfg(fine.grid_width, fine.grid_height, lv.grid_width, lv.grid_width * lv.grid_height,
+                               1.0, float(1 << k), float(o["subset_size"]), float(o["subset_stride"]),
+                               MODELS[o["motion_model"]], 0)
+                seed, active, weight = transfer_seed(scfg, self.history, self.temporal_links, return_weight=True)
+                links = np.full(lv.grid_width * lv.grid_height, 31, np.uint32) if self.temporal_links is None \
+                    else np.asarray(self.temporal_links, np.uint32)
+                seeded = int(np.sum((weight > 0) & ((links & 16) != 0)))
+                state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
+            elif k == self.levels_count - 1:
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k])
             else:
                 src = self.levels[k + 1]
@@ -259,6 +303,11 @@ class Pipeline:
                 seed, active = transfer_seed(scfg, outputs[k + 1], lv.seed_links)
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
             outputs[k] = self._median(k, state)
+            raw_fine = state
+        if self.temporal:
+            merged = temporal_merge(self.history, self.ages, raw_fine, outputs[0], self.outlier_policy == "replace",
+                                    self.temporal_max_age)
+            self.last_counts = (seeded, merged)
         stride = 16 if o["motion_model"] == "quadratic" else 8
         fine = outputs[0].reshape(self.levels[0].grid_height, self.levels[0].grid_width, stride)
         return (fine, outputs) if return_levels else fine

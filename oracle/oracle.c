@@ -626,7 +626,7 @@ static void accumulate_seed(const ora_seed_cfg* c, const float* src_state, int s
 }
 
 void ora_transfer_seed(const ora_seed_cfg* c, const float* src_state, const uint32_t* permissions,
-                       float* dst_seed, uint32_t* dst_active) {
+                       float* dst_seed, uint32_t* dst_active, float* dst_weight /* nullable */) {
     int q = c->motion_model == 2;
     int stride = q ? 16 : 8;
     for (int di = 0; di < c->destination_grid_count; ++di) {
@@ -686,6 +686,7 @@ void ora_transfer_seed(const ora_seed_cfg* c, const float* src_state, const uint
             o[q ? 12 : 6] = marker;
         }
         dst_active[di] = (perm >> 4u) & 1u;
+        if (dst_weight) dst_weight[di] = wsum;
     }
 }
 
