@@ -37,7 +37,23 @@ def test_pyramid_schedule_validation():
     with pytest.raises(ValueError, match="iterations"):
         ICGNWGPUFullPyramid(pyramid_levels=2, iterations_per_level=(8, 31))
     with pytest.raises(ValueError, match="initialization"):
-        ICGNWGPUFullPyramid(initialization="temporal")
+        ICGNWGPUFullPyramid(initialization="fourier")
+    with pytest.raises(ValueError, match="initialization"):
+        ICGNWGPUFullPyramid(initialization="predictive")
+
+
+def test_temporal_arguments_are_rejected_before_device_creation():
+    """test_temporal_init.py:15-27 of the reference."""
+    for value in (None, "", "predictive", 1):
+        with pytest.raises(ValueError, match="initialization"):
+            ICGNWGPUFullPyramid(initialization=value)
+    for value in (-1, True, 1.5, "2", 2**32):
+        with pytest.raises(ValueError, match="temporal_max_age"):
+            ICGNWGPUFullPyramid(temporal_max_age=value)
+    with pytest.raises(ValueError, match="temporal_prediction"):
+        ICGNWGPUFullPyramid(temporal_prediction="velocity")
+    with pytest.raises(ValueError, match="temporal_prediction"):
+        ICGNWGPUFullPyramid(initialization="temporal", temporal_prediction="sideways")
 
 
 def test_public_names():
