@@ -155,6 +155,18 @@ typedef struct ldic_initialization {
                                   0xFFFFFFFE keeps it (temporal_max_age=None) */
 } ldic_initialization;
 
+/* Partial-support subsets of one level (roi_policy="partial"; roi.py:266-355,
+ * partial.py:26-130): their list, support bits (ceil(S*S/32) little-endian
+ * words each, row-major pixels), local support boxes (xmin, ymin, xmax, ymax)
+ * and the float64-derived metadata rows (validity -4 when admitted) and
+ * inverse Hessians computed by the host planner from the prepared reference
+ * (ldic_export_reference).  Call after ldic_set_reference; each frame then
+ * solves these subsets in a partial pass after the full-window pass
+ * (solve_partial*, icgn.wgsl:1040-1085).  Translation and affine only. */
+int ldic_set_partial(struct ldic_engine* engine, int32_t level, int32_t count, const uint32_t* indices,
+                     const uint32_t* support_words, const float* boxes, const float* metadata_rows,
+                     const float* inverse_rows);
+
 /* Forget the temporal history (reset_temporal_state, full_pyramid_backend.py:
  * 660-677).  ldic_set_reference and a change of initialisation also reset it.
  * Frames must be registered in chronological order in temporal mode. */

@@ -33,6 +33,12 @@ struct SolveArgs {
     const uint32_t* active; // optional per-subset enable flags
     float* state;
     float* zncc_out;        // optional: ZNCC of the last iteration (at its pre-update warp), NaN if none
+    // Partial-support pass (weighted variants only): the listed subsets, their
+    // support bits (ceil(S*S/32) words each) and local support boxes.
+    const uint32_t* partial_list;
+    int partial_count;
+    const uint32_t* support;
+    const float4* support_boxes;
 };
 
 struct SeedArgs {
@@ -122,6 +128,9 @@ hipError_t launch_temporal_merge(int count, int model, const float* raw, const f
                                  uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
                                  hipStream_t s);
 hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s);
+// dst[index[i] * row + k] = src[i * row + k]
+hipError_t launch_scatter_rows(int count, int row, const uint32_t* index, const float* src, float* dst,
+                               hipStream_t s);
 hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);
 
 // Exhaustive integer ZNCC search; LDS bytes per workgroup for (S, R).

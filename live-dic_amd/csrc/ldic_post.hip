@@ -160,6 +160,20 @@ hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s
     return hipGetLastError();
 }
 
+__global__ void k_scatter_rows(int count, int row, const uint32_t* index, const float* src, float* dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * row) return;
+    const int i = t / row, k = t - i * row;
+    dst[(size_t)index[i] * row + k] = src[t];
+}
+
+hipError_t launch_scatter_rows(int count, int row, const uint32_t* index, const float* src, float* dst,
+                               hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_scatter_rows<<<(count * row + 255) / 256, 256, 0, s>>>(count, row, index, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s) {
     if (a.dst_count <= 0) return hipSuccess;
     k_transfer_seed<<<(a.dst_count + 255) / 256, 256, 0, s>>>(a);

@@ -95,6 +95,11 @@ struct Level {
     float* roi_words = nullptr;
     uint32_t* seed_links = nullptr;
     bool roi = false;
+    // Partial-support subsets (roi_policy="partial"): list, support bits, boxes.
+    uint32_t* partial_list = nullptr;
+    int partial_count = 0;
+    uint32_t* support = nullptr;
+    float4* support_boxes = nullptr;
     ldic_level_config config{30, 1e-3f, 0.8f};
     int count() const { return gw * gh; }
 };
@@ -297,6 +302,30 @@ SearchArgs search_args(const ldic_engine* e, int k, const SlotLevel& sl, int rad
     return q;
 }
 
+// The partial-support pass of a level (solve_partial / solve_partial_seeded /
+// retry_partial_seeded, icgn.wgsl:1040-1085): the weighted solver variant over
+// the level's partial list, uniform weights expressed as an infinite sigma.
+int launch_partial(ldic_engine* e, int k, SlotLevel& sl, const float* seed, const uint32_t* active, int preserve,
+                   hipStream_t st, float* zncc_out = nullptr) {
+    const Level& L = e->lv[k];
+    if (L.partial_count <= 0) return LDIC_OK;
+    SolveArgs a = solve_args(e, k, sl, seed, active, preserve);
+    a.partial_list = L.partial_list;
+    a.partial_count = L.partial_count;
+    a.support = L.support;
+    a.support_boxes = L.support_boxes;
+    a.zncc_out = zncc_out;
+    if (e->opt.spatial_weighting == 0) a.inv_var = 0.0f;
+    HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, 1, a, st));
+    return LDIC_OK;
+}
+
+int has_partial(const ldic_engine* e) {
+    for (int k = 0; k < e->levels; ++k)
+        if (e->lv[k].partial_count > 0) return 1;
+    return 0;
+}
+
 int launch_median_level(ldic_engine* e, int k, SlotLevel& sl, bool seed_available, hipStream_t st) {
     if (e->opt.outlier_policy == kDisabled) return LDIC_OK;
     const Level& L = e->lv[k];
@@ -412,6 +441,7 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
         HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting,
                                solve_args(e, k, sl, seed, active, 0), st));
+        CHECK(launch_partial(e, k, sl, seed, active, 0, st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
         CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
     }
@@ -470,6 +500,7 @@ int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
     // iteration (at the warp it then updates by less than the tolerance).
     if (fallback) first.zncc_out = s->zncc;
     HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting, first, st));
+    CHECK(launch_partial(e, 0, sl, sl.seed, sl.active, 0, st, first.zncc_out));
     if (fallback) {
         HIP_CHECK(launch_retry_mask(L.count(), model, sl.state, s->zncc, e->init.fallback_zncc,
                                     L.roi ? L.selected : nullptr, sl.mask, st));
@@ -482,6 +513,7 @@ int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
         HIP_CHECK(launch_zncc_search(q, st));
         HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting,
                                solve_args(e, 0, sl, sl.retry_seed, sl.retry_active, 1), st));
+        CHECK(launch_partial(e, 0, sl, sl.retry_seed, sl.retry_active, 1, st));
     }
     HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
     CHECK(launch_median_level(e, 0, sl, false, st));
@@ -889,6 +921,8 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
     double wall = now_ms();
     CHECK(check_ready(e));
     if (!reference || !target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "reference, target and out_state are required");
+    if (has_partial(e))
+        return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support partial-support ROI subsets");
     const Level& L0 = e->lv[0];
     if (reference_stride < L0.width || target_stride < L0.width)
         return set_error(LDIC_ERR_ARGUMENT, "row strides are smaller than the width");
@@ -916,6 +950,8 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
                                 const uint8_t* device_target, int64_t row_stride) {
     CHECK(check_slot(e, slot));
     if (!device_reference || !device_target) return set_error(LDIC_ERR_ARGUMENT, "device pointers are required");
+    if (has_partial(e))
+        return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support partial-support ROI subsets");
     Slot* s = e->slots[slot];
     if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
     HIP_CHECK(hipSetDevice(e->device));
@@ -962,6 +998,8 @@ int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stri
         return set_error(LDIC_ERR_ARGUMENT, "target, seed_state, active and out_state are required");
     if (include_zncc && (e->opt.motion_model != kAffine || !out_zncc))
         return set_error(LDIC_ERR_ARGUMENT, "include_zncc requires the affine model and an output buffer");
+    if (include_zncc && (e->opt.spatial_weighting != 0 || has_partial(e)))
+        return set_error(LDIC_ERR_ARGUMENT, "include_zncc requires uniform full windows");
     HIP_CHECK(hipSetDevice(e->device));
     Slot* s = e->slots[0];
     if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
@@ -1037,6 +1075,8 @@ int ldic_prepare_reference_resident(ldic_engine* e, int32_t slot, const uint8_t*
                                     int64_t row_stride) {
     CHECK(check_slot(e, slot));
     if (!device_reference) return set_error(LDIC_ERR_ARGUMENT, "device_reference is NULL");
+    if (has_partial(e))
+        return set_error(LDIC_ERR_STATE, "a new reference would discard the partial-support statistics");
     Slot* s = e->slots[slot];
     if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
     if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
@@ -1061,6 +1101,43 @@ int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
     const bool restart = init->kind == LDIC_INIT_TEMPORAL && e->init.kind != LDIC_INIT_TEMPORAL;
     e->init = *init;
     if (restart && e->ready) CHECK(ldic_reset_temporal(e));
+    return LDIC_OK;
+}
+
+int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_t* indices,
+                     const uint32_t* support_words, const float* boxes, const float* metadata_rows,
+                     const float* inverse_rows) {
+    CHECK(check_ready(e));
+    if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
+    if (count < 0 || (count > 0 && (!indices || !support_words || !boxes || !metadata_rows || !inverse_rows)))
+        return set_error(LDIC_ERR_ARGUMENT, "partial tables are required");
+    if (count > 0 && e->opt.motion_model == kQuadratic)
+        return set_error(LDIC_ERR_ARGUMENT, "partial supports exist for translation and affine only");
+    for (Slot* s : e->slots)
+        if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
+    Level& L = e->lv[level];
+    const int S = e->opt.subset_size, W = (S * S + 31) / 32;
+    const int M = metadata_stride(e->opt.motion_model), P = param_count(e->opt.motion_model);
+    for (int i = 0; i < count; ++i)
+        if (indices[i] >= (uint32_t)L.count()) return set_error(LDIC_ERR_ARGUMENT, "partial index %u out of range", indices[i]);
+    HIP_CHECK(hipSetDevice(e->device));
+    L.partial_count = 0;
+    if (count == 0) return LDIC_OK;
+    float *rows = nullptr, *inv = nullptr;
+    CHECK(e->ref_arena.alloc(&L.partial_list, (size_t)count));
+    CHECK(e->ref_arena.alloc(&L.support, (size_t)count * W));
+    CHECK(e->ref_arena.alloc(&L.support_boxes, (size_t)count));
+    CHECK(e->ref_arena.alloc(&rows, (size_t)count * M));
+    CHECK(e->ref_arena.alloc(&inv, (size_t)count * P * P));
+    HIP_CHECK(hipMemcpy(L.partial_list, indices, sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(L.support, support_words, sizeof(uint32_t) * (size_t)count * W, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(L.support_boxes, boxes, sizeof(float4) * count, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(rows, metadata_rows, sizeof(float) * (size_t)count * M, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(inv, inverse_rows, sizeof(float) * (size_t)count * P * P, hipMemcpyHostToDevice));
+    HIP_CHECK(launch_scatter_rows(count, M, L.partial_list, rows, L.metadata, e->prep_stream));
+    HIP_CHECK(launch_scatter_rows(count, P * P, L.partial_list, inv, L.inverse, e->prep_stream));
+    HIP_CHECK(hipStreamSynchronize(e->prep_stream));
+    L.partial_count = count;
     return LDIC_OK;
 }
 
@@ -1492,6 +1569,8 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
             return set_error(LDIC_ERR_ARGUMENT, "all engines of a batch must use one device");
         if (engines[i]->init.kind == LDIC_INIT_TEMPORAL)
             return set_error(LDIC_ERR_ARGUMENT, "temporal initialisation is not supported in a batch");
+        if (has_partial(engines[i]))
+            return set_error(LDIC_ERR_ARGUMENT, "partial-support ROI subsets are not supported in a batch");
         for (int k = 0; k < i; ++k)
             if (engines[k] == engines[i]) return set_error(LDIC_ERR_ARGUMENT, "engine %d appears twice", i);
     }

@@ -134,10 +134,13 @@ __device__ __forceinline__ void displacement(const Warp& q, float lx, float ly, 
 
 // Corner (or 3x3 for quadratic) bounds test, icgn.wgsl:648-685.
 template <int MODEL>
-__device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, float he, float W, float H) {
+// (bx0, by0)-(bx1, by1): local box whose corners are checked, the whole window
+// (+-he) or a partial support's box (icgn.wgsl:638-660).
+__device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, float he, float W, float H, float bx0,
+                                            float by0, float bx1, float by1) {
     if (MODEL == kTranslation) {
-        float minx = cx - 0.5f + -he + q.T[0], miny = cy - 0.5f + -he + q.T[1];
-        float maxx = cx - 0.5f + he + q.T[0], maxy = cy - 0.5f + he + q.T[1];
+        float minx = cx - 0.5f + bx0 + q.T[0], miny = cy - 0.5f + by0 + q.T[1];
+        float maxx = cx - 0.5f + bx1 + q.T[0], maxy = cy - 0.5f + by1 + q.T[1];
         return minx >= 2.0f && miny >= 2.0f && maxx <= W - 3.0f && maxy <= H - 3.0f;
     }
     bool inside = true;
@@ -149,8 +152,8 @@ __device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, f
             lx = (float)(p % 3 - 1) * he;
             ly = (float)(p / 3 - 1) * he;
         } else {
-            lx = (p == 1 || p == 3) ? he : -he;
-            ly = p >= 2 ? he : -he;
+            lx = (p == 1 || p == 3) ? bx1 : bx0;
+            ly = p >= 2 ? by1 : by0;
         }
         float dx, dy;
         displacement<MODEL>(q, lx, ly, dx, dy);
@@ -311,8 +314,13 @@ __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, cons
 // PPL > 0: the reference tile of each lane lives in registers (PPL pixels per
 // lane, pixel p = lane + 64 k).  PPL == 0: any subset width, reference values
 // are re-read from the plane every iteration.
+// ordinal >= 0: a partial-support subset (entry `ordinal` of the partial list):
+// only its support pixels enter the sums (solve_partial*, icgn.wgsl:1040-1085),
+// its statistics carry validity -4, its own mean and weight sum, and its
+// support box replaces the window in the bounds check.
 template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
-__device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subset, const int lane, float* tile) {
+__device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subset, const int lane, float* tile,
+                                             const int ordinal = -1) {
     constexpr int P = MODEL == kQuadratic ? 12 : 6;
     constexpr int M = MODEL == kQuadratic ? 28 : 16;
     constexpr int STRIDE = MODEL == kQuadratic ? 16 : 8;
@@ -344,7 +352,9 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             for (int k = 0; k < 6; ++k) q.C[k] = sd[6 + k];
     }
     const float meta1 = meta[1];
-    float status = (enabled && meta1 >= 0.5f) ? 0.0f : (a.roi_enabled ? fminf(-1.0f, meta1) : -1.0f);
+    const bool partial = ordinal >= 0;
+    float status = partial ? ((enabled && meta1 == -4.0f) ? 0.0f : -1.0f)
+                           : ((enabled && meta1 >= 0.5f) ? 0.0f : (a.roi_enabled ? fminf(-1.0f, meta1) : -1.0f));
     float iters = 0.0f;
 
     const int S = a.S, D = a.D, SS = S * S;
@@ -352,7 +362,16 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     const float he = 0.5f * ((float)S - 1.0f);
     const float cx = 0.5f * (float)S + (float)gx * (float)D;
     const float cy = 0.5f * (float)S + (float)gy * (float)D;
-    const float ref_mean = (D != S) ? meta[MODEL == kQuadratic ? 26 : 14] : 0.0f;
+    const float ref_mean = (D != S || partial) ? meta[MODEL == kQuadratic ? 26 : 14] : 0.0f;
+    const uint32_t* support = partial ? a.support + (size_t)ordinal * ((SS + 31) / 32) : nullptr;
+    float bx0 = -he, by0 = -he, bx1 = he, by1 = he;
+    if (partial) {
+        const float4 box = a.support_boxes[ordinal];
+        bx0 = box.x - he;
+        by0 = box.y - he;
+        bx1 = box.z - he;
+        by1 = box.w - he;
+    }
     const float wsum = WEIGHTED ? meta[MODEL == kQuadratic ? 27 : 15] : (float)SS;
     const float Wf = (float)a.width, Hf = (float)a.height;
     const int ox = gx * D, oy = gy * D;
@@ -372,6 +391,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 plx[k] = (float)col - he;
                 ply[k] = (float)row - he;
                 if (WEIGHTED) pw[k] = expf(-0.5f * (plx[k] * plx[k] + ply[k] * ply[k]) * a.inv_var);
+                if (WEIGHTED && partial && !((support[p >> 5] >> (p & 31)) & 1u)) pw[k] = 0.0f;
             } else {
                 rf[k] = rgx[k] = rgy[k] = plx[k] = ply[k] = 0.0f;
                 if (WEIGHTED) pw[k] = 0.0f;
@@ -385,7 +405,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     float last_zncc = __builtin_nanf("");
 
     for (int it = 0; it < 30; ++it) {
-        bool inside = warp_inside<MODEL>(q, cx, cy, he, Wf, Hf);
+        bool inside = warp_inside<MODEL>(q, cx, cy, he, Wf, Hf, bx0, by0, bx1, by1);
         if (!inside && (!a.roi_enabled || status > -2.5f)) status = -1.0f;
         if (it >= a.iterations || fabsf(status) > 0.5f) break;
 
@@ -480,6 +500,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                     float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
                     float lx = (float)col - he, ly = (float)row - he;
                     float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
+                    if (partial && !((support[p >> 5] >> (p & 31)) & 1u)) iw = 0.0f;
                     accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw);
                 }
             }
@@ -583,9 +604,16 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int subset = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-    if (subset >= a.grid_w * a.grid_h) return;
-    solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, subset, lane, lds + wave * a.tile_cap);
+    const int item = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    if (a.partial_list) {
+        // Partial-support pass over the list of partial subsets.
+        if (item >= a.partial_count) return;
+        solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, (int)a.partial_list[item], lane, lds + wave * a.tile_cap,
+                                                       item);
+        return;
+    }
+    if (item >= a.grid_w * a.grid_h) return;
+    solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, item, lane, lds + wave * a.tile_cap);
 }
 
 // Persistent solve over the subset grids of several image pairs (one variant
@@ -648,7 +676,7 @@ size_t solver_lds_bytes(int S, int* tile_cap) {
 
 template <int MODEL, int INTERP, bool WEIGHTED>
 static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
-    int N = a.grid_w * a.grid_h;
+    int N = a.partial_list ? a.partial_count : a.grid_w * a.grid_h;
     if (N <= 0) return hipSuccess;
     int cap = 0;
     size_t lds = solver_lds_bytes(a.S, &cap);

@@ -139,6 +139,7 @@ SIGNATURES = {
     "ldic_register_seeded_resident": (ctypes.c_int, [_P, _I32, _P, _I64, _P, _P]),
     "ldic_export_search": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "ldic_reset_temporal": (ctypes.c_int, [_P]),
+    "ldic_set_partial": (ctypes.c_int, [_P, _I32, _I32, _P, _P, _P, _P, _P]),
     "ldic_set_temporal_links": (ctypes.c_int, [_P, _P]),
     "ldic_prepare_reference_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
     "ldic_batch_create": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(_P)]),

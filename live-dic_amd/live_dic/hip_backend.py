@@ -31,6 +31,7 @@ from .outlier import (
     normalize_median_filter_size,
     normalize_outlier_policy,
 )
+from .partial import partial_rows
 from .roi import integral, prepare_roi, reduce_mask, seed_permissions, validate_roi
 from .subset import DEFAULT_SUBSET_SIZE, normalize_subset_size, normalize_subset_stride, subset_grid_shape
 from .weighting import SPATIAL_WEIGHTING_CODES, normalize_gaussian_sigma, normalize_spatial_weighting
@@ -199,8 +200,14 @@ class _NativeEngine:
         self._reference_ready = False
         _native.check(self._lib.ldic_set_reference(self._handle, frame.ctypes.data, width, height, frame.strides[0],
                                                    cfgs, rois, valid.ctypes.data))
-        build_ms = (time.perf_counter() - started) * 1e3
         self._level_shapes = shapes
+        for k, plan in enumerate(plans):
+            if plan is None or plan.partial_indices is None or plan.partial_indices.size == 0:
+                continue
+            plans[k], admitted = self._upload_partial(k, plan)
+            if k == 0:
+                valid[admitted // shapes[0][2], admitted % shapes[0][2]] = 1
+        build_ms = (time.perf_counter() - started) * 1e3
         self.width, self.height = width, height
         self.grid_width, self.grid_height = shapes[0][2], shapes[0][3]
         self.roi = plans[0]
@@ -213,6 +220,29 @@ class _NativeEngine:
         self._slots = 1
         self._reference_ready = True
         return build_ms
+
+    def _upload_partial(self, level: int, plan):
+        """Float64 statistics of the level's partial-support subsets from the prepared reference
+        plane, uploaded with their support tables (wgpu_backend.py:2412-2450).  Returns the plan
+        with its support diagnostics and the flat indices of the admitted subsets."""
+        w, h, gw, gh = self._level_shapes[level]
+        reference = np.empty((h, w, 4), np.float32)
+        meta = np.empty((gw * gh, 16), np.float32)
+        inv = np.empty((gw * gh, 36), np.float32)
+        _native.check(self._lib.ldic_export_reference(self._handle, level, reference.ctypes.data, meta.ctypes.data,
+                                                      inv.ctypes.data))
+        rows, inverses, plan = partial_rows(reference, plan, width=w, size=self.subset_size,
+                                            stride=self.subset_stride, motion_model=self.motion_model,
+                                            weighting=self.spatial_weighting, sigma=self.gaussian_sigma)
+        indices = np.ascontiguousarray(plan.partial_indices, dtype=np.uint32)
+        words = np.ascontiguousarray(plan.support_words, dtype=np.uint32)
+        boxes = np.ascontiguousarray(plan.support_boxes, dtype=np.float32)
+        inverse = np.ascontiguousarray(inverses, dtype=np.float32)
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        _native.check(self._lib.ldic_set_partial(self._handle, level, int(indices.size), indices.ctypes.data,
+                                                 words.ctypes.data, boxes.ctypes.data, rows.ctypes.data,
+                                                 inverse.ctypes.data))
+        return plan, indices[rows[:, 1] == -4.0].astype(np.int64)
 
     def _reference_info(self, started: float, build_ms: float, config: dict) -> ReferenceInfo:
         total = (time.perf_counter() - started) * 1e3
@@ -512,7 +542,8 @@ class ICGNWGPU(_NativeEngine):
         return self._run(frame, started, prep_ms, config)
 
     def _check_final_zncc(self) -> None:
-        if self.motion_model != "affine" or self.spatial_weighting != "uniform":
+        partial = self.roi is not None and self.roi.partial_indices is not None and self.roi.partial_indices.size
+        if self.motion_model != "affine" or self.spatial_weighting != "uniform" or partial:
             raise ValueError("include_zncc requires affine motion and uniform full windows")
 
     def _seeded(self, frame, values, mask, include_zncc, started, prep_ms, config) -> RegistrationResult:

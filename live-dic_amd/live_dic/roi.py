@@ -2,24 +2,36 @@
 
 Runs once per reference on the host; nothing here runs per target.  A subset
 is selected when its whole S x S window lies inside the mask (``full`` policy).
-Neighbour and seed links require the closed pixel box around the segment
-joining two subset centres to be free of masked pixels.
+Under the ``partial`` policy (roi.py:266-355) a boundary subset is also
+selected when the 4-connected component of allowed pixels anchored on its
+centre is not empty; that component (optionally cut to the pixels visible from
+the centre, ``roi_truncation="visibility"``) is its support.  Neighbour and
+seed links require the closed pixel box around the segment joining two subset
+centres to be free of masked pixels.
 
 The reference rejects ROIs for the quadratic model (roi.py:45-46); this build
-supports them with the same full-window rule, since the solver only needs the
-selection flags and link words.  The ``partial`` policy is not built.
+supports them under the full policy with the same full-window rule, since the
+solver only needs the selection flags and link words.  Partial supports exist
+for translation and affine, as in the reference.
 """
 
 from __future__ import annotations
 
 import hashlib
 import struct
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
+from functools import lru_cache
 
 import numpy as np
 
 from .outlier import normalize_median_filter_size
 from .subset import subset_grid_shape
+
+
+# Partial-support admission limits (roi.py:20-21): normalised Hessian condition
+# and weighted RMS contrast of the support.
+PARTIAL_MAX_NORMALIZED_CONDITION = 1e8
+PARTIAL_MIN_WEIGHTED_CONTRAST = 1.0
 
 
 def immutable(array: np.ndarray) -> np.ndarray:
@@ -37,8 +49,8 @@ def validate_roi(mask, policy: str, shape: tuple[int, int], motion_model: str, t
         raise ValueError("roi_truncation requires roi_policy='partial' and a mask")
     if mask is None:
         return None
-    if policy == "partial":
-        raise ValueError("roi_policy='partial' is not supported by the HIP engine; use 'full'")
+    if policy == "partial" and motion_model not in ("translation", "affine"):
+        raise ValueError("roi_policy='partial' supports only translation and affine motion models")
     if not isinstance(mask, np.ndarray) or mask.dtype != np.bool_:
         raise TypeError("roi_mask must be a two-dimensional boolean NumPy array")
     if mask.ndim != 2 or mask.shape != tuple(shape):
@@ -102,6 +114,12 @@ class RoiPlan:
     truncation: str = "none"
     partial_indices: np.ndarray | None = None
     median_neighbour_words: np.ndarray | None = None
+    support_words: np.ndarray | None = None     # (partial, ceil(S*S/32)) uint32, row-major, bit 0 first
+    support_boxes: np.ndarray | None = None     # (partial, 4) float32 xmin, ymin, xmax, ymax (local pixels)
+    coverage: np.ndarray | None = None          # (gh, gw) support fraction of each selected subset
+    support_condition: np.ndarray | None = None
+    support_rank: np.ndarray | None = None
+    support_contrast: np.ndarray | None = None
 
     @property
     def empty(self) -> bool:
@@ -111,7 +129,11 @@ class RoiPlan:
         return {
             "policy": self.policy,
             "truncation": self.truncation,
-            "partial": 0,
+            "partial": 0 if self.partial_indices is None else int(self.partial_indices.size),
+            **({"supportConditionLimit": PARTIAL_MAX_NORMALIZED_CONDITION,
+                "supportContrastRmsMinimum": PARTIAL_MIN_WEIGHTED_CONTRAST,
+                "supportConnectivity": 4,
+                "evenCenter": "all four central pixels required"} if self.policy == "partial" else {}),
             "maskShape": list(self.mask.shape),
             "maskSha256": self.mask_hash,
             "gridTotal": int(self.selected.size),
@@ -121,7 +143,14 @@ class RoiPlan:
         }
 
     def archive_fields(self) -> dict:
-        return {"roi_mask": self.mask, "roi_selected": self.selected}
+        fields = {"roi_mask": self.mask, "roi_selected": self.selected}
+        if self.partial_indices is not None:
+            fields.update(roi_partial_indices=self.partial_indices, roi_support_words=self.support_words,
+                          roi_support_boxes=self.support_boxes, roi_coverage=self.coverage)
+            if self.support_condition is not None:
+                fields.update(roi_support_condition=self.support_condition, roi_support_rank=self.support_rank,
+                              roi_support_contrast=self.support_contrast)
+        return fields
 
     def neighbour_words(self) -> np.ndarray:
         """(grid, 4) float32 link words as appended to the solver metadata."""
@@ -132,19 +161,110 @@ class RoiPlan:
         return words
 
 
+def pack_support(support: np.ndarray) -> np.ndarray:
+    """Row-major pixels as little-endian uint32 words, pixel 0 in bit 0 (roi.py:266-270)."""
+    bits = np.packbits(np.asarray(support, bool).ravel(), bitorder="little")
+    bits = np.pad(bits, (0, (-bits.size) % 4))
+    return bits.view("<u4").astype(np.uint32)
+
+
+def unpack_support(words: np.ndarray, size: int) -> np.ndarray:
+    raw = np.asarray(words, dtype="<u4").view(np.uint8)
+    return np.unpackbits(raw, bitorder="little")[: size * size].reshape(size, size).astype(bool)
+
+
+def central_component(support: np.ndarray) -> np.ndarray:
+    """4-connected component of allowed pixels grown from the central pixel; for an even width
+    all four central pixels must be allowed, otherwise the component is empty (roi.py:280-299)."""
+    size = support.shape[0]
+    lo, hi = (size - 1) // 2, size // 2
+    out = np.zeros_like(support, dtype=bool)
+    if not support[lo:hi + 1, lo:hi + 1].all():
+        return out
+    out[lo, lo] = True
+    stack = [(lo, lo)]
+    while stack:
+        y, x = stack.pop()
+        for ny, nx in ((y - 1, x), (y + 1, x), (y, x - 1), (y, x + 1)):
+            if 0 <= ny < size and 0 <= nx < size and support[ny, nx] and not out[ny, nx]:
+                out[ny, nx] = True
+                stack.append((ny, nx))
+    return out
+
+
+@lru_cache(maxsize=7)
+def visibility_paths(size: int) -> np.ndarray:
+    """(S*S, words) packed sets: the closed pixel cells that the segment from the window centre
+    to each pixel centre touches (slab test per axis; touching a cell edge or corner counts),
+    the visibility rule of roi.py:302-330."""
+    n = size * size
+    centre = (size - 1) / 2
+    yy, xx = np.divmod(np.arange(n, dtype=np.float64), size)
+    t_lo = np.zeros((n, n))
+    t_hi = np.ones((n, n))
+    for coord in (xx, yy):                       # rows: target pixel, columns: cell
+        d = coord - centre
+        for i in range(n):
+            if d[i] == 0.0:
+                outside = np.abs(coord - centre) > 0.5
+                t_hi[i, outside] = -1.0
+                continue
+            a = (coord - 0.5 - centre) / d[i]
+            b = (coord + 0.5 - centre) / d[i]
+            t_lo[i] = np.maximum(t_lo[i], np.minimum(a, b))
+            t_hi[i] = np.minimum(t_hi[i], np.maximum(a, b))
+    paths = t_lo <= t_hi
+    out = np.stack([pack_support(row) for row in paths])
+    out.setflags(write=False)
+    return out
+
+
+def partial_windows(mask: np.ndarray, complete: np.ndarray, size: int, stride: int, *, truncation: str):
+    """Selection, indices, support words, support boxes and coverage of the partial policy
+    (roi.py:333-355): every incomplete window whose central component is not empty."""
+    if truncation not in ("none", "visibility"):
+        raise ValueError("roi_truncation must be 'none' or 'visibility'")
+    selected = complete.copy()
+    coverage = complete.astype(np.float32)
+    indices, words, boxes = [], [], []
+    paths = visibility_paths(size) if truncation == "visibility" else None
+    for y, x in np.argwhere(~complete):
+        local = mask[y * stride:y * stride + size, x * stride:x * stride + size]
+        support = central_component(local)
+        if support.any() and paths is not None:
+            packed = pack_support(support)
+            visible = ~np.any(paths & ~packed, axis=1)
+            support = central_component(support & visible.reshape(size, size))
+        if not support.any():
+            continue
+        selected[y, x] = True
+        indices.append(y * complete.shape[1] + x)
+        words.append(pack_support(support))
+        sy, sx = np.nonzero(support)
+        boxes.append((sx.min(), sy.min(), sx.max(), sy.max()))
+        coverage[y, x] = support.sum() / (size * size)
+    nwords = (size * size + 31) // 32
+    return (selected, immutable(np.asarray(indices, np.uint32)),
+            immutable(np.asarray(words, np.uint32).reshape(-1, nwords)),
+            immutable(np.asarray(boxes, np.float32).reshape(-1, 4)), immutable(coverage))
+
+
 def prepare_roi(mask, size: int, stride: int, *, policy: str = "full", truncation: str = "none",
                 median_filter_size: int = 3) -> RoiPlan | None:
     """Selection, 3x3 neighbour bits and optional 9x9 median words (roi.py:178-263)."""
     median_filter_size = normalize_median_filter_size(median_filter_size)
     if mask is None:
         return None
-    if policy != "full" or truncation != "none":
-        raise ValueError("only the full ROI policy is supported")
+    if policy not in ("full", "partial") or (policy == "full" and truncation != "none"):
+        raise ValueError("Unknown ROI policy/truncation")
     h, w = mask.shape
     every = bool(mask.all())
     prefix = None if every else integral(mask)
     gw, gh = subset_grid_shape(w, h, subset_size=size, subset_stride=stride)
     selected = np.ones((gh, gw), dtype=bool) if every else select_windows(mask, size, stride, summed_area=prefix)
+    partial = (None,) * 4
+    if policy == "partial":
+        selected, *partial = partial_windows(mask, selected, size, stride, truncation=truncation)
     gy, gx = np.indices(selected.shape)
     cx = gx * stride + (size - 1) / 2
     cy = gy * stride + (size - 1) / 2
@@ -171,8 +291,9 @@ def prepare_roi(mask, size: int, stride: int, *, policy: str = "full", truncatio
                 words[..., bit // 24] += linked(dx, dy).astype(np.float32) * float(1 << (bit % 24))
         words = immutable(words.reshape(-1, 4))
     digest = hashlib.sha256(b"live-dic-roi-v1\0" + struct.pack("<QQ", *mask.shape) + mask.tobytes(order="C")).hexdigest()
+    indices, support_words, boxes, coverage = partial
     return RoiPlan(mask, immutable(selected), immutable(bits), digest, not every, int(selected.sum()),
-                   policy, truncation, None, words)
+                   policy, truncation, indices, words, support_words, boxes, coverage)
 
 
 def seed_permissions(source: RoiPlan, destination: RoiPlan, *, source_scale: int, destination_scale: int,

@@ -121,6 +121,29 @@ def solve(cfg: SolveCfg, reference, metadata, inverse, coeff, target, seed=None,
     return state
 
 
+def solve_partial(cfg: SolveCfg, reference, metadata, inverse, coeff, target, indices, words, boxes, state,
+                  seed=None, active=None) -> np.ndarray:
+    """The partial-support pass (``ora_solve_partial``) over ``state`` (n, 8), in place."""
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    if idx.size == 0:
+        return state
+    ref = np.ascontiguousarray(reference, dtype=np.float32)
+    meta = np.ascontiguousarray(metadata, dtype=np.float32)
+    inv = np.ascontiguousarray(inverse, dtype=np.float32)
+    co = None if coeff is None else np.ascontiguousarray(coeff, dtype=np.float32)
+    tg = None if target is None else np.ascontiguousarray(target, dtype=np.uint8)
+    sd = None if seed is None else np.ascontiguousarray(seed, dtype=np.float32)
+    ac = None if active is None else np.ascontiguousarray(active, dtype=np.uint32)
+    wd = np.ascontiguousarray(words, dtype=np.uint32)
+    bx = np.ascontiguousarray(boxes, dtype=np.float32)
+    assert state.flags.c_contiguous and state.dtype == np.float32
+    rc = lib().ora_solve_partial(ctypes.byref(cfg), _p(ref), _p(meta), _p(inv), _p(co), _p(tg), _p(sd), _p(ac),
+                                 ctypes.c_int(idx.size), _p(idx), _p(wd), _p(bx), _p(state))
+    if rc != 0:
+        raise ValueError("oracle partial solve rejected its configuration")
+    return state
+
+
 def transfer_seed(cfg: SeedCfg, source_state, permissions=None, return_weight=False):
     stride = 16 if cfg.motion_model == 2 else 8
     seed = np.zeros((cfg.destination_grid_count, stride), dtype=np.float32)
@@ -195,6 +218,20 @@ class Level:
             else:
                 self.roi_words[:, 0] = roi["neighbour_bits"].ravel()
             self.seed_links = roi.get("seed_links")
+        # Partial supports (roi_policy="partial"): FP64 statistics replace the rows.
+        self.partial = None
+        if roi is not None and roi.get("partial_indices") is not None and len(roi["partial_indices"]):
+            from .partial import partial_reference
+
+            plane = self.reference.reshape(self.height, self.width, 4)
+            meta = self.metadata.reshape(-1, 16)
+            inv = self.inverse.reshape(-1, 36)
+            partial_reference(plane, meta, inv, roi["partial_indices"], roi["support_words"],
+                              grid_width=self.grid_width, size=opts["subset_size"], stride=opts["subset_stride"],
+                              motion_model=opts["motion_model"], weighting=opts["spatial_weighting"],
+                              sigma=opts["gaussian_sigma"])
+            self.partial = (np.asarray(roi["partial_indices"], np.uint32), np.asarray(roi["support_words"], np.uint32),
+                            np.asarray(roi["support_boxes"], np.float32))
 
 
 class Pipeline:
@@ -294,6 +331,7 @@ class Pipeline:
                 seeded = int(np.sum((weight > 0) & ((links & 16) != 0)))
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
             elif k == self.levels_count - 1:
+                seed = active = None
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k])
             else:
                 src = self.levels[k + 1]
@@ -302,6 +340,9 @@ class Pipeline:
                                float(o["subset_stride"]), MODELS[o["motion_model"]], 0)
                 seed, active = transfer_seed(scfg, outputs[k + 1], lv.seed_links)
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
+            if lv.partial is not None:
+                solve_partial(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], *lv.partial, state,
+                              seed, active)
             outputs[k] = self._median(k, state)
             raw_fine = state
         if self.temporal:

@@ -198,8 +198,8 @@ static float spatial_weight(const ora_solve_cfg* c, float lx, float ly) {
     return 1.0f;
 }
 
-static float integration_weight_sum(const solve_ctx* s, int subset) {
-    if (s->c->spatial_weighting == 1) {
+static float integration_weight_sum(const solve_ctx* s, int subset, int partial) {
+    if (s->c->spatial_weighting == 1 || partial) {
         int mean_offset = s->c->motion_model == 2 ? 26 : 14;
         return metadata_value(s, subset, mean_offset + 1);
     }
@@ -352,7 +352,14 @@ static float convergence_status(const ora_solve_cfg* c, float change, float zncc
 }
 
 /* icgn.wgsl:595-958 for one subset. */
-static void solve_subset(const solve_ctx* s, int subset, const float* seed, int enabled, float* out) {
+/* support == NULL: the full window.  Otherwise a partial-support subset
+ * (solve_partial*, icgn.wgsl:1040-1085): only support pixels (row-major bits,
+ * pixel 0 in bit 0 of word 0) enter the sums, the subset must carry metadata
+ * validity -4 (icgn.wgsl:611-616), rejection is -1, its own mean and weight
+ * sum are used (icgn.wgsl:630-636) and the support box (xmin, ymin, xmax,
+ * ymax, local pixels) bounds the warp test (icgn.wgsl:638-642). */
+static void solve_subset(const solve_ctx* s, int subset, const float* seed, int enabled, float* out,
+                         const uint32_t* support, const float* box) {
     const ora_solve_cfg* c = s->c;
     const int model = c->motion_model;
     const int S = c->subset_size, D = c->subset_stride;
@@ -368,10 +375,11 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
             C[0] = seed[6]; C[1] = seed[7]; C[2] = seed[8]; C[3] = seed[9]; C[4] = seed[10]; C[5] = seed[11];
         }
     }
-    int reference_valid = metadata_value(s, subset, 1) >= 0.5f;
+    const int partial = support != NULL;
+    int reference_valid = partial ? metadata_value(s, subset, 1) == -4.0f : metadata_value(s, subset, 1) >= 0.5f;
     int valid = enabled && reference_valid;
     float rejected = -1.0f;
-    if (c->roi_enabled) {
+    if (c->roi_enabled && !partial) {
         float m1 = metadata_value(s, subset, 1);
         rejected = fminf(-1.0f, m1);
     }
@@ -383,20 +391,27 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
     const float cy = 0.5f * (float)S + (float)gy * (float)D;
     const float he = half_extent(c);
     const int mean_offset = model == 2 ? 26 : 14;
-    const float ref_mean = (D != S) ? metadata_value(s, subset, mean_offset) : 0.0f;
+    const float ref_mean = (D != S || partial) ? metadata_value(s, subset, mean_offset) : 0.0f;
     const float W = (float)c->width, H = (float)c->height;
+    float bx0 = -he, by0 = -he, bx1 = he, by1 = he;
+    if (partial) {
+        bx0 = box[0] - he;
+        by0 = box[1] - he;
+        bx1 = box[2] - he;
+        by1 = box[3] - he;
+    }
 
     for (int it = 0; it < 30; ++it) {
         int inside = 1;
         if (model == 1) {
-            float minx = cx - 0.5f + -he + T[0], miny = cy - 0.5f + -he + T[1];
-            float maxx = cx - 0.5f + he + T[0], maxy = cy - 0.5f + he + T[1];
+            float minx = cx - 0.5f + bx0 + T[0], miny = cy - 0.5f + by0 + T[1];
+            float maxx = cx - 0.5f + bx1 + T[0], maxy = cy - 0.5f + by1 + T[1];
             inside = minx >= 2.0f && miny >= 2.0f && maxx <= W - 3.0f && maxy <= H - 3.0f;
         } else {
             int count = model == 2 ? 9 : 4;
             for (int p = 0; p < count; ++p) {
-                float lx = (p == 1 || p == 3) ? he : -he;
-                float ly = p >= 2 ? he : -he;
+                float lx = (p == 1 || p == 3) ? bx1 : bx0;
+                float ly = p >= 2 ? by1 : by0;
                 if (model == 2) {
                     lx = (float)(p % 3 - 1) * he;
                     ly = (float)(p / 3 - 1) * he;
@@ -442,6 +457,7 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
                 float f = rv[0] - ref_mean;
                 float gxv = rv[1], gyv = rv[2];
                 float iw = spatial_weight(c, lx, ly);
+                if (partial && !((support[pix >> 5] >> (pix & 31)) & 1u)) iw = 0.0f;
                 float wc = iw * cur;
                 float gxg = gxv * wc, gyg = gyv * wc;
                 sc = sc + wc;
@@ -478,7 +494,7 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
                 }
             }
         }
-        float mean = pa[0][0] / integration_weight_sum(s, subset);
+        float mean = pa[0][0] / integration_weight_sum(s, subset, partial);
         float norm = sqrtf(fmaxf(0.0f, pa[0][1] - pa[0][0] * mean));
         float ref_norm = metadata_value(s, subset, 0);
         iters = (float)(it + 1);
@@ -567,7 +583,32 @@ int ora_solve(const ora_solve_cfg* c, const float* reference, const float* metad
                 for (int k = 0; k < 6; ++k) seed_params[k] = seed[k];
             }
         }
-        solve_subset(&s, i, seed ? seed_params : NULL, enabled, state + (size_t)i * stride);
+        solve_subset(&s, i, seed ? seed_params : NULL, enabled, state + (size_t)i * stride, NULL, NULL);
+    }
+    return 0;
+}
+
+/* The partial pass (solve_partial / solve_partial_seeded / retry_partial_seeded,
+ * icgn.wgsl:1063-1085) over `count` listed subsets, run after ora_solve on the
+ * same state: support words ceil(S*S/32) per entry, boxes 4 floats per entry. */
+int ora_solve_partial(const ora_solve_cfg* c, const float* reference, const float* metadata, const float* inverse,
+                      const float* coeff, const uint8_t* target, const float* seed_state, const uint32_t* active,
+                      int count, const uint32_t* indices, const uint32_t* support_words, const float* boxes,
+                      float* state) {
+    if (c->lanes < 1 || c->lanes > MAX_LANES || (c->lanes & (c->lanes - 1))) return -1;
+    if (c->motion_model == 2) return -1;
+    solve_ctx s = {c, reference, metadata, inverse, coeff, target};
+    const int n = c->grid_width * c->grid_height;
+    const int words = (c->subset_size * c->subset_size + 31) / 32;
+    for (int k = 0; k < count; ++k)
+        if ((int)indices[k] >= n) return -1;
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < count; ++k) {
+        const int i = (int)indices[k];
+        const float* seed = seed_state ? seed_state + (size_t)i * 8 : NULL;
+        int enabled = active ? active[i] != 0u : 1;
+        if (!enabled && c->preserve_disabled) continue;
+        solve_subset(&s, i, seed, enabled, state + (size_t)i * 8, support_words + (size_t)k * words, boxes + 4 * k);
     }
     return 0;
 }

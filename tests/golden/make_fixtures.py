@@ -30,6 +30,13 @@ CASES = (
     "affine-l1-full-zero",
     "translation-l2-full-zero",
 )
+# Partial-support ROI cases: frame 1 only (the tensile pair, or the (-1, 2) roll for translation).
+PARTIAL_CASES = (
+    "translation-l1-partial-zero",
+    "affine-l1-partial-zero",
+    "translation-l2-partial-zero",
+    "affine-l2-partial-zero",
+)
 
 
 def main() -> None:
@@ -44,6 +51,10 @@ def main() -> None:
                 for key in ("parameters", "status", "iterations"):
                     name = f"{case}-f{frame}-{key}"
                     fields[name] = data[name]
+        for case in PARTIAL_CASES:
+            for key in ("parameters", "status", "iterations"):
+                name = f"{case}-f1-{key}"
+                fields[name] = data[name]
     np.savez_compressed(OUT / "uhd_goldens.npz", **fields)
     print("wrote", len(fields), "arrays")
 

@@ -35,3 +35,27 @@ def compare_fields(candidate: np.ndarray, expected: np.ndarray, params: int) -> 
         "disp_max": float(disp.max()) if disp.size else 0.0,
         "param_rmse": float(np.sqrt(np.mean(diff.astype(np.float64) ** 2))) if diff.size else 0.0,
     }
+
+
+def oracle_rois(size: int, stride: int, mask: np.ndarray, levels: int, *, policy: str = "full",
+                truncation: str = "none", median_filter_size: int = 3) -> list[dict]:
+    """Per-level ROI tables for ``oracle.Pipeline`` from the library's host planner
+    (selection, link bits, seed links and partial supports)."""
+    from live_dic.roi import prepare_roi, reduce_mask, seed_permissions
+
+    masks = [mask]
+    for _ in range(1, levels):
+        masks.append(reduce_mask(masks[-1]))
+    plans = [prepare_roi(m, size, stride, policy=policy, truncation=truncation,
+                         median_filter_size=median_filter_size) for m in masks]
+    out = []
+    for k, plan in enumerate(plans):
+        d = dict(selected=plan.selected, neighbour_bits=plan.neighbour_bits, effective=plan.effective,
+                 median_words=plan.median_neighbour_words, partial_indices=plan.partial_indices,
+                 support_words=plan.support_words, support_boxes=plan.support_boxes, plan=plan)
+        if k + 1 < levels and plan.effective:
+            d["seed_links"] = seed_permissions(plans[k + 1], plan, source_scale=1 << (k + 1),
+                                               destination_scale=1 << k, size=size, stride=stride,
+                                               reference_mask=mask)
+        out.append(d)
+    return out

@@ -226,8 +226,13 @@ def _uhd_mask():
 
 
 @pytest.mark.parametrize("key,model,levels", [("affine-l1-full-zero", "affine", 1),
-                                                ("translation-l2-full-zero", "translation", 2)])
+                                                ("translation-l2-full-zero", "translation", 2),
+                                                ("translation-l1-partial-zero", "translation", 1),
+                                                ("affine-l2-partial-zero", "affine", 2)])
 def test_uhd_roi_goldens(key, model, levels):
+    """The partial cases of this capture have no partial subsets: at 16/16 every window cut by the
+    mask's edges has its even centre on an edge, so they pin that the partial policy without
+    admissible partial windows gives the full-policy field."""
     from live_dic import ICGNWGPUFullPyramid
 
     ref = load_gray("uhd_reference.png")
@@ -235,7 +240,7 @@ def test_uhd_roi_goldens(key, model, levels):
     goldens = np.load(oracle_golden_path(), allow_pickle=False)
     _say("uhd roi golden", key)
     with ICGNWGPUFullPyramid(pyramid_levels=levels, motion_model=model, outlier_policy="median") as eng:
-        eng.set_reference(ref, roi_mask=_uhd_mask())
+        eng.set_reference(ref, roi_mask=_uhd_mask(), roi_policy=key.split("-")[2])
         res = eng.register(tgt)
     raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
     golden = np.concatenate([goldens[f"{key}-f1-parameters"], goldens[f"{key}-f1-status"][..., None],
@@ -245,7 +250,9 @@ def test_uhd_roi_goldens(key, model, levels):
     assert np.array_equal(res.status == -3, golden[..., 6] == -3)
     assert stats["status_equal"] >= 0.999
     if model == "translation":
-        assert stats["disp_within_1e3"] >= 0.99
+        # One level: the capture and the oracle differ by up to 0.018 px on 3 % of subsets (the CPU
+        # oracle test of the same case meets 0.95); two levels agree to 1e-3 on >= 99 %.
+        assert stats["disp_within_1e3"] >= (0.99 if levels == 2 else 0.95)
 
 
 def test_consecutive_pair_mode_matches_set_reference_then_register():
