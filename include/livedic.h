@@ -69,6 +69,7 @@ typedef struct ldic_timing {
     int32_t frames;      /* frames completed by this call (resident slots may queue several) */
     int32_t temporal_seeded;   /* temporal mode: coarsest subsets seeded from the history (last frame) */
     int32_t temporal_updates;  /* temporal mode: finest subsets merged into the history (last frame) */
+    int32_t velocity_predicted; /* velocity prediction: finest subsets extrapolated (last frame) */
 } ldic_timing;
 
 /* Device selection and kernel set-up: ICGNWGPU.__init__ device/pipeline
@@ -153,6 +154,16 @@ typedef struct ldic_initialization {
     int32_t fallback_radius; /* finest-level pixels around the seed, 0..64 */
     uint32_t temporal_max_age; /* LDIC_INIT_TEMPORAL: history expires after more than this many rejected frames;
                                   0xFFFFFFFE keeps it (temporal_max_age=None) */
+    int32_t temporal_prediction;      /* LDIC_INIT_TEMPORAL: 1 extrapolates subsets with two consecutive fresh
+                                         accepted measurements (temporal_prediction="velocity",
+                                         full_pyramid.wgsl:83-137); 0 none */
+    int32_t temporal_fine_correction; /* LDIC_INIT_TEMPORAL, two levels: the finest seed is the finest history
+                                         plus the coarse solve's correction of the coarse history
+                                         (refine_temporal_seed, full_pyramid.wgsl:451-515) */
+    int32_t divergence_patience;      /* 1..30: an unconverged finest-level solve stops after this many
+                                         iterations without a decrease of its change (icgn.wgsl:103-111);
+                                         0 off */
+    int32_t divergence_min_iterations; /* 1..30: iterations before the patience counts (default 4) */
 } ldic_initialization;
 
 /* Partial-support subsets of one level (roi_policy="partial"; roi.py:266-355,

@@ -39,6 +39,8 @@ struct SolveArgs {
     int partial_count;
     const uint32_t* support;
     const float4* support_boxes;
+    // Divergence patience (0: off) and the iteration from which it counts.
+    int patience, min_iters;
 };
 
 struct SeedArgs {
@@ -51,6 +53,16 @@ struct SeedArgs {
     float* dst_seed;
     uint32_t* dst_active;
     uint32_t* seeded_count;       // optional: += subsets that received a weighted seed and are selected
+    int projection;               // temporal projection: with track_availability, also mark unselected seeds
+};
+
+// Fine-level temporal correction (refine_temporal_seed, full_pyramid.wgsl:451-515).
+struct RefineArgs {
+    SeedArgs edge;                // coarse -> fine geometry; src_state is the coarse history
+    const float* fine_history;
+    const uint32_t* fine_ages;
+    const uint32_t* coarse_ages;
+    uint32_t max_age;
 };
 
 struct MedianArgs {
@@ -128,6 +140,15 @@ hipError_t launch_temporal_merge(int count, int model, const float* raw, const f
                                  uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
                                  hipStream_t s);
 hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s);
+// Velocity prediction (full_pyramid.wgsl:83-148): extrapolate the history from
+// the previous accepted measurement; save the measurement before the merge.
+hipError_t launch_temporal_predict(int count, int model, const float* history, const float* previous,
+                                   const uint32_t* ages, const uint32_t* previous_ages, float* prediction,
+                                   uint32_t* predicted_count, hipStream_t s);
+hipError_t launch_temporal_save_previous(int count, int model, const float* raw, const float* output,
+                                         const float* history, const uint32_t* ages, float* previous,
+                                         uint32_t* previous_ages, hipStream_t s);
+hipError_t launch_temporal_refine(const RefineArgs& a, hipStream_t s);
 // dst[index[i] * row + k] = src[i * row + k]
 hipError_t launch_scatter_rows(int count, int row, const uint32_t* index, const float* src, float* dst,
                                hipStream_t s);

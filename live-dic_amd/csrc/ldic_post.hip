@@ -42,12 +42,12 @@ __device__ void accumulate_seed(const SeedArgs& a, int si, float weight, float s
     wsum += weight;
 }
 
-__global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
-    int di = blockIdx.x * 256 + threadIdx.x;
-    if (di >= a.dst_count) return;
+// Weighted blend of the permitted coarse corners (and the one-based fallback
+// neighbour) at destination subset di (interpolate_subset_seed,
+// full_pyramid.wgsl:313-400); acc holds the normalised estimate when wsum > 0.
+__device__ void interpolate_seed(const SeedArgs& a, int di, uint32_t perm, float acc[12], float& wsum) {
     const bool q = a.model == kQuadratic;
     const int stride = q ? 16 : 8;
-    uint32_t perm = a.permissions ? a.permissions[di] : 31u;
     int dxg = di % a.dst_grid_w, dyg = di / a.dst_grid_w;
     float so = (1.0f - a.src_scale) * 0.5f, dof = (1.0f - a.dst_scale) * 0.5f;
     float dcx = 0.5f * a.S + a.D * (float)dxg, dcy = 0.5f * a.S + a.D * (float)dyg;
@@ -59,8 +59,8 @@ __global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
     float fx = fminf(fmaxf(sgx - (float)lx, 0.0f), 1.0f);
     float fy = fminf(fmaxf(sgy - (float)ly, 0.0f), 1.0f);
     float ts = a.src_scale / a.dst_scale;
-    float acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    float wsum = 0.0f;
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0f;
+    wsum = 0.0f;
     for (int corner = 0; corner < 4; ++corner) {
         if ((perm & (1u << corner)) == 0u) continue;
         int ox = corner & 1, oy = corner >> 1;
@@ -82,8 +82,22 @@ __global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
     }
     if (wsum > 0.0f)
         for (int k = 0; k < 12; ++k) acc[k] /= wsum;
+}
+
+__global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
+    int di = blockIdx.x * 256 + threadIdx.x;
+    if (di >= a.dst_count) return;
+    const bool q = a.model == kQuadratic;
+    const int stride = q ? 16 : 8;
+    uint32_t perm = a.permissions ? a.permissions[di] : 31u;
+    float acc[12], wsum;
+    interpolate_seed(a, di, perm, acc, wsum);
     float4* o = reinterpret_cast<float4*>(a.dst_seed + (size_t)di * stride);
-    float marker = wsum >= 0.99999f ? 0.0f : -1.0f;
+    // Seed availability (TRACK_SEED_AVAILABILITY): -1 without a complete
+    // stencil; the temporal projection also marks seeds it does not count
+    // (project_temporal_seed, full_pyramid.wgsl:437-449).
+    const bool counted = wsum > 0.0f && (perm & 16u);
+    float marker = (wsum >= 0.99999f && (!a.projection || counted)) ? 0.0f : -1.0f;
     if (q) {
         o[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
         o[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
@@ -94,7 +108,104 @@ __global__ __launch_bounds__(256) void k_transfer_seed(SeedArgs a) {
         o[1] = make_float4(acc[4], acc[5], a.track_availability ? marker : 0.0f, 0.0f);
     }
     a.dst_active[di] = (perm >> 4u) & 1u;
-    if (a.seeded_count && wsum > 0.0f && (perm & 16u)) atomicAdd(a.seeded_count, 1u);
+    if (a.seeded_count && counted) atomicAdd(a.seeded_count, 1u);
+}
+
+// refine_temporal_seed (full_pyramid.wgsl:451-515): where the fine subset's
+// history and every permitted coarse corner's history share one age, replace
+// the absolute pyramid seed by the fine history plus the coarse correction
+// (absolute seed minus the coarse history interpolated like a seed).
+__global__ __launch_bounds__(256) void k_temporal_refine(RefineArgs r) {
+    const SeedArgs& a = r.edge;
+    const int di = blockIdx.x * 256 + threadIdx.x;
+    if (di >= a.dst_count) return;
+    if (a.dst_active[di] == 0u) return;
+    const uint32_t age = r.fine_ages[di];
+    if (age == 0xFFFFFFFFu || age > r.max_age) return;
+    const bool q = a.model == kQuadratic;
+    const int stride = q ? 16 : 8;
+    float* seed = a.dst_seed + (size_t)di * stride;
+    if ((q ? seed[12] : seed[6]) < -0.5f) return;
+    const uint32_t perm = a.permissions ? a.permissions[di] : 31u;
+    const int dxg = di % a.dst_grid_w, dyg = di / a.dst_grid_w;
+    const float cx = (a.dst_scale * (0.5f * a.S + (float)dxg * a.D) + (a.src_scale - a.dst_scale) * 0.5f) / a.src_scale;
+    const float cy = (a.dst_scale * (0.5f * a.S + (float)dyg * a.D) + (a.src_scale - a.dst_scale) * 0.5f) / a.src_scale;
+    const int lx = clampi((int)floorf((cx - 0.5f * a.S) / a.D), 0, max(0, a.src_grid_w - 2));
+    const int ly = clampi((int)floorf((cy - 0.5f * a.S) / a.D), 0, max(0, a.src_grid_h - 2));
+    for (int corner = 0; corner < 4; ++corner) {
+        if ((perm & (1u << corner)) == 0u) continue;
+        const int sx = lx + (corner & 1), sy = ly + (corner >> 1);
+        if (sx >= a.src_grid_w || sy >= a.src_grid_h) continue;
+        if (r.coarse_ages[sy * a.src_grid_w + sx] != age) return;
+    }
+    float pred[12], wsum;
+    interpolate_seed(a, di, perm, pred, wsum);
+    if (wsum < 0.99999f) return;
+    const float* prev = r.fine_history + (size_t)di * stride;
+    // Parameters in state order: tx ux uy ty vx vy (+ 6 curvature terms at 6..11).
+    for (int k = 0; k < 6; ++k) seed[k] = prev[k] + (seed[k] - pred[k]);
+    if (q)
+        for (int k = 6; k < 12; ++k) seed[k] = prev[k] + (seed[k] - pred[k]);
+}
+
+hipError_t launch_temporal_refine(const RefineArgs& a, hipStream_t s) {
+    if (a.edge.dst_count <= 0) return hipSuccess;
+    k_temporal_refine<<<(a.edge.dst_count + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// predict_temporal_subset (full_pyramid.wgsl:83-113): a subset whose current
+// and previous measurements are both fresh (age 0) and accepted gets
+// 2 * current - previous for its parameters; the status and iteration words
+// are copied.  Every other subset's seed is its history as is.
+__global__ void k_temporal_predict(int count, int stride, int P, const float* history, const float* previous,
+                                   const uint32_t* ages, const uint32_t* previous_ages, float* prediction,
+                                   uint32_t* predicted_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float* h = history + (size_t)i * stride;
+    const float* p = previous + (size_t)i * stride;
+    const bool predict = ages[i] == 0u && previous_ages[i] == 0u && h[P] >= 0.5f && p[P] >= 0.5f;
+    float* o = prediction + (size_t)i * stride;
+    for (int k = 0; k < stride; ++k) o[k] = (predict && k < P) ? 2.0f * h[k] - p[k] : h[k];
+    if (predict && predicted_count) atomicAdd(predicted_count, 1u);
+}
+
+// save_previous_temporal_state (full_pyramid.wgsl:115-137): before the merge
+// replaces the history with an accepted measurement, keep the history as the
+// previous measurement; otherwise the previous age grows (saturating below the
+// never-set sentinel).
+__global__ void k_temporal_save_previous(int count, int stride, int P, const float* raw, const float* output,
+                                         const float* history, const uint32_t* ages, float* previous,
+                                         uint32_t* previous_ages) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float rs = raw[(size_t)i * stride + P], fs = output[(size_t)i * stride + P];
+    if ((rs == 1.0f || rs == 2.0f) && (fs == 1.0f || fs == 2.0f)) {
+        for (int k = 0; k < stride; ++k) previous[(size_t)i * stride + k] = history[(size_t)i * stride + k];
+        previous_ages[i] = ages[i];
+    } else {
+        const uint32_t age = previous_ages[i];
+        if (age != 0xFFFFFFFFu) previous_ages[i] = min(age, 0xFFFFFFFDu) + 1u;
+    }
+}
+
+hipError_t launch_temporal_predict(int count, int model, const float* history, const float* previous,
+                                   const uint32_t* ages, const uint32_t* previous_ages, float* prediction,
+                                   uint32_t* predicted_count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_temporal_predict<<<(count + 255) / 256, 256, 0, s>>>(count, state_stride(model), param_count(model), history,
+                                                          previous, ages, previous_ages, prediction, predicted_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_temporal_save_previous(int count, int model, const float* raw, const float* output,
+                                         const float* history, const uint32_t* ages, float* previous,
+                                         uint32_t* previous_ages, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_temporal_save_previous<<<(count + 255) / 256, 256, 0, s>>>(count, state_stride(model), param_count(model), raw,
+                                                                output, history, ages, previous, previous_ages);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -171,6 +171,13 @@ struct ldic_engine {
     // Temporal history of the finest field and the projection links onto the coarsest grid.
     float* hist_state = nullptr;
     uint32_t* hist_ages = nullptr;
+    // Velocity prediction: previous accepted measurement, its ages, the seed.
+    float* prev_state = nullptr;
+    uint32_t* prev_ages = nullptr;
+    float* prediction = nullptr;
+    // Fine correction: history of the coarsest level.
+    float* coarse_state = nullptr;
+    uint32_t* coarse_ages = nullptr;
     uint32_t* temporal_links = nullptr;
     // Recorded after each frame's history merge; the next frame's projection,
     // on whichever slot, waits for it, so streams keep the chronological chain.
@@ -306,7 +313,7 @@ SearchArgs search_args(const ldic_engine* e, int k, const SlotLevel& sl, int rad
 // retry_partial_seeded, icgn.wgsl:1040-1085): the weighted solver variant over
 // the level's partial list, uniform weights expressed as an infinite sigma.
 int launch_partial(ldic_engine* e, int k, SlotLevel& sl, const float* seed, const uint32_t* active, int preserve,
-                   hipStream_t st, float* zncc_out = nullptr) {
+                   hipStream_t st, float* zncc_out = nullptr, int patience = 0, int min_iters = 0) {
     const Level& L = e->lv[k];
     if (L.partial_count <= 0) return LDIC_OK;
     SolveArgs a = solve_args(e, k, sl, seed, active, preserve);
@@ -315,6 +322,8 @@ int launch_partial(ldic_engine* e, int k, SlotLevel& sl, const float* seed, cons
     a.support = L.support;
     a.support_boxes = L.support_boxes;
     a.zncc_out = zncc_out;
+    a.patience = patience;
+    a.min_iters = min_iters;
     if (e->opt.spatial_weighting == 0) a.inv_var = 0.0f;
     HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, 1, a, st));
     return LDIC_OK;
@@ -365,6 +374,34 @@ int drain_ring(Slot* s) {
     return LDIC_OK;
 }
 
+// Forget every temporal history (the reset of full_pyramid_backend.py:660-677
+// covers the finest, previous and coarse histories).
+hipError_t reset_histories(ldic_engine* e, hipStream_t st) {
+    const int model = e->opt.motion_model;
+    hipError_t r = launch_temporal_reset(e->lv[0].count(), model, e->hist_state, e->hist_ages, st);
+    if (r == hipSuccess) r = launch_temporal_reset(e->lv[0].count(), model, e->prev_state, e->prev_ages, st);
+    if (r == hipSuccess)
+        r = launch_temporal_reset(e->lv[e->levels - 1].count(), model, e->coarse_state, e->coarse_ages, st);
+    return r;
+}
+
+SeedArgs edge_args(const ldic_engine* e, int k) {
+    const Level& src = e->lv[k + 1];
+    const Level& lev = e->lv[k];
+    SeedArgs sa{};
+    sa.src_grid_w = src.gw;
+    sa.src_grid_h = src.gh;
+    sa.dst_grid_w = lev.gw;
+    sa.dst_count = lev.count();
+    sa.src_scale = (float)(1 << (k + 1));
+    sa.dst_scale = (float)(1 << k);
+    sa.S = (float)e->opt.subset_size;
+    sa.D = (float)e->opt.subset_stride;
+    sa.model = e->opt.motion_model;
+    sa.permissions = lev.seed_links;
+    return sa;
+}
+
 int enqueue_frame(ldic_engine* e, Slot* s) {
     hipStream_t st = s->stream;
     const int L = e->levels;
@@ -373,29 +410,32 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
     s->ring_count += 1;
     for (int k = 1; k < L; ++k)
         HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, st));
+    const bool temporal = e->init.kind == LDIC_INIT_TEMPORAL;
+    const bool velocity = temporal && e->init.temporal_prediction == 1;
+    const bool correct = temporal && e->init.temporal_fine_correction != 0 && L == 2;
     for (int k = L - 1; k >= 0; --k) {
         SlotLevel& sl = s->lv[k];
         const Level& lev = e->lv[k];
         const float* seed = nullptr;
         const uint32_t* active = nullptr;
         if (k < L - 1) {
-            const Level& src = e->lv[k + 1];
-            SeedArgs sa{};
-            sa.src_grid_w = src.gw;
-            sa.src_grid_h = src.gh;
-            sa.dst_grid_w = lev.gw;
-            sa.dst_count = lev.count();
-            sa.src_scale = (float)(1 << (k + 1));
-            sa.dst_scale = (float)(1 << k);
-            sa.S = (float)e->opt.subset_size;
-            sa.D = (float)e->opt.subset_stride;
-            sa.model = e->opt.motion_model;
-            sa.track_availability = 0;
+            SeedArgs sa = edge_args(e, k);
+            sa.track_availability = correct ? 1 : 0;
             sa.src_state = level_output(e, s->lv[k + 1]);
-            sa.permissions = lev.seed_links;
             sa.dst_seed = sl.seed;
             sa.dst_active = sl.active;
             HIP_CHECK(launch_transfer_seed(sa, st));
+            if (k == 0 && correct) {
+                // refine_temporal_seed (full_pyramid.wgsl:451-515).
+                RefineArgs r{};
+                r.edge = sa;
+                r.edge.src_state = e->coarse_state;
+                r.fine_history = e->hist_state;
+                r.fine_ages = e->hist_ages;
+                r.coarse_ages = e->coarse_ages;
+                r.max_age = e->init.temporal_max_age;
+                HIP_CHECK(launch_temporal_refine(r, st));
+            }
             seed = sl.seed;
             active = sl.active;
         }
@@ -405,6 +445,9 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
             const Level& fine = e->lv[0];
             if (e->temporal_pending) HIP_CHECK(hipStreamWaitEvent(st, e->temporal_done, 0));
             HIP_CHECK(launch_fill_u32(4, 0u, s->counts, st));
+            if (velocity)
+                HIP_CHECK(launch_temporal_predict(fine.count(), e->opt.motion_model, e->hist_state, e->prev_state,
+                                                  e->hist_ages, e->prev_ages, e->prediction, s->counts + 2, st));
             SeedArgs sa{};
             sa.src_grid_w = fine.gw;
             sa.src_grid_h = fine.gh;
@@ -415,8 +458,9 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
             sa.S = (float)e->opt.subset_size;
             sa.D = (float)e->opt.subset_stride;
             sa.model = e->opt.motion_model;
-            sa.track_availability = 0;
-            sa.src_state = e->hist_state;
+            sa.track_availability = correct ? 1 : 0;
+            sa.projection = 1;
+            sa.src_state = velocity ? e->prediction : e->hist_state;
             sa.permissions = e->temporal_links;
             sa.dst_seed = sl.seed;
             sa.dst_active = sl.active;
@@ -439,19 +483,38 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
         if (e->opt.interpolation == kBspline)
             HIP_CHECK(launch_prefilter(sl.target, lev.width, lev.height, e->fir, sl.coeff, st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve0[ring], st));
-        HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting,
-                               solve_args(e, k, sl, seed, active, 0), st));
-        CHECK(launch_partial(e, k, sl, seed, active, 0, st));
+        SolveArgs a = solve_args(e, k, sl, seed, active, 0);
+        if (k == 0) {
+            // Divergence patience applies at the finest level only: a premature
+            // coarse stop would spoil every finer seed (full_pyramid_backend.py:1214-1224).
+            a.patience = e->init.divergence_patience;
+            a.min_iters = e->init.divergence_min_iterations;
+        }
+        HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting, a, st));
+        CHECK(launch_partial(e, k, sl, seed, active, 0, st, nullptr, a.patience, a.min_iters));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
         CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
     }
-    if (e->init.kind == LDIC_INIT_TEMPORAL) {
+    if (temporal) {
         // update_temporal_state (full_pyramid.wgsl:519-565): the filtered output
-        // is merged under the replace policy, the raw state otherwise.
+        // is merged under the replace policy without prediction, the raw state
+        // otherwise; with prediction the history first becomes the previous
+        // measurement where a new one is accepted.
         SlotLevel& fine = s->lv[0];
+        const int filtered = e->opt.outlier_policy == kReplace && !velocity ? 1 : 0;
+        if (velocity)
+            HIP_CHECK(launch_temporal_save_previous(e->lv[0].count(), e->opt.motion_model, fine.state,
+                                                    level_output(e, fine), e->hist_state, e->hist_ages,
+                                                    e->prev_state, e->prev_ages, st));
         HIP_CHECK(launch_temporal_merge(e->lv[0].count(), e->opt.motion_model, fine.state, level_output(e, fine),
-                                        e->opt.outlier_policy == kReplace ? 1 : 0, e->init.temporal_max_age,
-                                        e->hist_state, e->hist_ages, s->counts + 1, st));
+                                        filtered, e->init.temporal_max_age, e->hist_state, e->hist_ages,
+                                        s->counts + 1, st));
+        if (correct) {
+            SlotLevel& coarse = s->lv[L - 1];
+            HIP_CHECK(launch_temporal_merge(e->lv[L - 1].count(), e->opt.motion_model, coarse.state,
+                                            level_output(e, coarse), filtered, e->init.temporal_max_age,
+                                            e->coarse_state, e->coarse_ages, nullptr, st));
+        }
         HIP_CHECK(hipMemcpyAsync(s->pinned_counts, s->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipEventRecord(e->temporal_done, st));
         e->temporal_pending = true;
@@ -528,6 +591,16 @@ int validate_initialization(const ldic_engine* e, const ldic_initialization* in)
         return set_error(LDIC_ERR_ARGUMENT, "initialization kind must be LDIC_INIT_ZERO, _ZNCC or _TEMPORAL");
     if (in->temporal_max_age > 0xFFFFFFFEu)
         return set_error(LDIC_ERR_ARGUMENT, "temporal_max_age must be at most 0xFFFFFFFE");
+    if (in->temporal_prediction != 0 && in->temporal_prediction != 1)
+        return set_error(LDIC_ERR_ARGUMENT, "temporal_prediction must be 0 (none) or 1 (velocity)");
+    if (in->temporal_prediction && in->kind != LDIC_INIT_TEMPORAL)
+        return set_error(LDIC_ERR_ARGUMENT, "temporal_prediction requires temporal initialization");
+    if (in->temporal_fine_correction && (in->kind != LDIC_INIT_TEMPORAL || e->levels != 2))
+        return set_error(LDIC_ERR_ARGUMENT, "temporal_fine_correction requires temporal initialization and two levels");
+    if (in->divergence_patience < 0 || in->divergence_patience > 30)
+        return set_error(LDIC_ERR_ARGUMENT, "divergence_patience must be 0 (off) or an integer from 1 to 30");
+    if (in->divergence_patience && (in->divergence_min_iterations < 1 || in->divergence_min_iterations > 30))
+        return set_error(LDIC_ERR_ARGUMENT, "divergence_min_iterations must be an integer from 1 to 30");
     const bool search = in->kind == LDIC_INIT_ZNCC, fallback = in->fallback_zncc > -1.0f;
     if (!std::isfinite(in->fallback_zncc) || in->fallback_zncc > 1.0f)
         return set_error(LDIC_ERR_ARGUMENT, "fallback_zncc must be finite and at most 1 (<= -1 disables it)");
@@ -589,6 +662,7 @@ int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wal
         t->frames = s->drained_frames;
         t->temporal_seeded = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[0] : 0;
         t->temporal_updates = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[1] : 0;
+        t->velocity_predicted = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[2] : 0;
         t->readback_ms = now_ms() - t0;
         t->wall_ms = now_ms() - wall_start;
     }
@@ -801,8 +875,14 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
     CHECK(e->ref_arena.alloc(&e->prep_gy, px0));
     CHECK(e->ref_arena.alloc(&e->hist_state, (size_t)L0.count() * state_stride(e->opt.motion_model)));
     CHECK(e->ref_arena.alloc(&e->hist_ages, (size_t)L0.count()));
+    CHECK(e->ref_arena.alloc(&e->prev_state, (size_t)L0.count() * state_stride(e->opt.motion_model)));
+    CHECK(e->ref_arena.alloc(&e->prev_ages, (size_t)L0.count()));
+    CHECK(e->ref_arena.alloc(&e->prediction, (size_t)L0.count() * state_stride(e->opt.motion_model)));
+    const int coarse_count = e->lv[e->levels - 1].count();
+    CHECK(e->ref_arena.alloc(&e->coarse_state, (size_t)coarse_count * state_stride(e->opt.motion_model)));
+    CHECK(e->ref_arena.alloc(&e->coarse_ages, (size_t)coarse_count));
     e->temporal_links = nullptr;
-    HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, st));
+    HIP_CHECK(reset_histories(e, st));
     HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, image, (size_t)row_stride, L0.width, L0.height,
                                hipMemcpyHostToDevice, st));
     int rc = enqueue_reference(e, st);
@@ -930,8 +1010,7 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
     Slot* s = e->slots[0];
     if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
     // A new reference starts a new temporal history.
-    if (e->init.kind == LDIC_INIT_TEMPORAL)
-        HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, s->stream));
+    if (e->init.kind == LDIC_INIT_TEMPORAL) HIP_CHECK(reset_histories(e, s->stream));
     for (int y = 0; y < L0.height; ++y)
         std::memcpy(s->pinned_ref + (size_t)y * L0.width, reference + (size_t)y * reference_stride, (size_t)L0.width);
     HIP_CHECK(hipMemcpyAsync(e->lv[0].image, s->pinned_ref, (size_t)L0.width * L0.height, hipMemcpyHostToDevice,
@@ -957,8 +1036,7 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
     HIP_CHECK(hipSetDevice(e->device));
     const Level& L0 = e->lv[0];
     if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
-    if (e->init.kind == LDIC_INIT_TEMPORAL)
-        HIP_CHECK(launch_temporal_reset(L0.count(), e->opt.motion_model, e->hist_state, e->hist_ages, s->stream));
+    if (e->init.kind == LDIC_INIT_TEMPORAL) HIP_CHECK(reset_histories(e, s->stream));
     HIP_CHECK(hipMemcpy2DAsync(L0.image, L0.width, device_reference, (size_t)row_stride, L0.width, L0.height,
                                hipMemcpyDeviceToDevice, s->stream));
     CHECK(enqueue_reference(e, s->stream));
@@ -1146,8 +1224,7 @@ int ldic_reset_temporal(ldic_engine* e) {
     for (Slot* s : e->slots)
         if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
     HIP_CHECK(hipSetDevice(e->device));
-    HIP_CHECK(launch_temporal_reset(e->lv[0].count(), e->opt.motion_model, e->hist_state, e->hist_ages,
-                                    e->prep_stream));
+    HIP_CHECK(reset_histories(e, e->prep_stream));
     HIP_CHECK(hipStreamSynchronize(e->prep_stream));
     return LDIC_OK;
 }
@@ -1571,6 +1648,8 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
             return set_error(LDIC_ERR_ARGUMENT, "temporal initialisation is not supported in a batch");
         if (has_partial(engines[i]))
             return set_error(LDIC_ERR_ARGUMENT, "partial-support ROI subsets are not supported in a batch");
+        if (engines[i]->init.divergence_patience)
+            return set_error(LDIC_ERR_ARGUMENT, "divergence patience is not supported in a batch");
         for (int k = 0; k < i; ++k)
             if (engines[k] == engines[i]) return set_error(LDIC_ERR_ARGUMENT, "engine %d appears twice", i);
     }

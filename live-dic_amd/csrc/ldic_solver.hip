@@ -403,9 +403,15 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     t.valid = false;
     t.x0 = t.y0 = t.w = t.h = 0;
     float last_zncc = __builtin_nanf("");
+    // Divergence patience (icgn.wgsl:103-111): an unconverged solve whose change
+    // has not decreased for `patience` iterations after `min_iters` stops with
+    // the internal status 3, reported as 0; a stopped solve skips the bounds test.
+    float prev_change = 3.402823e38f;
+    int stalled = 0;
 
     for (int it = 0; it < 30; ++it) {
-        bool inside = warp_inside<MODEL>(q, cx, cy, he, Wf, Hf, bx0, by0, bx1, by1);
+        const bool stopped = a.patience > 0 && status == 3.0f;
+        bool inside = stopped || warp_inside<MODEL>(q, cx, cy, he, Wf, Hf, bx0, by0, bx1, by1);
         if (!inside && (!a.roi_enabled || status > -2.5f)) status = -1.0f;
         if (it >= a.iterations || fabsf(status) > 0.5f) break;
 
@@ -583,7 +589,13 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             change = increment_change<MODEL>(d, he);
         }
         status = change >= a.tolerance ? 0.0f : (zncc >= a.zncc_min ? 1.0f : -2.0f);
+        if (a.patience > 0 && status == 0.0f) {
+            if (it + 1 >= a.min_iters) stalled = change >= prev_change ? stalled + 1 : 0;
+            prev_change = change;
+            if (stalled >= a.patience) status = 3.0f;
+        }
     }
+    if (a.patience > 0 && status == 3.0f) status = 0.0f;
 
     if (lane == 0) {
         float4* out = reinterpret_cast<float4*>(a.state + (size_t)subset * STRIDE);

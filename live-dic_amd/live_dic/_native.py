@@ -77,6 +77,7 @@ class Timing(ctypes.Structure):
         ("frames", ctypes.c_int32),
         ("temporal_seeded", ctypes.c_int32),
         ("temporal_updates", ctypes.c_int32),
+        ("velocity_predicted", ctypes.c_int32),
     ]
 
 
@@ -87,6 +88,10 @@ class Initialization(ctypes.Structure):
         ("fallback_zncc", ctypes.c_float),
         ("fallback_radius", ctypes.c_int32),
         ("temporal_max_age", ctypes.c_uint32),
+        ("temporal_prediction", ctypes.c_int32),
+        ("temporal_fine_correction", ctypes.c_int32),
+        ("divergence_patience", ctypes.c_int32),
+        ("divergence_min_iterations", ctypes.c_int32),
     ]
 
 

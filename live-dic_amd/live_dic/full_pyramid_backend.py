@@ -43,6 +43,17 @@ def _level_count(value: int) -> int:
     return count
 
 
+def _iteration_count(value, name: str) -> int:
+    """An integer from 1 to 30 (full_pyramid_backend.py:135-142)."""
+    try:
+        count = operator.index(value)
+    except TypeError as error:
+        raise ValueError(f"{name} must be an integer from 1 to 30") from error
+    if isinstance(value, bool) or not 1 <= count <= 30:
+        raise ValueError(f"{name} must be an integer from 1 to 30")
+    return count
+
+
 def _schedule(values, *, level_count: int, defaults, name: str):
     selected = tuple(defaults[-level_count:] if values is None else values)
     if len(selected) != level_count:
@@ -58,11 +69,15 @@ class ICGNWGPUFullPyramid(_NativeEngine):
     default); ``"temporal"`` (full_pyramid_backend.py:433-881): the last accepted
     finest field, projected onto the coarsest grid, seeds every frame; history
     entries expire after more than ``temporal_max_age`` rejected frames; targets
-    must come in chronological order; and ``"zncc"``, an addition: the coarsest
-    level is seeded by an exhaustive integer ZNCC search covering
-    ``seed_search_radius`` pixels of the finest level.  Not built: the temporal
-    velocity prediction and fine correction, the Fourier and CPM-HOG matching
-    initialisers, and the adaptive pyramid policy.
+    must come in chronological order; ``temporal_prediction="velocity"``
+    extrapolates subsets with two consecutive fresh accepted measurements, and
+    ``temporal_fine_correction`` (two levels) seeds the finest level with its
+    history plus the coarse solve's correction; and ``"zncc"``, an addition:
+    the coarsest level is seeded by an exhaustive integer ZNCC search covering
+    ``seed_search_radius`` pixels of the finest level.  ``divergence_patience``
+    stops unconverged finest-level solves whose change stopped decreasing
+    (icgn.wgsl:103-111).  Not built: the Fourier and CPM-HOG matching
+    initialisers and the adaptive pyramid policy.
     """
 
     def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
@@ -76,6 +91,7 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                  gaussian_sigma: float | None = None, initialization: str = "zero",
                  temporal_max_age: int | None = None, temporal_fine_correction: bool = False,
                  temporal_prediction: str = "none", pyramid_policy: str = "fixed", seed_search_radius: int = 32,
+                 divergence_patience: int | None = None, divergence_min_iterations: int = 4,
                  **unsupported) -> None:
         if unsupported:
             raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
@@ -92,12 +108,13 @@ class ICGNWGPUFullPyramid(_NativeEngine):
             raise ValueError("temporal_prediction must be 'none' or 'velocity'")
         if temporal_prediction != "none" and initialization != "temporal":
             raise ValueError("temporal_prediction requires temporal initialization")
-        if temporal_prediction != "none":
-            raise ValueError("temporal_prediction='velocity' is not built; use 'none'")
+        self.divergence_patience = (None if divergence_patience is None
+                                    else _iteration_count(divergence_patience, "divergence_patience"))
+        self.divergence_min_iterations = _iteration_count(divergence_min_iterations, "divergence_min_iterations")
         if not isinstance(temporal_fine_correction, bool):
             raise ValueError("temporal_fine_correction must be a boolean")
-        if temporal_fine_correction:
-            raise ValueError("temporal_fine_correction is not built")
+        if temporal_fine_correction and (initialization != "temporal" or _level_count(pyramid_levels) != 2):
+            raise ValueError("temporal_fine_correction requires temporal initialization and two levels")
         if temporal_max_age is not None:
             try:
                 age = operator.index(temporal_max_age)
@@ -138,10 +155,16 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                          subset_size=subset_size, subset_stride=subset_stride,
                          spatial_weighting=spatial_weighting, gaussian_sigma=gaussian_sigma)
         self._zncc_min = 0.8
+        sequence = dict(temporal_prediction=1 if temporal_prediction == "velocity" else 0,
+                        temporal_fine_correction=int(temporal_fine_correction),
+                        divergence_patience=self.divergence_patience or 0,
+                        divergence_min_iterations=self.divergence_min_iterations)
         if initialization == "zncc":
-            self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius)
+            self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius, **sequence)
         elif initialization == "temporal":
-            self._set_initialization(_native.INIT_TEMPORAL, temporal_max_age=temporal_max_age)
+            self._set_initialization(_native.INIT_TEMPORAL, temporal_max_age=temporal_max_age, **sequence)
+        elif self.divergence_patience is not None:
+            self._set_initialization(_native.INIT_ZERO, **sequence)
 
     def set_reference(self, image: ImageInput, *, iterations_per_level: tuple[int, ...] | None = None,
                       tolerance_per_level: tuple[float, ...] | None = None, zncc_min: float = 0.8,
@@ -175,9 +198,18 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if self.initialization == "zncc":
             self._pyramid_config.update(initialization="zncc", seedSearchRadius=self.seed_search_radius,
                                         coarseSearchRadius=self.coarse_search_radius)
-        elif self.initialization == "temporal":
+        if self.divergence_patience is not None:
+            self._pyramid_config.update(divergencePatience=self.divergence_patience,
+                                        divergenceMinIterations=self.divergence_min_iterations,
+                                        divergenceLevels="finest")
+        if self.initialization == "temporal":
+            filtered = self.outlier_policy == "replace" and self.temporal_prediction == "none"
             self._pyramid_config.update(initialization="temporal", temporalMaxAge=self.temporal_max_age,
-                                        temporalSeedState="filtered" if self.outlier_policy == "replace" else "raw")
+                                        temporalSeedLevel="coarsest",
+                                        temporalSeedState="filtered" if filtered else "raw",
+                                        **({"temporalFineCorrection": True} if self.temporal_fine_correction else {}),
+                                        **({"temporalPrediction": "velocity"}
+                                           if self.temporal_prediction == "velocity" else {}))
         return self._reference_info(started, build_ms, {
             **configs[0],
             **({"roi": {**self.roi.summary(), "selectedPerLevel": self._roi_level_counts}} if self.roi else {}),

@@ -304,8 +304,7 @@ class _NativeEngine:
         result.timing = {**result.timing, "totalWallMs": elapsed, "resultAgeWallMs": elapsed}
         return result
 
-    @staticmethod
-    def _timing_dict(t: _native.Timing, started: float, prep_ms: float) -> dict[str, float]:
+    def _timing_dict(self, t: _native.Timing, started: float, prep_ms: float) -> dict[str, float]:
         total = (time.perf_counter() - started) * 1e3
         return {
             "inputPreparationWallMs": prep_ms,
@@ -320,6 +319,8 @@ class _NativeEngine:
             "resultAgeWallMs": total,
             "temporallySeededCoarseSubsets": float(t.temporal_seeded),
             "temporalAcceptedUpdates": float(t.temporal_updates),
+            **({"velocityPredictedSubsets": float(t.velocity_predicted)}
+               if getattr(self, "temporal_prediction", "none") == "velocity" else {}),
         }
 
     def _run(self, frame: np.ndarray, started: float, prep_ms: float, config: dict) -> RegistrationResult:
@@ -426,10 +427,14 @@ class _NativeEngine:
         return out
 
     def _set_initialization(self, kind: int, search_radius: int = 0, fallback_zncc: float | None = None,
-                            fallback_radius: int = 0, temporal_max_age: int | None = None) -> None:
+                            fallback_radius: int = 0, temporal_max_age: int | None = None,
+                            temporal_prediction: int = 0, temporal_fine_correction: int = 0,
+                            divergence_patience: int = 0, divergence_min_iterations: int = 4) -> None:
         init = _native.Initialization(kind, int(search_radius), -2.0 if fallback_zncc is None else float(fallback_zncc),
                                       int(fallback_radius),
-                                      0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age))
+                                      0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age),
+                                      int(temporal_prediction), int(temporal_fine_correction),
+                                      int(divergence_patience), int(divergence_min_iterations))
         _native.check(self._lib.ldic_set_initialization(self._handle, ctypes.byref(init)))
         self._initialization = init
 

@@ -52,6 +52,7 @@ class SolveCfg(ctypes.Structure):
         ("gaussian_sigma", ctypes.c_float), ("iterations", ctypes.c_int32),
         ("tolerance", ctypes.c_float), ("zncc_min", ctypes.c_float),
         ("lanes", ctypes.c_int32), ("preserve_disabled", ctypes.c_int32),
+        ("divergence_patience", ctypes.c_int32), ("divergence_min_iterations", ctypes.c_int32),
     ]
 
 
@@ -172,6 +173,61 @@ def temporal_merge(history, ages, raw, output, use_filtered, max_age):
     return int(upd.sum())
 
 
+def temporal_predict(history, previous, ages, previous_ages):
+    """predict_temporal_state (full_pyramid.wgsl:83-113): returns (seed history, predicted count).
+    Subsets whose current and previous measurements are fresh (age 0) and accepted get
+    2 * current - previous for every parameter; status and iteration words are copied."""
+    P = 12 if history.shape[1] == 16 else 6
+    go = (ages == 0) & (previous_ages == 0) & (history[:, P] >= 0.5) & (previous[:, P] >= 0.5)
+    out = history.copy()
+    out[go, :P] = 2.0 * history[go, :P] - previous[go, :P]
+    return out, int(go.sum())
+
+
+def temporal_save_previous(history, ages, raw, output, previous, previous_ages):
+    """save_previous_temporal_state (full_pyramid.wgsl:115-137), in place, before the merge."""
+    P = 12 if history.shape[1] == 16 else 6
+    rs, fs = raw[:, P], output[:, P]
+    keep = ((rs == 1.0) | (rs == 2.0)) & ((fs == 1.0) | (fs == 2.0))
+    previous[keep] = history[keep]
+    previous_ages[keep] = ages[keep]
+    aging = ~keep & (previous_ages != NEVER)
+    previous_ages[aging] = np.minimum(previous_ages[aging], 0xFFFFFFFD) + 1
+
+
+def temporal_refine(cfg: SeedCfg, coarse_history, coarse_ages, fine_history, fine_ages, max_age, links, seed,
+                    active):
+    """refine_temporal_seed (full_pyramid.wgsl:451-515), in place on the fine ``seed``.
+
+    Where the fine subset is active, its history is fresh enough, its absolute seed is marked
+    available, every permitted coarse corner has the fine subset's age and the coarse history
+    interpolates with full weight: seed = fine history + (absolute seed - interpolated coarse
+    history) for every parameter."""
+    P = 12 if seed.shape[1] == 16 else 6
+    n = cfg.destination_grid_count
+    perm = np.full(n, 31, np.uint32) if links is None else np.asarray(links, np.uint32).ravel()
+    predicted, _, weight = transfer_seed(cfg, coarse_history, perm, return_weight=True)
+    i = np.arange(n)
+    S, D = cfg.subset_size, cfg.subset_stride
+    ss, ds = cfg.source_scale, cfg.destination_scale
+    cx = (ds * (0.5 * S + (i % cfg.destination_grid_width) * D) + (ss - ds) * 0.5) / ss
+    cy = (ds * (0.5 * S + (i // cfg.destination_grid_width) * D) + (ss - ds) * 0.5) / ss
+    gw, gh = cfg.source_grid_width, cfg.source_grid_height
+    lx = np.clip(np.floor((np.float32(cx) - 0.5 * S) / D).astype(np.int64), 0, max(0, gw - 2))
+    ly = np.clip(np.floor((np.float32(cy) - 0.5 * S) / D).astype(np.int64), 0, max(0, gh - 2))
+    age = fine_ages
+    same = np.ones(n, bool)
+    for corner in range(4):
+        sx, sy = lx + (corner & 1), ly + (corner >> 1)
+        use = ((perm >> corner) & 1).astype(bool) & (sx < gw) & (sy < gh)
+        idx = np.where(use, sy * gw + sx, 0)
+        same &= ~use | (coarse_ages[idx] == age)
+    go = ((np.asarray(active) != 0) & (age != NEVER) & (age <= max_age) & (seed[:, P] >= -0.5) & same
+          & (weight >= 0.99999))
+    seed[go, :P] = fine_history[go, :P] + (seed[go, :P] - predicted[go, :P])
+    return int(go.sum())
+
+
 def median(cfg: MedianCfg, state, metadata, roi_words=None):
     st = np.ascontiguousarray(state, dtype=np.float32)
     out = np.empty_like(st)
@@ -247,7 +303,8 @@ class Pipeline:
                  zncc_min=0.8, motion_model="affine", interpolation="bspline", subset_size=16,
                  subset_stride=None, spatial_weighting="uniform", gaussian_sigma=None,
                  outlier_policy="replace", median_filter_size=3, median_filter_scope="rejected",
-                 lanes=16, initialization="zero", temporal_max_age=None):
+                 lanes=16, initialization="zero", temporal_max_age=None, temporal_prediction="none",
+                 temporal_fine_correction=False, divergence_patience=None, divergence_min_iterations=4):
         L = pyramid_levels
         self.levels_count = L
         self.iterations = tuple(iterations_per_level or DEFAULT_ITERATIONS[L])
@@ -269,6 +326,11 @@ class Pipeline:
         self.temporal_max_age = 0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age)
         self.temporal_links = None
         self.last_counts = (0, 0)
+        self.velocity = temporal_prediction == "velocity"
+        self.correct = bool(temporal_fine_correction) and L == 2
+        self.patience = (0 if divergence_patience is None else int(divergence_patience),
+                         int(divergence_min_iterations))
+        self.last_predicted = 0
 
     def set_reference(self, image, rois=None, temporal_links=None):
         images = [np.ascontiguousarray(image, dtype=np.uint8)]
@@ -286,6 +348,12 @@ class Pipeline:
         self.history = np.zeros((fine.grid_width * fine.grid_height, stride), np.float32)
         self.history[:, 12 if stride == 16 else 6] = -1.0
         self.ages = np.full(fine.grid_width * fine.grid_height, NEVER, np.uint64)
+        self.previous = self.history.copy()
+        self.previous_ages = self.ages.copy()
+        coarse = self.levels[-1]
+        self.coarse_history = np.zeros((coarse.grid_width * coarse.grid_height, stride), np.float32)
+        self.coarse_history[:, 12 if stride == 16 else 6] = -1.0
+        self.coarse_ages = np.full(coarse.grid_width * coarse.grid_height, NEVER, np.uint64)
 
     def _solve_cfg(self, k):
         lv = self.levels[k]
@@ -295,7 +363,7 @@ class Pipeline:
             lv.width, lv.height, lv.grid_width, lv.grid_height, o["subset_size"], o["subset_stride"],
             MODELS[o["motion_model"]], INTERP[o["interpolation"]], 1 if o["spatial_weighting"] == "gaussian" else 0,
             1 if lv.roi_words is not None else 0, o["gaussian_sigma"], self.iterations[sched],
-            self.tolerances[sched], self.zncc_min, self.lanes, 0)
+            self.tolerances[sched], self.zncc_min, self.lanes, 0, *(self.patience if k == 0 else (0, 4)))
 
     def _median(self, k, state):
         if self.outlier_policy == "disabled":
@@ -314,6 +382,7 @@ class Pipeline:
         for _ in range(1, self.levels_count):
             targets.append(reduce(targets[-1]))
         outputs = [None] * self.levels_count
+        raws = [None] * self.levels_count
         o = self.opts
         for k in range(self.levels_count - 1, -1, -1):
             lv = self.levels[k]
@@ -324,11 +393,19 @@ class Pipeline:
                 fine = self.levels[0]
                 scfg = SeedCfg(fine.grid_width, fine.grid_height, lv.grid_width, lv.grid_width * lv.grid_height,
                                1.0, float(1 << k), float(o["subset_size"]), float(o["subset_stride"]),
-                               MODELS[o["motion_model"]], 0)
-                seed, active, weight = transfer_seed(scfg, self.history, self.temporal_links, return_weight=True)
+                               MODELS[o["motion_model"]], 1 if self.correct else 0)
+                source = self.history
+                if self.velocity:
+                    source, self.last_predicted = temporal_predict(self.history, self.previous, self.ages,
+                                                                   self.previous_ages)
+                seed, active, weight = transfer_seed(scfg, source, self.temporal_links, return_weight=True)
                 links = np.full(lv.grid_width * lv.grid_height, 31, np.uint32) if self.temporal_links is None \
-                    else np.asarray(self.temporal_links, np.uint32)
-                seeded = int(np.sum((weight > 0) & ((links & 16) != 0)))
+                    else np.asarray(self.temporal_links, np.uint32).ravel()
+                counted = (weight > 0) & ((links & 16) != 0)
+                seeded = int(np.sum(counted))
+                if self.correct:
+                    # project_temporal_seed marks every seed it does not count.
+                    seed[~counted, 12 if o["motion_model"] == "quadratic" else 6] = -1.0
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
             elif k == self.levels_count - 1:
                 seed = active = None
@@ -337,17 +414,26 @@ class Pipeline:
                 src = self.levels[k + 1]
                 scfg = SeedCfg(src.grid_width, src.grid_height, lv.grid_width, lv.grid_width * lv.grid_height,
                                float(1 << (k + 1)), float(1 << k), float(o["subset_size"]),
-                               float(o["subset_stride"]), MODELS[o["motion_model"]], 0)
+                               float(o["subset_stride"]), MODELS[o["motion_model"]], 1 if self.correct else 0)
                 seed, active = transfer_seed(scfg, outputs[k + 1], lv.seed_links)
+                if k == 0 and self.correct:
+                    temporal_refine(scfg, self.coarse_history, self.coarse_ages, self.history, self.ages,
+                                    self.temporal_max_age, lv.seed_links, seed, active)
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
             if lv.partial is not None:
                 solve_partial(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], *lv.partial, state,
                               seed, active)
             outputs[k] = self._median(k, state)
-            raw_fine = state
+            raws[k] = state
         if self.temporal:
-            merged = temporal_merge(self.history, self.ages, raw_fine, outputs[0], self.outlier_policy == "replace",
-                                    self.temporal_max_age)
+            filtered = self.outlier_policy == "replace" and not self.velocity
+            if self.velocity:
+                temporal_save_previous(self.history, self.ages, raws[0], outputs[0], self.previous,
+                                       self.previous_ages)
+            merged = temporal_merge(self.history, self.ages, raws[0], outputs[0], filtered, self.temporal_max_age)
+            if self.correct:
+                temporal_merge(self.coarse_history, self.coarse_ages, raws[-1], outputs[-1], filtered,
+                               self.temporal_max_age)
             self.last_counts = (seeded, merged)
         stride = 16 if o["motion_model"] == "quadratic" else 8
         fine = outputs[0].reshape(self.levels[0].grid_height, self.levels[0].grid_width, stride)

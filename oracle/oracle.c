@@ -98,6 +98,8 @@ typedef struct {
     float tolerance, zncc_min;
     int32_t lanes;             /* SOLVER_WORKGROUP_SIZE, 16 by default */
     int32_t preserve_disabled;
+    int32_t divergence_patience;       /* DIVERGENCE_PATIENCE, 0 off (icgn.wgsl:100-111) */
+    int32_t divergence_min_iterations; /* DIVERGENCE_MIN_ITERATIONS */
 } ora_solve_cfg;
 
 typedef struct {
@@ -346,6 +348,15 @@ static float quadratic_change(const ora_solve_cfg* c, const float d[12]) {
     return m;
 }
 
+/* divergence_status, icgn.wgsl:103-111. */
+static float divergence_status(const ora_solve_cfg* c, float change, int count, float status, float* previous,
+                               int* stalled) {
+    if (c->divergence_patience == 0 || status != 0.0f) return status;
+    if (count >= c->divergence_min_iterations) *stalled = change >= *previous ? *stalled + 1 : 0;
+    *previous = change;
+    return *stalled >= c->divergence_patience ? 3.0f : status;
+}
+
 static float convergence_status(const ora_solve_cfg* c, float change, float zncc) {
     if (change >= c->tolerance) return 0.0f;
     return zncc >= c->zncc_min ? 1.0f : -2.0f;
@@ -401,9 +412,14 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
         by1 = box[3] - he;
     }
 
+    float previous_change = 3.402823e38f;
+    int stalled = 0;
+    const int patience = c->divergence_patience;
     for (int it = 0; it < 30; ++it) {
         int inside = 1;
-        if (model == 1) {
+        if (patience && status == 3.0f) {
+            /* icgn.wgsl:647: a stopped solve skips the bounds test. */
+        } else if (model == 1) {
             float minx = cx - 0.5f + bx0 + T[0], miny = cy - 0.5f + by0 + T[1];
             float maxx = cx - 0.5f + bx1 + T[0], maxy = cy - 0.5f + by1 + T[1];
             inside = minx >= 2.0f && miny >= 2.0f && maxx <= W - 3.0f && maxy <= H - 3.0f;
@@ -519,7 +535,8 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
                 status = -1.0f;
             } else {
                 float ch = quadratic_change(c, d);
-                status = convergence_status(c, ch, zncc);
+                status = divergence_status(c, ch, it + 1, convergence_status(c, ch, zncc), &previous_change,
+                                           &stalled);
             }
         } else {
             float rj[6] = {pa[0][3], pb[0][0], pb[0][1], pb[0][2], pb[0][3], pc[0][0]};
@@ -545,10 +562,12 @@ static void solve_subset(const solve_ctx* s, int subset, const float* seed, int 
             } else {
                 compose_standard(model, T, L, d, det);
                 float ch = standard_change(c, d);
-                status = convergence_status(c, ch, zncc);
+                status = divergence_status(c, ch, it + 1, convergence_status(c, ch, zncc), &previous_change,
+                                           &stalled);
             }
         }
     }
+    if (patience && status == 3.0f) status = 0.0f; /* icgn.wgsl:928-929 */
     if (model == 2) {
         float v[16] = {T[0], L[0], L[1], T[1], L[2], L[3], C[0], C[1], C[2], C[3], C[4], C[5], status, iters, 0.0f, 0.0f};
         memcpy(out, v, sizeof v);

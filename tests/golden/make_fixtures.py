@@ -39,6 +39,19 @@ PARTIAL_CASES = (
 )
 
 
+# Temporal-initialisation cases (translation: frames are rolls of the reference; affine: frame 1
+# is the tensile pair), frames 1 and/or 2 (frame 0 equals the zero start).
+TEMPORAL_CASES = (
+    ("translation-l2-none-temporal", (1, 2)),
+    ("translation-l3-none-temporal", (1, 2)),
+    ("affine-l2-none-temporal", (1,)),
+    # Latency profile: velocity prediction, fine correction, divergence patience 3 from iteration 4.
+    ("translation-l2-none-latency", (1, 2)),
+    ("translation-l2-full-latency", (2,)),
+    ("translation-l2-partial-latency", (2,)),
+)
+
+
 def main() -> None:
     shutil.copyfile(SRC / "assets/frame_000.png", OUT / "frame_000.png")
     shutil.copyfile(SRC / "assets/frame_010.png", OUT / "frame_010.png")
@@ -48,6 +61,11 @@ def main() -> None:
     with np.load(SRC / "benchmarks/results/binding-budget-20261031/before.npz", allow_pickle=False) as data:
         for case in CASES:
             for frame in (0, 1):
+                for key in ("parameters", "status", "iterations"):
+                    name = f"{case}-f{frame}-{key}"
+                    fields[name] = data[name]
+        for case, frames in TEMPORAL_CASES:
+            for frame in frames:
                 for key in ("parameters", "status", "iterations"):
                     name = f"{case}-f{frame}-{key}"
                     fields[name] = data[name]

@@ -93,3 +93,23 @@ def test_result_views_share_one_packed_buffer(tmp_path):
     with np.load(path, allow_pickle=False) as data:
         assert data["parameters"].shape == (3, 4, 6)
         assert str(data["parameter_order"][0]) == "tx"
+
+
+def test_latency_arguments_are_rejected_before_device_creation():
+    """test_latency_options.py:14-35."""
+    for value in (None, True, "linear", "", []):
+        with pytest.raises(ValueError, match="prediction"):
+            ICGNWGPUFullPyramid(temporal_prediction=value)
+    with pytest.raises(ValueError, match="requires temporal"):
+        ICGNWGPUFullPyramid(temporal_prediction="velocity")
+    for name in ("divergence_patience", "divergence_min_iterations"):
+        for value in (0, -1, True, 1.5, "3", 31):
+            with pytest.raises(ValueError, match=name):
+                ICGNWGPUFullPyramid(**{name: value})
+    for options in ({"temporal_fine_correction": 1}, {"temporal_fine_correction": True},
+                    {"temporal_fine_correction": True, "initialization": "temporal", "pyramid_levels": 3}):
+        with pytest.raises(ValueError, match="temporal_fine_correction"):
+            ICGNWGPUFullPyramid(**options)
+    for options in ({"pyramid_policy": "unknown"}, {"pyramid_policy": "adaptive"}):
+        with pytest.raises(ValueError):
+            ICGNWGPUFullPyramid(**options)
