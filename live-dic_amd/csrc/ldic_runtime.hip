@@ -595,7 +595,7 @@ int validate_initialization(const ldic_engine* e, const ldic_initialization* in)
         return set_error(LDIC_ERR_ARGUMENT, "temporal_prediction must be 0 (none) or 1 (velocity)");
     if (in->temporal_prediction && in->kind != LDIC_INIT_TEMPORAL)
         return set_error(LDIC_ERR_ARGUMENT, "temporal_prediction requires temporal initialization");
-    if (in->temporal_fine_correction && (in->kind != LDIC_INIT_TEMPORAL || e->levels != 2))
+    if (in->temporal_fine_correction && (in->kind != LDIC_INIT_TEMPORAL || e->opt.pyramid_levels != 2))
         return set_error(LDIC_ERR_ARGUMENT, "temporal_fine_correction requires temporal initialization and two levels");
     if (in->divergence_patience < 0 || in->divergence_patience > 30)
         return set_error(LDIC_ERR_ARGUMENT, "divergence_patience must be 0 (off) or an integer from 1 to 30");
