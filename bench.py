@@ -83,6 +83,7 @@ LARGE_PAIRS = 6
 LARGE_LEVELS = 4                       # ZNCC path: 1/8-resolution search covers +-512 px
 LARGE_SEARCH_RADIUS = 504              # finest-level pixels (coarsest-level radius 64)
 LARGE_ITERATIONS = (12, 8, 8, 16)
+LARGE_RESEED = 2                        # neighbour re-seeding passes after the finest solve, both paths
 
 
 def _large_targets(count: int, seed: int = 5150):
@@ -158,7 +159,7 @@ def learned_vs_zncc(reps: int = 3) -> dict:
 
     zeng = ICGNWGPUFullPyramid(pyramid_levels=LARGE_LEVELS, iterations_per_level=LARGE_ITERATIONS,
                                subset_size=SUBSET, subset_stride=STRIDE, initialization="zncc",
-                               seed_search_radius=LARGE_SEARCH_RADIUS)
+                               seed_search_radius=LARGE_SEARCH_RADIUS, reseed_passes=LARGE_RESEED)
     info = zeng.set_reference(host)
     gw, gh = info.grid
     zncc = timed(lambda t: _native.check(lib.ldic_register_resident(zeng._handle, 0, t.address, WIDTH)),
@@ -170,14 +171,14 @@ def learned_vs_zncc(reps: int = 3) -> dict:
     leng.set_reference(host, iterations=LARGE_ITERATIONS[-1], tolerance=5e-3)
     net = seednet.SeedNet()
     net.load(seednet.WEIGHTS_PATH)
-    seeder = seednet.LearnedSeeder(leng, net)
+    seeder = seednet.LearnedSeeder(leng, net, reseed_passes=LARGE_RESEED)
     seeder.set_reference(ref.address)
     learned = timed(lambda t: seeder.submit(t.address), leng._handle, (gh, gw, 8))
     leng.close()
     zncc["path"] = f"{LARGE_LEVELS}-level pyramid {list(LARGE_ITERATIONS)}, coarsest-level ZNCC search " \
-                   f"+-{LARGE_SEARCH_RADIUS // 8 + 1} px"
+                   f"+-{LARGE_SEARCH_RADIUS // 8 + 1} px, {LARGE_RESEED} neighbour re-seeding passes"
     learned.update({"path": "seed network (BF16 MFMA) + finest-level IC-GN (16 iterations, tolerance 5e-3), "
-                            "ZNCC fallback (+-8 px) below 0.9",
+                            f"ZNCC fallback (+-8 px) below 0.9, {LARGE_RESEED} neighbour re-seeding passes",
                     "network_parameters": net.arch.parameter_count(),
                     "network_gmac_per_frame": net.arch.macs(HEIGHT >> 2, WIDTH >> 2) / 1e9,
                     "network_forward_ms": seeder.tiles.get("forward_ms")})

@@ -164,6 +164,10 @@ typedef struct ldic_initialization {
                                          iterations without a decrease of its change (icgn.wgsl:103-111);
                                          0 off */
     int32_t divergence_min_iterations; /* 1..30: iterations before the patience counts (default 4) */
+    int32_t reseed_passes;    /* seeded registration, 0..8: after the fallback, subsets still unaccepted are
+                                 seeded from the warp of their nearest accepted subset (one with at least two
+                                 accepted neighbours), extrapolated to their centre, and re-solved; translation
+                                 and affine.  An addition without a reference counterpart. */
 } ldic_initialization;
 
 /* Partial-support subsets of one level (roi_policy="partial"; roi.py:266-355,

@@ -153,6 +153,11 @@ hipError_t launch_temporal_refine(const RefineArgs& a, hipStream_t s);
 hipError_t launch_scatter_rows(int count, int row, const uint32_t* index, const float* src, float* dst,
                                hipStream_t s);
 hipError_t launch_affine_zncc(const ZnccArgs& a, hipStream_t s);
+// Neighbour re-seeding of unaccepted subsets from their nearest accepted
+// subset (jump flooding); writes seed and active for a preserving retry.
+hipError_t launch_reseed(int grid_w, int grid_h, int model, float D, const float* state, const float* metadata,
+                         const uint8_t* selected, uint32_t* near_a, uint32_t* near_b, float* seed, uint32_t* active,
+                         hipStream_t s);
 
 // Exhaustive integer ZNCC search; LDS bytes per workgroup for (S, R).
 size_t zncc_search_lds_bytes(int S, int R);

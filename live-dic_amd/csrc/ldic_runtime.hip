@@ -116,6 +116,8 @@ struct SlotLevel {
     uint8_t* mask = nullptr;        // subsets to re-solve from a ZNCC seed
     float* retry_seed = nullptr;
     uint32_t* retry_active = nullptr;
+    uint32_t* near_a = nullptr;     // neighbour re-seeding: nearest accepted subset (ping-pong)
+    uint32_t* near_b = nullptr;
 };
 
 constexpr int kEventRing = 256;
@@ -236,6 +238,8 @@ int create_slot(ldic_engine* e, Slot** out) {
         CHECK(s->arena.alloc(&sl.mask, n));
         CHECK(s->arena.alloc(&sl.retry_seed, n * stride));
         CHECK(s->arena.alloc(&sl.retry_active, n));
+        CHECK(s->arena.alloc(&sl.near_a, n));
+        CHECK(s->arena.alloc(&sl.near_b, n));
     }
     CHECK(s->arena.alloc(&s->zncc, (size_t)e->lv[0].count()));
     CHECK(s->arena.alloc(&s->counts, 4));
@@ -326,6 +330,25 @@ int launch_partial(ldic_engine* e, int k, SlotLevel& sl, const float* seed, cons
     a.min_iters = min_iters;
     if (e->opt.spatial_weighting == 0) a.inv_var = 0.0f;
     HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, 1, a, st));
+    return LDIC_OK;
+}
+
+// Neighbour re-seeding passes of the finest level: unaccepted subsets restart
+// from the warp of their nearest accepted subset and are re-solved; the rest
+// keep their state.
+int reseed_passes(ldic_engine* e, SlotLevel& sl, hipStream_t st) {
+    const Level& L = e->lv[0];
+    const int model = e->opt.motion_model;
+    for (int pass = 0; pass < e->init.reseed_passes; ++pass) {
+        HIP_CHECK(launch_reseed(L.gw, L.gh, model, (float)e->opt.subset_stride, sl.state, L.metadata,
+                                L.roi ? L.selected : nullptr, sl.near_a, sl.near_b, sl.retry_seed, sl.retry_active,
+                                st));
+        SolveArgs a = solve_args(e, 0, sl, sl.retry_seed, sl.retry_active, 1);
+        a.patience = e->init.divergence_patience;
+        a.min_iters = e->init.divergence_min_iterations;
+        HIP_CHECK(launch_solve(model, e->opt.interpolation, e->opt.spatial_weighting, a, st));
+        CHECK(launch_partial(e, 0, sl, sl.retry_seed, sl.retry_active, 1, st, nullptr, a.patience, a.min_iters));
+    }
     return LDIC_OK;
 }
 
@@ -492,6 +515,7 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
         }
         HIP_CHECK(launch_solve(e->opt.motion_model, e->opt.interpolation, e->opt.spatial_weighting, a, st));
         CHECK(launch_partial(e, k, sl, seed, active, 0, st, nullptr, a.patience, a.min_iters));
+        if (k == 0) CHECK(reseed_passes(e, sl, st));
         if (k == 0) HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
         CHECK(launch_median_level(e, k, sl, e->opt.outlier_policy == kMedian && k > 0, st));
     }
@@ -578,6 +602,7 @@ int enqueue_seeded(ldic_engine* e, Slot* s, bool include_zncc) {
                                solve_args(e, 0, sl, sl.retry_seed, sl.retry_active, 1), st));
         CHECK(launch_partial(e, 0, sl, sl.retry_seed, sl.retry_active, 1, st));
     }
+    CHECK(reseed_passes(e, sl, st));
     HIP_CHECK(hipEventRecord(s->ev_solve1[ring], st));
     CHECK(launch_median_level(e, 0, sl, false, st));
     if (include_zncc) HIP_CHECK(launch_affine_zncc(zncc_args(e, sl, level_output(e, sl), s->zncc), st));
@@ -599,6 +624,10 @@ int validate_initialization(const ldic_engine* e, const ldic_initialization* in)
         return set_error(LDIC_ERR_ARGUMENT, "temporal_fine_correction requires temporal initialization and two levels");
     if (in->divergence_patience < 0 || in->divergence_patience > 30)
         return set_error(LDIC_ERR_ARGUMENT, "divergence_patience must be 0 (off) or an integer from 1 to 30");
+    if (in->reseed_passes < 0 || in->reseed_passes > 8)
+        return set_error(LDIC_ERR_ARGUMENT, "reseed_passes must be an integer from 0 to 8");
+    if (in->reseed_passes && e->opt.motion_model == kQuadratic)
+        return set_error(LDIC_ERR_ARGUMENT, "neighbour re-seeding supports translation and affine");
     if (in->divergence_patience && (in->divergence_min_iterations < 1 || in->divergence_min_iterations > 30))
         return set_error(LDIC_ERR_ARGUMENT, "divergence_min_iterations must be an integer from 1 to 30");
     const bool search = in->kind == LDIC_INIT_ZNCC, fallback = in->fallback_zncc > -1.0f;
@@ -1648,8 +1677,8 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
             return set_error(LDIC_ERR_ARGUMENT, "temporal initialisation is not supported in a batch");
         if (has_partial(engines[i]))
             return set_error(LDIC_ERR_ARGUMENT, "partial-support ROI subsets are not supported in a batch");
-        if (engines[i]->init.divergence_patience)
-            return set_error(LDIC_ERR_ARGUMENT, "divergence patience is not supported in a batch");
+        if (engines[i]->init.divergence_patience || engines[i]->init.reseed_passes)
+            return set_error(LDIC_ERR_ARGUMENT, "divergence patience and re-seeding are not supported in a batch");
         for (int k = 0; k < i; ++k)
             if (engines[k] == engines[i]) return set_error(LDIC_ERR_ARGUMENT, "engine %d appears twice", i);
     }

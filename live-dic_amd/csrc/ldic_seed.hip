@@ -21,6 +21,8 @@
 #include "ldic_device.h"
 #include "ldic_kernels.h"
 
+#include <utility>
+
 namespace ldic {
 namespace {
 
@@ -304,7 +306,122 @@ __global__ void k_retry_mask(int count, int P, int stride, const float* state, c
     mask[i] = (selected == nullptr || selected[i]) && !good ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Neighbour re-seeding: every selected subset that is still unaccepted after
+// the seeded solve (and its ZNCC fallback) takes the warp of its nearest
+// accepted subset, extrapolated to its own centre, as a new seed.  Sources
+// are accepted subsets with at least two accepted 8-neighbours, so a lone
+// false acceptance does not spread.  The nearest source is found by jump
+// flooding over the subset grid (Rong & Tan 2006, "Jump flooding in GPU with
+// applications to Voronoi diagram and distance transform"): steps N/2, ..., 1
+// and one more step 1; ties go to the smaller subset index.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoSource = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool accepted_at(const float* state, int stride, int P, int i) {
+    return state[(size_t)i * stride + P] >= 0.5f;
+}
+
+__global__ void k_reseed_sources(int gw, int gh, int P, int stride, const float* state, uint32_t* nearest) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= gw * gh) return;
+    uint32_t out = kNoSource;
+    if (accepted_at(state, stride, P, i)) {
+        const int x = i % gw, y = i / gw;
+        int n = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = x + dx, ny = y + dy;
+                if ((dx || dy) && nx >= 0 && ny >= 0 && nx < gw && ny < gh && accepted_at(state, stride, P, ny * gw + nx))
+                    ++n;
+            }
+        if (n >= 2) out = (uint32_t)i;
+    }
+    nearest[i] = out;
+}
+
+__global__ void k_jump_flood(int gw, int gh, int step, const uint32_t* in, uint32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= gw * gh) return;
+    const int x = i % gw, y = i / gw;
+    uint32_t best = in[i];
+    int bd = 0x7FFFFFFF;
+    if (best != kNoSource) {
+        const int bx = (int)(best % (uint32_t)gw) - x, by = (int)(best / (uint32_t)gw) - y;
+        bd = bx * bx + by * by;
+    }
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int nx = x + dx * step, ny = y + dy * step;
+            if ((dx == 0 && dy == 0) || nx < 0 || ny < 0 || nx >= gw || ny >= gh) continue;
+            const uint32_t c = in[ny * gw + nx];
+            if (c == kNoSource) continue;
+            const int cx = (int)(c % (uint32_t)gw) - x, cy = (int)(c / (uint32_t)gw) - y;
+            const int d = cx * cx + cy * cy;
+            if (d < bd || (d == bd && c < best)) {
+                best = c;
+                bd = d;
+            }
+        }
+    out[i] = best;
+}
+
+// The extrapolation is written with explicit roundings so that it does not
+// depend on floating-point contraction.
+__global__ void k_reseed_apply(int gw, int count, int P, int stride, int M, float D, const float* state,
+                               const float* metadata, const uint8_t* selected, const uint32_t* nearest, float* seed,
+                               uint32_t* active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float meta1 = metadata[(size_t)i * M + 1];
+    const bool usable = (meta1 >= 0.5f || meta1 == -4.0f) && (selected == nullptr || selected[i]);
+    const uint32_t src = nearest[i];
+    if (!usable || accepted_at(state, stride, P, i) || src == kNoSource) {
+        active[i] = 0u;
+        return;
+    }
+    const float* q = state + (size_t)src * stride;
+    const float dx = D * (float)(i % gw - (int)(src % (uint32_t)gw));
+    const float dy = D * (float)(i / gw - (int)(src / (uint32_t)gw));
+    float* o = seed + (size_t)i * stride;
+    o[0] = __fadd_rn(__fadd_rn(q[0], __fmul_rn(q[1], dx)), __fmul_rn(q[2], dy));
+    o[1] = q[1];
+    o[2] = q[2];
+    o[3] = __fadd_rn(__fadd_rn(q[3], __fmul_rn(q[4], dx)), __fmul_rn(q[5], dy));
+    o[4] = q[4];
+    o[5] = q[5];
+    for (int k = 6; k < stride; ++k) o[k] = 0.0f;
+    active[i] = 1u;
+}
+
 }  // namespace
+
+hipError_t launch_reseed(int grid_w, int grid_h, int model, float D, const float* state, const float* metadata,
+                         const uint8_t* selected, uint32_t* near_a, uint32_t* near_b, float* seed, uint32_t* active,
+                         hipStream_t s) {
+    const int count = grid_w * grid_h;
+    if (count <= 0) return hipSuccess;
+    const int P = param_count(model), stride = state_stride(model), M = metadata_stride(model);
+    const dim3 grid((count + 255) / 256), block(256);
+    hipLaunchKernelGGL(k_reseed_sources, grid, block, 0, s, grid_w, grid_h, P, stride, state, near_a);
+    int step = 1;
+    while (step * 2 < max(grid_w, grid_h)) step *= 2;
+    uint32_t *in = near_a, *out = near_b;
+    for (bool extra = false;; ) {
+        hipLaunchKernelGGL(k_jump_flood, grid, block, 0, s, grid_w, grid_h, step, in, out);
+        std::swap(in, out);
+        if (step > 1) {
+            step /= 2;
+        } else if (!extra) {
+            extra = true;
+        } else {
+            break;
+        }
+    }
+    hipLaunchKernelGGL(k_reseed_apply, grid, block, 0, s, grid_w, count, P, stride, M, D, state, metadata, selected,
+                       in, seed, active);
+    return hipGetLastError();
+}
 
 hipError_t launch_retry_mask(int count, int model, const float* state, const float* zncc, float zncc_min,
                              const uint8_t* selected, uint8_t* mask, hipStream_t s) {

@@ -92,6 +92,7 @@ class Initialization(ctypes.Structure):
         ("temporal_fine_correction", ctypes.c_int32),
         ("divergence_patience", ctypes.c_int32),
         ("divergence_min_iterations", ctypes.c_int32),
+        ("reseed_passes", ctypes.c_int32),
     ]
 
 

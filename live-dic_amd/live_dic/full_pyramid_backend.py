@@ -76,8 +76,9 @@ class ICGNWGPUFullPyramid(_NativeEngine):
     the coarsest level is seeded by an exhaustive integer ZNCC search covering
     ``seed_search_radius`` pixels of the finest level.  ``divergence_patience``
     stops unconverged finest-level solves whose change stopped decreasing
-    (icgn.wgsl:103-111).  Not built: the Fourier and CPM-HOG matching
-    initialisers and the adaptive pyramid policy.
+    (icgn.wgsl:103-111).  ``reseed_passes`` (an addition, 0-8) re-solves finest-level
+    subsets still unaccepted from the warp of their nearest accepted subset.  Not built: the
+    Fourier and CPM-HOG matching initialisers and the adaptive pyramid policy.
     """
 
     def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
@@ -92,7 +93,7 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                  temporal_max_age: int | None = None, temporal_fine_correction: bool = False,
                  temporal_prediction: str = "none", pyramid_policy: str = "fixed", seed_search_radius: int = 32,
                  divergence_patience: int | None = None, divergence_min_iterations: int = 4,
-                 **unsupported) -> None:
+                 reseed_passes: int = 0, **unsupported) -> None:
         if unsupported:
             raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
         if not isinstance(pyramid_policy, str) or pyramid_policy not in ("fixed", "adaptive"):
@@ -111,6 +112,11 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         self.divergence_patience = (None if divergence_patience is None
                                     else _iteration_count(divergence_patience, "divergence_patience"))
         self.divergence_min_iterations = _iteration_count(divergence_min_iterations, "divergence_min_iterations")
+        if isinstance(reseed_passes, bool) or not isinstance(reseed_passes, int) or not 0 <= reseed_passes <= 8:
+            raise ValueError("reseed_passes must be an integer from 0 to 8")
+        if reseed_passes and motion_model == "quadratic":
+            raise ValueError("reseed_passes supports the translation and affine models")
+        self.reseed_passes = reseed_passes
         if not isinstance(temporal_fine_correction, bool):
             raise ValueError("temporal_fine_correction must be a boolean")
         if temporal_fine_correction and (initialization != "temporal" or _level_count(pyramid_levels) != 2):
@@ -158,12 +164,13 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         sequence = dict(temporal_prediction=1 if temporal_prediction == "velocity" else 0,
                         temporal_fine_correction=int(temporal_fine_correction),
                         divergence_patience=self.divergence_patience or 0,
-                        divergence_min_iterations=self.divergence_min_iterations)
+                        divergence_min_iterations=self.divergence_min_iterations,
+                        reseed_passes=reseed_passes)
         if initialization == "zncc":
             self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius, **sequence)
         elif initialization == "temporal":
             self._set_initialization(_native.INIT_TEMPORAL, temporal_max_age=temporal_max_age, **sequence)
-        elif self.divergence_patience is not None:
+        elif self.divergence_patience is not None or reseed_passes:
             self._set_initialization(_native.INIT_ZERO, **sequence)
 
     def set_reference(self, image: ImageInput, *, iterations_per_level: tuple[int, ...] | None = None,
@@ -198,6 +205,8 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if self.initialization == "zncc":
             self._pyramid_config.update(initialization="zncc", seedSearchRadius=self.seed_search_radius,
                                         coarseSearchRadius=self.coarse_search_radius)
+        if self.reseed_passes:
+            self._pyramid_config.update(reseedPasses=self.reseed_passes)
         if self.divergence_patience is not None:
             self._pyramid_config.update(divergencePatience=self.divergence_patience,
                                         divergenceMinIterations=self.divergence_min_iterations,

@@ -429,12 +429,13 @@ class _NativeEngine:
     def _set_initialization(self, kind: int, search_radius: int = 0, fallback_zncc: float | None = None,
                             fallback_radius: int = 0, temporal_max_age: int | None = None,
                             temporal_prediction: int = 0, temporal_fine_correction: int = 0,
-                            divergence_patience: int = 0, divergence_min_iterations: int = 4) -> None:
+                            divergence_patience: int = 0, divergence_min_iterations: int = 4,
+                            reseed_passes: int = 0) -> None:
         init = _native.Initialization(kind, int(search_radius), -2.0 if fallback_zncc is None else float(fallback_zncc),
                                       int(fallback_radius),
                                       0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age),
                                       int(temporal_prediction), int(temporal_fine_correction),
-                                      int(divergence_patience), int(divergence_min_iterations))
+                                      int(divergence_patience), int(divergence_min_iterations), int(reseed_passes))
         _native.check(self._lib.ldic_set_initialization(self._handle, ctypes.byref(init)))
         self._initialization = init
 
@@ -573,12 +574,14 @@ class ICGNWGPU(_NativeEngine):
     def register_seeded(self, target: ImageInput, parameters, active, *, iterations: int | None = None,
                         tolerance: float | None = None, zncc_min: float | None = None,
                         include_zncc: bool = False, fallback_zncc: float | None = None,
-                        fallback_radius: int = 8) -> RegistrationResult:
+                        fallback_radius: int = 8, reseed_passes: int = 0) -> RegistrationResult:
         """Solve active affine subsets from supplied initial states (wgpu_backend.py:2773-2819).
 
         ``fallback_zncc`` (an addition; None keeps the reference behaviour) re-solves every
         selected subset that did not converge or whose final ZNCC is below it from the best
-        integer ZNCC shift within ``fallback_radius`` pixels of its seed.
+        integer ZNCC shift within ``fallback_radius`` pixels of its seed.  ``reseed_passes``
+        (an addition, 0-8) then re-solves the subsets still unaccepted from the warp of their
+        nearest accepted subset, extrapolated to their centre, once per pass.
         """
         if not self._reference_ready:
             raise ICGNError("Call set_reference() before register_seeded().")
@@ -597,15 +600,18 @@ class ICGNWGPU(_NativeEngine):
             raise ValueError("active parameters must be finite")
         if fallback_zncc is not None and not (-1.0 < float(fallback_zncc) <= 1.0):
             raise ValueError("fallback_zncc must be in (-1, 1] or None")
+        if isinstance(reseed_passes, bool) or not isinstance(reseed_passes, int) or not 0 <= reseed_passes <= 8:
+            raise ValueError("reseed_passes must be an integer from 0 to 8")
         started = time.perf_counter()
         frame = self._prepare_target(target)
         config = self._resolve_config(iterations, tolerance, zncc_min)
         prep_ms = (time.perf_counter() - started) * 1e3
-        self._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius if fallback_zncc is not None else 0)
+        self._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius if fallback_zncc is not None else 0,
+                                 reseed_passes=reseed_passes)
         try:
             return self._seeded(frame, values, mask, include_zncc, started, prep_ms, config)
         finally:
-            if fallback_zncc is not None:
+            if fallback_zncc is not None or reseed_passes:
                 self._set_initialization(_native.INIT_ZERO)
 
     @_exclusive_stream

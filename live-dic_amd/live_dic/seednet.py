@@ -571,10 +571,12 @@ class LearnedSeeder:
     """Finest-level seeds of a single-level engine from the network, queued on the engine's slot
     stream: the target is reduced twice by the integer pyramid, the network runs on the pair,
     and ``ldic_nn_seed_from_flow`` reads translation and gradient seeds at every subset centre.
-    The seeded solve, with the ZNCC fallback, then runs on the same stream."""
+    The seeded solve, with the ZNCC fallback and ``reseed_passes`` passes of neighbour
+    re-seeding (unaccepted subsets restart from their nearest accepted subset's warp), then
+    runs on the same stream."""
 
     def __init__(self, engine, net: SeedNet, *, slot: int = 0, fallback_zncc: float = 0.9,
-                 fallback_radius: int = 8, gradients: bool = True):
+                 fallback_radius: int = 8, reseed_passes: int = 2, gradients: bool = True):
         torch = net.torch
         self.engine, self.net, self.slot = engine, net, slot
         lib = engine._lib
@@ -582,7 +584,7 @@ class LearnedSeeder:
         _native.check(lib.ldic_slot_stream(engine._handle, slot, ctypes.byref(handle)))
         self.stream = int(handle.value)
         net.use_stream(self.stream)
-        engine._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius)
+        engine._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius, reseed_passes=reseed_passes)
         self.width, self.height, self.gw, self.gh = engine._level_shapes[0]
         dev = net.device
         L = net.arch.input_level

@@ -105,10 +105,15 @@ def prefilter(image: np.ndarray, fir: np.ndarray | None = None) -> np.ndarray:
     return out
 
 
-def solve(cfg: SolveCfg, reference, metadata, inverse, coeff, target, seed=None, active=None) -> np.ndarray:
+def solve(cfg: SolveCfg, reference, metadata, inverse, coeff, target, seed=None, active=None,
+          state=None) -> np.ndarray:
+    """One solver pass; with ``cfg.preserve_disabled`` and ``state`` (updated in place) the
+    inactive subsets keep their previous state (retry_seeded, icgn.wgsl:1010-1017)."""
     stride = 16 if cfg.motion_model == 2 else 8
     n = cfg.grid_width * cfg.grid_height
-    state = np.zeros((n, stride), dtype=np.float32)
+    if state is None:
+        state = np.zeros((n, stride), dtype=np.float32)
+    assert state.shape == (n, stride) and state.dtype == np.float32 and state.flags.c_contiguous
     ref = np.ascontiguousarray(reference, dtype=np.float32)
     meta = np.ascontiguousarray(metadata, dtype=np.float32)
     inv = np.ascontiguousarray(inverse, dtype=np.float32)
@@ -171,6 +176,70 @@ def temporal_merge(history, ages, raw, output, use_filtered, max_age):
     ages[aging] = np.minimum(ages[aging], 0xFFFFFFFD) + 1
     history[aging & (ages > max_age), P] = -1.0
     return int(upd.sum())
+
+
+NO_SOURCE = 0xFFFFFFFF
+
+
+def jump_flood_nearest(sources, gw, gh):
+    """Nearest source subset of every grid point by jump flooding (steps N/2 .. 1, then one more
+    step 1; candidates are the 3x3 points at the step; ties to the smaller index), as the
+    library's neighbour re-seeding computes it."""
+    n = gw * gh
+    near = np.where(sources, np.arange(n, dtype=np.int64), -1).reshape(gh, gw)
+    yy, xx = np.indices((gh, gw))
+    steps = []
+    step = 1
+    while step * 2 < max(gw, gh):
+        step *= 2
+    while step >= 1:
+        steps.append(step)
+        step //= 2
+    steps.append(1)
+    for step in steps:
+        best = near.copy()
+        bd = np.where(best >= 0, (best % gw - xx) ** 2 + (best // gw - yy) ** 2, np.iinfo(np.int64).max)
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dx == 0 and dy == 0:
+                    continue
+                ny, nx = yy + dy * step, xx + dx * step
+                ok = (nx >= 0) & (ny >= 0) & (nx < gw) & (ny < gh)
+                c = np.full((gh, gw), -1, np.int64)
+                c[ok] = near[ny[ok], nx[ok]]
+                d = np.where(c >= 0, (c % gw - xx) ** 2 + (c // gw - yy) ** 2, np.iinfo(np.int64).max)
+                take = (c >= 0) & ((d < bd) | ((d == bd) & (c < best)))
+                best = np.where(take, c, best)
+                bd = np.where(take, d, bd)
+        near = best
+    return near.ravel()
+
+
+def reseed(state, metadata, gw, gh, stride_px, selected=None):
+    """Neighbour re-seeding (the library's addition after the seeded fallback): sources are
+    accepted subsets with at least two accepted 8-neighbours; every usable unaccepted subset
+    takes its nearest source's warp extrapolated to its centre.  Returns (seed, active)."""
+    n = gw * gh
+    st = np.asarray(state, np.float32).reshape(n, -1)
+    P = 12 if st.shape[1] == 16 else 6
+    acc = (st[:, P] >= 0.5).reshape(gh, gw)
+    pad = np.pad(acc, 1)
+    nb = sum(pad[1 + dy:1 + dy + gh, 1 + dx:1 + dx + gw].astype(int) for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+             if dx or dy)
+    near = jump_flood_nearest((acc & (nb >= 2)).ravel(), gw, gh)
+    meta1 = np.asarray(metadata, np.float32).reshape(n, -1)[:, 1]
+    usable = ((meta1 >= 0.5) | (meta1 == -4.0)) & (True if selected is None else np.asarray(selected).ravel() != 0)
+    active = usable & ~acc.ravel() & (near >= 0)
+    seed = np.zeros_like(st)
+    i = np.flatnonzero(active)
+    src = near[i]
+    q = st[src]
+    dx = np.float32(stride_px) * (i % gw - src % gw).astype(np.float32)
+    dy = np.float32(stride_px) * (i // gw - src // gw).astype(np.float32)
+    seed[i, 0] = (q[:, 0] + q[:, 1] * dx) + q[:, 2] * dy
+    seed[i, 3] = (q[:, 3] + q[:, 4] * dx) + q[:, 5] * dy
+    seed[i, 1], seed[i, 2], seed[i, 4], seed[i, 5] = q[:, 1], q[:, 2], q[:, 4], q[:, 5]
+    return seed, active.astype(np.uint32)
 
 
 def temporal_predict(history, previous, ages, previous_ages):

@@ -171,3 +171,57 @@ def test_seeded_fallback_recovers_bad_seeds():
     keep = (plain.status == 1.0) & (np.nan_to_num(plain.zncc, nan=0.0) >= 0.9)
     np.testing.assert_array_equal(fixed.parameters[keep], plain.parameters[keep])
     np.testing.assert_array_equal(again.parameters, plain.parameters)
+
+
+def test_neighbour_reseeding_recovers_a_failed_region_and_matches_oracle():
+    """Seeds 14 px off on a block fail; two re-seeding passes restart them from their nearest
+    accepted neighbours' warps.  The whole sequence (seeded solve, re-seed, preserving retry,
+    twice) matches the oracle's, and subsets accepted by the first solve are untouched."""
+    import oracle
+    from helpers import compare_fields
+    from live_dic import ICGNWGPU, synthetic
+
+    _say("register_seeded with neighbour re-seeding, 512^2")
+    w = h = 512
+    fld = synthetic.affine_field(w, h, 14.0, -9.0, math.radians(1.0), 0.004, 0.0)
+    ref, tgt = _speckle_pair(w, h, fld, 17)
+    S, D = 21, 8
+    X, Y = synthetic.subset_centres(w, h, S, D)
+    u, v = fld.displacement(X, Y)
+    grad = fld.gradient(X, Y)
+    params = np.zeros((*X.shape, 6), np.float32)
+    params[..., 0], params[..., 3] = u, v
+    params[..., 1], params[..., 2], params[..., 4], params[..., 5] = (grad[..., k] for k in range(4))
+    bad = np.zeros(X.shape, bool)
+    bad[10:30, 10:40] = True
+    params[bad] = 0.0
+    active = np.ones(X.shape, bool)
+    interior = np.zeros(X.shape, bool)
+    interior[2:-2, 2:-2] = True
+    with ICGNWGPU(subset_size=S, subset_stride=D, outlier_policy="disabled") as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-3)
+        plain = eng.register_seeded(tgt, params, active)
+        fixed = eng.register_seeded(tgt, params, active, reseed_passes=2)
+        plane, meta, inv = eng.export_reference(0)
+        coeff = eng.export_coefficients(0)
+        gw, gh = eng.grid_width, eng.grid_height
+    err = lambda r: np.hypot(r.parameters[..., 0] - u, r.parameters[..., 3] - v)
+    good_plain = (plain.status > 0.5) & (err(plain) < 0.05)
+    good_fixed = (fixed.status > 0.5) & (err(fixed) < 0.05)
+    _say("bad block recovered:", good_plain[bad].mean(), "->", good_fixed[bad].mean())
+    assert good_plain[bad].mean() < 0.2
+    assert good_fixed[bad & interior].mean() > 0.98
+    keep = plain.status == 1.0
+    np.testing.assert_array_equal(fixed.parameters[keep], plain.parameters[keep])
+    cfg = oracle.SolveCfg(w, h, gw, gh, S, D, 0, 0, 0, 0, S / 4, 20, 1e-3, 0.8, 16, 0)
+    seeds = np.zeros((gw * gh, 8), np.float32)
+    seeds[:, :6] = params.reshape(-1, 6)
+    state = oracle.solve(cfg, plane, meta, inv, coeff, tgt, seeds, np.ones(gw * gh, np.uint32))
+    cfg.preserve_disabled = 1
+    for _ in range(2):
+        seed, act = oracle.reseed(state, meta, gw, gh, D)
+        oracle.solve(cfg, plane, meta, inv, coeff, tgt, seed, act, state=state)
+    got = np.concatenate([fixed.parameters, fixed.status[..., None], fixed.iterations[..., None]], -1)
+    stats = compare_fields(got, state.reshape(gh, gw, 8), 6)
+    _say("vs oracle", stats)
+    assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999

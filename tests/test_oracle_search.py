@@ -148,3 +148,54 @@ def test_parabola_refines_a_smooth_peak(true_shift):
     # The parabola's known bias on broad peaks stays well inside the rounding error it removes.
     assert good.sum() > 10 and float(err.max()) < 0.33
     assert float(np.median(err)) < 0.75 * float(np.median(integer_err))
+
+
+# -- neighbour re-seeding ----------------------------------------------------------------------
+@pytest.mark.parametrize("gw,gh,density,seed", [(40, 30, 0.02, 1), (97, 53, 0.005, 2), (64, 64, 0.3, 3), (7, 200, 0.01, 4)])
+def test_jump_flood_finds_nearest_sources(gw, gh, density, seed):
+    """Jump flooding against brute force: every point gets a source, and the found distance is the
+    exact nearest distance for (almost) every point."""
+    rng = np.random.default_rng(seed)
+    src = rng.random(gw * gh) < density
+    src[rng.integers(gw * gh)] = True
+    near = oracle.jump_flood_nearest(src, gw, gh)
+    assert np.all(near >= 0) and np.all(src[near])
+    ys, xs = np.divmod(np.arange(gw * gh), gw)
+    sy, sx = np.divmod(np.flatnonzero(src), gw)
+    exact = ((xs[:, None] - sx[None]) ** 2 + (ys[:, None] - sy[None]) ** 2).min(1)
+    found = (xs - near % gw) ** 2 + (ys - near // gw) ** 2
+    assert np.all(found >= exact)
+    assert np.mean(found == exact) >= 0.99
+    np.testing.assert_array_equal(near[src], np.flatnonzero(src))
+
+
+def test_reseed_extrapolates_accepted_neighbours_and_ignores_lone_acceptances():
+    gw, gh, D = 12, 9, 8.0
+    n = gw * gh
+    yy, xx = np.divmod(np.arange(n), gw)
+    cx, cy = 15.5 + D * xx, 15.5 + D * yy
+    # An exact affine field: u = 3 + 0.01 x - 0.002 y, v = -1 + 0.003 x + 0.004 y.
+    state = np.zeros((n, 8), np.float32)
+    state[:, 0] = 3 + 0.01 * cx - 0.002 * cy
+    state[:, 1], state[:, 2] = 0.01, -0.002
+    state[:, 3] = -1 + 0.003 * cx + 0.004 * cy
+    state[:, 4], state[:, 5] = 0.003, 0.004
+    state[:, 6] = 1.0
+    failed = (xx >= 4) & (xx < 9) & (yy >= 2) & (yy < 7)
+    state[failed, 6] = 0.0
+    lone = 6 * gw + 10
+    state[:, 6][(yy >= 5) & (xx >= 9)] = -1.0          # a rejected corner around one lone acceptance
+    state[lone, 6] = 1.0
+    state[lone, 0] = 99.0
+    meta = np.zeros((n, 16), np.float32)
+    meta[:, 1] = 1.0
+    meta[3, 1] = 0.0                                  # an invalid reference subset is never re-seeded
+    state[3, 6] = 0.0
+    seed, active = oracle.reseed(state, meta, gw, gh, D)
+    assert active[3] == 0 and active[lone] == 0
+    on = active.astype(bool)
+    assert np.all(on == ((state[:, 6] < 0.5) & (np.arange(n) != 3)))
+    # Every seed reproduces the exact field at its own centre, and none comes from the lone subset.
+    np.testing.assert_allclose(seed[on, 0], (3 + 0.01 * cx - 0.002 * cy)[on], atol=2e-5)
+    np.testing.assert_allclose(seed[on, 3], (-1 + 0.003 * cx + 0.004 * cy)[on], atol=2e-5)
+    np.testing.assert_array_equal(seed[on, 1], np.float32(0.01))
