@@ -113,3 +113,11 @@ def test_latency_arguments_are_rejected_before_device_creation():
     for options in ({"pyramid_policy": "unknown"}, {"pyramid_policy": "adaptive"}):
         with pytest.raises(ValueError):
             ICGNWGPUFullPyramid(**options)
+
+
+def test_reseed_arguments_are_rejected_before_device_creation():
+    for value in (-1, 9, True, 1.5, "2"):
+        with pytest.raises(ValueError, match="reseed_passes"):
+            ICGNWGPUFullPyramid(reseed_passes=value)
+    with pytest.raises(ValueError, match="reseed_passes"):
+        ICGNWGPUFullPyramid(reseed_passes=2, motion_model="quadratic")
