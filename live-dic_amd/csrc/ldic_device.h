@@ -16,10 +16,18 @@ constexpr int kQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
 constexpr int kRowHalfMirror = 0x141;
 constexpr int kRowMirror = 0x140;
+constexpr int kRowBcast15 = 0x142;   // lane 15 of each row to the next row
+constexpr int kRowBcast31 = 0x143;   // lane 31 to rows 2 and 3
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_f32(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// DPP move restricted to the rows in ROWS; the other rows read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_rows_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
 
 template <int CTRL>
@@ -49,6 +57,20 @@ __device__ __forceinline__ float wave_sum(float v) {
     v += dpp_f32<kRowHalfMirror>(v);
     v += dpp_f32<kRowMirror>(v);
     return (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+}
+
+// The same sum, with the cross-row step done by row broadcasts: rows 1 and 3
+// add rows 0 and 2, then rows 2 and 3 add lane 31, and lane 63 holds
+// (r3 + r2) + (r1 + r0), bit-identical to wave_sum because float addition is
+// commutative.  Seven VALU operations instead of eleven.
+__device__ __forceinline__ float wave_sum_bcast(float v) {
+    v += dpp_f32<kQuadXor1>(v);
+    v += dpp_f32<kQuadXor2>(v);
+    v += dpp_f32<kRowHalfMirror>(v);
+    v += dpp_f32<kRowMirror>(v);
+    v += dpp_rows_f32<kRowBcast15, 0xA>(v);
+    v += dpp_rows_f32<kRowBcast31, 0xC>(v);
+    return readlane_f32(v, 63);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

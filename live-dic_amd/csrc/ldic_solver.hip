@@ -15,16 +15,20 @@
 
 namespace ldic {
 
-constexpr float kSixth = 1.0f / 6.0f;
-
-// Cubic B-spline weights, icgn.wgsl:230-240 (the division by 6 becomes a product).
-__device__ __forceinline__ void bspline_weights(float t, float w[4]) {
+// Cubic B-spline weights times 6, icgn.wgsl:230-240.  The solver's sums are
+// invariant to a common scale of the sampled values (the mean, norm, ratio and
+// ZNCC all cancel it), so the samplers leave out the 1/36 of the two axes and
+// the norm threshold is scaled instead (kSampleScale).
+__device__ __forceinline__ void bspline_weights6(float t, float w[4]) {
     float t2 = t * t, t3 = t2 * t, inv = 1.0f - t;
-    w[0] = inv * inv * inv * kSixth;
-    w[1] = (3.0f * t3 - 6.0f * t2 + 4.0f) * kSixth;
-    w[2] = (-3.0f * t3 + 3.0f * t2 + 3.0f * t + 1.0f) * kSixth;
-    w[3] = t3 * kSixth;
+    w[0] = inv * inv * inv;
+    w[1] = fmaf(3.0f, t3, fmaf(-6.0f, t2, 4.0f));
+    w[3] = t3;
+    w[2] = fmaf(3.0f, t + t2 - t3, 1.0f);
 }
+
+template <int INTERP>
+constexpr float kSampleScale = INTERP == kBspline ? 36.0f : 1.0f;
 
 __device__ __forceinline__ void keys_weights(float t, float w[4]) {
     float t2 = t * t, t3 = t2 * t;
@@ -49,7 +53,7 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
     int ix = (int)fx, iy = (int)fy;
     float tx = px - fx, ty = py - fy;
     if (INTERP == kLinear) {
-        const float* b = tile + (iy - t.y0) * pitch + (ix - t.x0);
+        const float* b = tile + (__mul24(iy - t.y0, pitch) + (ix - t.x0));
         float top = b[0] * (1.0f - tx) + b[1] * tx;
         float bottom = b[pitch] * (1.0f - tx) + b[pitch + 1] * tx;
         return top * (1.0f - ty) + bottom * ty;
@@ -59,10 +63,10 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
         keys_weights(tx, wx);
         keys_weights(ty, wy);
     } else {
-        bspline_weights(tx, wx);
-        bspline_weights(ty, wy);
+        bspline_weights6(tx, wx);
+        bspline_weights6(ty, wy);
     }
-    const float* b = tile + (iy - 1 - t.y0) * pitch + (ix - 1 - t.x0);
+    const float* b = tile + (__mul24(iy - 1 - t.y0, pitch) + (ix - 1 - t.x0));
     float v;
 #pragma unroll
     for (int yy = 0; yy < 4; ++yy) {
@@ -97,8 +101,8 @@ __device__ __forceinline__ float global_sample(const SolveArgs& a, float px, flo
         keys_weights(tx, wx);
         keys_weights(ty, wy);
     } else {
-        bspline_weights(tx, wx);
-        bspline_weights(ty, wy);
+        bspline_weights6(tx, wx);
+        bspline_weights6(ty, wy);
     }
     float v = 0.0f;
 #pragma unroll
@@ -259,7 +263,8 @@ __device__ __forceinline__ void compose_standard(Warp& q, const float d[6], floa
         q.T[1] = q.T[1] - d[3];
         return;
     }
-    float i11 = (1.0f + d[5]) / det, i12 = -d[2] / det, i21 = -d[4] / det, i22 = (1.0f + d[1]) / det;
+    const float r = 1.0f / det;
+    float i11 = (1.0f + d[5]) * r, i12 = -d[2] * r, i21 = -d[4] * r, i22 = (1.0f + d[1]) * r;
     float w11 = 1.0f + q.L[0], w12 = q.L[1], w21 = q.L[2], w22 = 1.0f + q.L[3];
     float itx = -(i11 * d[0] + i12 * d[3]);
     float ity = -(i21 * d[0] + i22 * d[3]);
@@ -375,6 +380,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     const float wsum = WEIGHTED ? meta[MODEL == kQuadratic ? 27 : 15] : (float)SS;
     const float Wf = (float)a.width, Hf = (float)a.height;
     const int ox = gx * D, oy = gy * D;
+    const float cx0 = cx - 0.5f, cy0 = cy - 0.5f;
 
     // Register-resident reference tile.
     float rf[NP], rgx[NP], rgy[NP], plx[NP], ply[NP], pw[WEIGHTED ? NP : 1];
@@ -462,11 +468,26 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         float sq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         // Sample position (centre - 0.5 + l) + d(l): the first sum is an exact pixel
         // coordinate, so the position is rounded once, as in icgn.wgsl:741-743.
-        auto accumulate = [&](auto sampler, float lx, float ly, float f, float gxv, float gyv, float iw) {
-            float dx, dy;
-            displacement<MODEL>(q, lx, ly, dx, dy);
-            float px = (cx - 0.5f + lx) + dx, py = (cy - 0.5f + ly) + dy;
-            float cur = sampler(px, py);
+        // The local position plus displacement is formed first (|l| + |d| is small,
+        // so it rounds finely), then the pixel coordinate of the centre is added
+        // with one rounding per sample, as in icgn.wgsl:741-743.
+        const float ux = 1.0f + q.L[0], vy = 1.0f + q.L[3];
+        auto accumulate = [&](auto sampler, float lx, float ly, float f, float gxv, float gyv, float iw,
+                              float valid) {
+            float ex, ey;
+            if (MODEL == kQuadratic) {
+                float dx, dy;
+                displacement<MODEL>(q, lx, ly, dx, dy);
+                ex = lx + dx;
+                ey = ly + dy;
+            } else if (MODEL == kTranslation) {
+                ex = lx + q.T[0];
+                ey = ly + q.T[1];
+            } else {
+                ex = fmaf(q.L[1], ly, fmaf(ux, lx, q.T[0]));
+                ey = fmaf(q.L[2], lx, fmaf(vy, ly, q.T[1]));
+            }
+            float cur = sampler(cx0 + ex, cy0 + ey) * valid;
             float wc = iw * cur;
             float gxg = gxv * wc, gyg = gyv * wc;
             sc += wc;
@@ -492,13 +513,17 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         };
         auto sweep = [&](auto sampler) {
             if (PPL > 0) {
-                // Slots below `full` hold a pixel in every lane (a wave-uniform test);
-                // only slot `full` is partial.
+                // Slots below `full` hold a pixel in every lane; slot `full` only in
+                // lanes below `rem`: its other lanes sample the subset centre and
+                // contribute zero, so every slot runs with the whole wave and the
+                // slot test is a scalar branch.
                 const int full = SS / kWave, rem = SS - full * kWave;
+                const float tail = lane < rem ? 1.0f : 0.0f;
 #pragma unroll
                 for (int k = 0; k < NP; ++k) {
-                    if (k < full || (k == full && lane < rem))
-                        accumulate(sampler, plx[k], ply[k], rf[k], rgx[k], rgy[k], WEIGHTED ? pw[k] : 1.0f);
+                    if (k <= full)
+                        accumulate(sampler, plx[k], ply[k], rf[k], rgx[k], rgy[k], WEIGHTED ? pw[k] : 1.0f,
+                                   k == full ? tail : 1.0f);
                 }
             } else {
                 for (int p = lane; p < SS; p += kWave) {
@@ -507,7 +532,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                     float lx = (float)col - he, ly = (float)row - he;
                     float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
                     if (partial && !((support[p >> 5] >> (p & 31)) & 1u)) iw = 0.0f;
-                    accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw);
+                    accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw, 1.0f);
                 }
             }
         };
@@ -517,31 +542,34 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         else
             sweep([&](float px, float py) { return global_sample<INTERP>(a, px, py); });
 
-        sc = wave_sum(sc);
-        sc2 = wave_sum(sc2);
-        src = wave_sum(src);
-        sj[0] = wave_sum(sj[0]);
-        sj[3] = wave_sum(sj[3]);
+        sc = wave_sum_bcast(sc);
+        sc2 = wave_sum_bcast(sc2);
+        src = wave_sum_bcast(src);
+        sj[0] = wave_sum_bcast(sj[0]);
+        sj[3] = wave_sum_bcast(sj[3]);
         if (MODEL != kTranslation) {
-            sj[1] = wave_sum(sj[1]);
-            sj[2] = wave_sum(sj[2]);
-            sj[4] = wave_sum(sj[4]);
-            sj[5] = wave_sum(sj[5]);
+            sj[1] = wave_sum_bcast(sj[1]);
+            sj[2] = wave_sum_bcast(sj[2]);
+            sj[4] = wave_sum_bcast(sj[4]);
+            sj[5] = wave_sum_bcast(sj[5]);
         }
         if (MODEL == kQuadratic)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) sq[k] = wave_sum(sq[k]);
+            for (int k = 0; k < 6; ++k) sq[k] = wave_sum_bcast(sq[k]);
 
-        const float mean = sc / wsum;
+        // Hardware reciprocals (1 ulp) of per-subset constants replace two of the
+        // iteration's divisions; the norm keeps a correctly rounded one.
+        const float mean = sc * __builtin_amdgcn_rcpf(wsum);
         const float norm = sqrtf(fmaxf(0.0f, sc2 - sc * mean));
         const float ref_norm = meta[0];
         iters = (float)(it + 1);
-        if (norm < 1e-9f || ref_norm < 1e-9f) {
+        if (norm < 1e-9f * kSampleScale<INTERP> || ref_norm < 1e-9f) {
             status = -2.0f;
             continue;
         }
-        const float ratio = ref_norm / norm;
-        const float zncc = src / (ref_norm * norm);
+        const float inv_norm = 1.0f / norm;
+        const float ratio = ref_norm * inv_norm;
+        const float zncc = src * inv_norm * __builtin_amdgcn_rcpf(ref_norm);
         last_zncc = zncc;
         float change;
         if (MODEL == kQuadratic) {
@@ -611,8 +639,16 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     }
 }
 
+// Register budget: the unweighted translation/affine B-spline and bilinear
+// variants with up to 16 pixels per lane fit 168 VGPRs without spilling, i.e.
+// three waves per SIMD; the others keep the compiler's own allocation.
+constexpr int solve_min_waves(int model, int interp, int ppl, bool weighted) {
+    return (model != kQuadratic && interp != kKeys && !weighted && ppl > 0 && ppl <= 16) ? 3 : 1;
+}
+
 template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
+                          amdgpu_waves_per_eu(solve_min_waves(MODEL, INTERP, PPL, WEIGHTED)))) void k_solve(SolveArgs a) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
