@@ -70,6 +70,8 @@ typedef struct ldic_timing {
     int32_t temporal_seeded;   /* temporal mode: coarsest subsets seeded from the history (last frame) */
     int32_t temporal_updates;  /* temporal mode: finest subsets merged into the history (last frame) */
     int32_t velocity_predicted; /* velocity prediction: finest subsets extrapolated (last frame) */
+    int32_t levels_used;       /* pyramid levels the last frame solved (the adaptive pyramid may use 1) */
+    int32_t temporal_expired;  /* adaptive pyramid: finest history entries that expired (last frame) */
 } ldic_timing;
 
 /* Device selection and kernel set-up: ICGNWGPU.__init__ device/pipeline
@@ -164,6 +166,12 @@ typedef struct ldic_initialization {
                                          iterations without a decrease of its change (icgn.wgsl:103-111);
                                          0 off */
     int32_t divergence_min_iterations; /* 1..30: iterations before the patience counts (default 4) */
+    int32_t adaptive_pyramid; /* LDIC_INIT_TEMPORAL, two levels (pyramid_policy="adaptive",
+                                 full_pyramid_backend.py:1775-1791): after a frame whose merged finest subsets
+                                 reach adaptive_acceptance of the reference-valid ones and none expired, the
+                                 next frame solves the finest level only, seeded straight from the history;
+                                 otherwise both levels.  One frame in flight at a time. */
+    float adaptive_acceptance; /* (0, 1] */
     int32_t reseed_passes;    /* seeded registration, 0..8: after the fallback, subsets still unaccepted are
                                  seeded from the warp of their nearest accepted subset (one with at least two
                                  accepted neighbours), extrapolated to their centre, and re-solved; translation

@@ -136,9 +136,10 @@ hipError_t launch_transfer_seed(const SeedArgs& a, hipStream_t s);
 hipError_t launch_median(const MedianArgs& a, hipStream_t s);
 // Temporal history (full_pyramid.wgsl:519-575, full_pyramid_backend.py:660-677).
 hipError_t launch_temporal_reset(int count, int model, float* history, uint32_t* ages, hipStream_t s);
+hipError_t launch_links_from_selected(int count, const uint8_t* selected, uint32_t* links, hipStream_t s);
 hipError_t launch_temporal_merge(int count, int model, const float* raw, const float* output, int use_filtered,
                                  uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
-                                 hipStream_t s);
+                                 hipStream_t s, uint32_t* expired_count = nullptr);
 hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s);
 // Velocity prediction (full_pyramid.wgsl:83-148): extrapolate the history from
 // the previous accepted measurement; save the measurement before the merge.

@@ -223,12 +223,15 @@ __global__ void k_temporal_reset(int count, int stride, int P, float* history, u
     ages[i] = 0xFFFFFFFFu;
 }
 
+// raw == nullptr: no measurement this frame (a level the adaptive pyramid
+// skipped), so every entry ages.  expired_count (optional) counts entries whose
+// age has just passed max_age (REPORT_HISTORY_EXPIRY, full_pyramid.wgsl:548-556).
 __global__ void k_temporal_merge(int count, int stride, int P, const float* raw, const float* output,
                                  int use_filtered, uint32_t max_age, float* history, uint32_t* ages,
-                                 uint32_t* accepted_count) {
+                                 uint32_t* accepted_count, uint32_t* expired_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const float rs = raw[(size_t)i * stride + P], fs = output[(size_t)i * stride + P];
+    const float rs = raw ? raw[(size_t)i * stride + P] : 0.0f, fs = raw ? output[(size_t)i * stride + P] : 0.0f;
     const bool raw_ok = rs == 1.0f || rs == 2.0f, final_ok = fs == 1.0f || fs == 2.0f;
     if (final_ok && (use_filtered || raw_ok)) {
         const float* src = (use_filtered ? output : raw) + (size_t)i * stride;
@@ -240,9 +243,25 @@ __global__ void k_temporal_merge(int count, int stride, int P, const float* raw,
         if (age != 0xFFFFFFFFu) {
             const uint32_t next = min(age, 0xFFFFFFFDu) + 1u;
             ages[i] = next;
-            if (next > max_age) history[(size_t)i * stride + P] = -1.0f;
+            if (next > max_age) {
+                history[(size_t)i * stride + P] = -1.0f;
+                if (expired_count && next == max_age + 1u) atomicAdd(expired_count, 1u);
+            }
         }
     }
+}
+
+// Links of a projection onto the finest grid: all four corners where the
+// subset is selected, none elsewhere.
+__global__ void k_links_from_selected(int count, const uint8_t* selected, uint32_t* links) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) links[i] = selected[i] ? 31u : 0u;
+}
+
+hipError_t launch_links_from_selected(int count, const uint8_t* selected, uint32_t* links, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    k_links_from_selected<<<(count + 255) / 256, 256, 0, s>>>(count, selected, links);
+    return hipGetLastError();
 }
 
 __global__ void k_fill_u32(int count, uint32_t value, uint32_t* p) {
@@ -258,10 +277,11 @@ hipError_t launch_temporal_reset(int count, int model, float* history, uint32_t*
 
 hipError_t launch_temporal_merge(int count, int model, const float* raw, const float* output, int use_filtered,
                                  uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
-                                 hipStream_t s) {
+                                 hipStream_t s, uint32_t* expired_count) {
     if (count <= 0) return hipSuccess;
     k_temporal_merge<<<(count + 255) / 256, 256, 0, s>>>(count, state_stride(model), param_count(model), raw, output,
-                                                         use_filtered, max_age, history, ages, accepted_count);
+                                                         use_filtered, max_age, history, ages, accepted_count,
+                                                         expired_count);
     return hipGetLastError();
 }
 

@@ -141,6 +141,9 @@ struct Slot {
     float* pinned_zncc = nullptr;
     uint32_t* counts = nullptr;         // temporal: seeded coarsest subsets, merged finest subsets
     uint32_t* pinned_counts = nullptr;
+    int levels_used = 0;                // pyramid levels of the slot's last frame
+    bool adaptive_frame = false;        // it ran under the adaptive pyramid
+    bool feedback_pending = false;      // its counts have not yet set the next frame's levels
     bool pending = false;
     bool resident = false;
     double upload_ms = 0.0;
@@ -180,6 +183,12 @@ struct ldic_engine {
     // Fine correction: history of the coarsest level.
     float* coarse_state = nullptr;
     uint32_t* coarse_ages = nullptr;
+    // Adaptive pyramid: levels of the next frame, reference-valid finest
+    // subsets, and the links of a projection onto the finest grid.
+    int adaptive_next = 0;
+    int64_t adaptive_population = 0;
+    std::vector<uint8_t> valid_host;    // finest reference_valid, partial admissions included
+    uint32_t* fine_links = nullptr;
     uint32_t* temporal_links = nullptr;
     // Recorded after each frame's history merge; the next frame's projection,
     // on whichever slot, waits for it, so streams keep the chronological chain.
@@ -401,6 +410,7 @@ int drain_ring(Slot* s) {
 // covers the finest, previous and coarse histories).
 hipError_t reset_histories(ldic_engine* e, hipStream_t st) {
     const int model = e->opt.motion_model;
+    e->adaptive_next = e->levels;
     hipError_t r = launch_temporal_reset(e->lv[0].count(), model, e->hist_state, e->hist_ages, st);
     if (r == hipSuccess) r = launch_temporal_reset(e->lv[0].count(), model, e->prev_state, e->prev_ages, st);
     if (r == hipSuccess)
@@ -427,7 +437,15 @@ SeedArgs edge_args(const ldic_engine* e, int k) {
 
 int enqueue_frame(ldic_engine* e, Slot* s) {
     hipStream_t st = s->stream;
-    const int L = e->levels;
+    const bool adaptive = e->init.kind == LDIC_INIT_TEMPORAL && e->init.adaptive_pyramid && e->levels == 2;
+    if (adaptive)
+        for (Slot* o : e->slots)
+            if (o->pending)
+                return set_error(LDIC_ERR_STATE, "the adaptive pyramid takes one frame at a time; retire the last one");
+    const int L = adaptive ? e->adaptive_next : e->levels;
+    s->levels_used = L;
+    s->adaptive_frame = adaptive;
+    s->feedback_pending = adaptive;
     if (s->ring_count == kEventRing) CHECK(drain_ring(s));
     const int ring = (s->ring_first + s->ring_count) % kEventRing;
     s->ring_count += 1;
@@ -435,7 +453,7 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
         HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, st));
     const bool temporal = e->init.kind == LDIC_INIT_TEMPORAL;
     const bool velocity = temporal && e->init.temporal_prediction == 1;
-    const bool correct = temporal && e->init.temporal_fine_correction != 0 && L == 2;
+    const bool correct = temporal && e->init.temporal_fine_correction != 0 && e->levels == 2;
     for (int k = L - 1; k >= 0; --k) {
         SlotLevel& sl = s->lv[k];
         const Level& lev = e->lv[k];
@@ -484,7 +502,9 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
             sa.track_availability = correct ? 1 : 0;
             sa.projection = 1;
             sa.src_state = velocity ? e->prediction : e->hist_state;
-            sa.permissions = e->temporal_links;
+            // The adaptive pyramid's finest-only frames project onto the finest
+            // grid (fine_projection, full_pyramid_backend.py:838-868).
+            sa.permissions = k == e->levels - 1 ? e->temporal_links : e->fine_links;
             sa.dst_seed = sl.seed;
             sa.dst_active = sl.active;
             sa.seeded_count = s->counts;
@@ -532,10 +552,12 @@ int enqueue_frame(ldic_engine* e, Slot* s) {
                                                     e->prev_state, e->prev_ages, st));
         HIP_CHECK(launch_temporal_merge(e->lv[0].count(), e->opt.motion_model, fine.state, level_output(e, fine),
                                         filtered, e->init.temporal_max_age, e->hist_state, e->hist_ages,
-                                        s->counts + 1, st));
+                                        s->counts + 1, st, adaptive ? s->counts + 3 : nullptr));
         if (correct) {
-            SlotLevel& coarse = s->lv[L - 1];
-            HIP_CHECK(launch_temporal_merge(e->lv[L - 1].count(), e->opt.motion_model, coarse.state,
+            // A frame that skipped the coarse level leaves no coarse measurement.
+            const int c = e->levels - 1;
+            SlotLevel& coarse = s->lv[c];
+            HIP_CHECK(launch_temporal_merge(e->lv[c].count(), e->opt.motion_model, L > c ? coarse.state : nullptr,
                                             level_output(e, coarse), filtered, e->init.temporal_max_age,
                                             e->coarse_state, e->coarse_ages, nullptr, st));
         }
@@ -624,6 +646,10 @@ int validate_initialization(const ldic_engine* e, const ldic_initialization* in)
         return set_error(LDIC_ERR_ARGUMENT, "temporal_fine_correction requires temporal initialization and two levels");
     if (in->divergence_patience < 0 || in->divergence_patience > 30)
         return set_error(LDIC_ERR_ARGUMENT, "divergence_patience must be 0 (off) or an integer from 1 to 30");
+    if (in->adaptive_pyramid && (in->kind != LDIC_INIT_TEMPORAL || e->opt.pyramid_levels != 2))
+        return set_error(LDIC_ERR_ARGUMENT, "adaptive pyramid requires temporal initialization and two levels");
+    if (in->adaptive_pyramid && !(in->adaptive_acceptance > 0.0f && in->adaptive_acceptance <= 1.0f))
+        return set_error(LDIC_ERR_ARGUMENT, "adaptive_acceptance must be finite and in (0, 1]");
     if (in->reseed_passes < 0 || in->reseed_passes > 8)
         return set_error(LDIC_ERR_ARGUMENT, "reseed_passes must be an integer from 0 to 8");
     if (in->reseed_passes && e->opt.motion_model == kQuadratic)
@@ -676,11 +702,25 @@ int check_slot(ldic_engine* e, int slot) {
     return LDIC_OK;
 }
 
+// The adaptive pyramid's feedback from a retired frame (full_pyramid_backend.py:
+// 1781-1791): the finest level alone next time if the merged finest subsets reach
+// the acceptance fraction of the reference-valid ones and none expired.
+void adaptive_feedback(ldic_engine* e, Slot* s) {
+    if (!s->feedback_pending) return;
+    s->feedback_pending = false;
+    const double accepted = (double)s->pinned_counts[1];
+    const bool good = e->adaptive_population > 0 &&
+                      accepted / (double)e->adaptive_population >= (double)e->init.adaptive_acceptance &&
+                      s->pinned_counts[3] == 0;
+    e->adaptive_next = good ? 1 : 2;
+}
+
 int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wall_start) {
     double t0 = now_ms();
     HIP_CHECK(hipEventSynchronize(s->ev_end));
     CHECK(drain_ring(s));
     s->pending = false;
+    adaptive_feedback(e, s);
     if (out_state) std::memcpy(out_state, s->pinned_out, sizeof(float) * (size_t)output_floats(e));
     if (t) {
         float gpu = 0.0f;
@@ -692,6 +732,8 @@ int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wal
         t->temporal_seeded = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[0] : 0;
         t->temporal_updates = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[1] : 0;
         t->velocity_predicted = e->init.kind == LDIC_INIT_TEMPORAL ? (int32_t)s->pinned_counts[2] : 0;
+        t->levels_used = s->levels_used;
+        t->temporal_expired = s->adaptive_frame ? (int32_t)s->pinned_counts[3] : 0;
         t->readback_ms = now_ms() - t0;
         t->wall_ms = now_ms() - wall_start;
     }
@@ -918,7 +960,18 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
     hipError_t sync = hipStreamSynchronize(st);
     if (rc != LDIC_OK) return rc;
     if (sync != hipSuccess) return set_error(LDIC_ERR_DEVICE, "reference preparation failed: %s", hipGetErrorString(sync));
-    if (out_valid) HIP_CHECK(hipMemcpy(out_valid, L0.valid, (size_t)L0.count(), hipMemcpyDeviceToHost));
+    std::vector<uint8_t>& valid = e->valid_host;
+    valid.assign((size_t)L0.count(), 0);
+    HIP_CHECK(hipMemcpy(valid.data(), L0.valid, valid.size(), hipMemcpyDeviceToHost));
+    if (out_valid) std::memcpy(out_valid, valid.data(), valid.size());
+    e->adaptive_population = 0;
+    for (uint8_t v : valid) e->adaptive_population += v != 0;
+    e->fine_links = nullptr;
+    if (L0.roi) {
+        CHECK(e->ref_arena.alloc(&e->fine_links, (size_t)L0.count()));
+        HIP_CHECK(launch_links_from_selected(L0.count(), L0.selected, e->fine_links, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
     CHECK(ensure_slots(e, 1));
     e->ready = true;
     return LDIC_OK;
@@ -1032,6 +1085,8 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
     if (!reference || !target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "reference, target and out_state are required");
     if (has_partial(e))
         return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support partial-support ROI subsets");
+    if (e->init.adaptive_pyramid)
+        return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support the adaptive pyramid");
     const Level& L0 = e->lv[0];
     if (reference_stride < L0.width || target_stride < L0.width)
         return set_error(LDIC_ERR_ARGUMENT, "row strides are smaller than the width");
@@ -1060,6 +1115,8 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
     if (!device_reference || !device_target) return set_error(LDIC_ERR_ARGUMENT, "device pointers are required");
     if (has_partial(e))
         return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support partial-support ROI subsets");
+    if (e->init.adaptive_pyramid)
+        return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support the adaptive pyramid");
     Slot* s = e->slots[slot];
     if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
     HIP_CHECK(hipSetDevice(e->device));
@@ -1245,6 +1302,13 @@ int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_
     HIP_CHECK(launch_scatter_rows(count, P * P, L.partial_list, inv, L.inverse, e->prep_stream));
     HIP_CHECK(hipStreamSynchronize(e->prep_stream));
     L.partial_count = count;
+    if (level == 0 && e->valid_host.size() == (size_t)L.count())
+        for (int i = 0; i < count; ++i) {
+            // A partial subset is reference-valid when its support is admitted.
+            const uint8_t admitted = metadata_rows[(size_t)i * M + 1] == -4.0f ? 1 : 0;
+            e->adaptive_population += (int64_t)admitted - (int64_t)(e->valid_host[indices[i]] != 0);
+            e->valid_host[indices[i]] = admitted;
+        }
     return LDIC_OK;
 }
 
@@ -1307,6 +1371,7 @@ int ldic_export_level_state(ldic_engine* e, int32_t slot, int32_t level, float* 
         CHECK(drain_ring(s));
         s->drained_solve_ms = 0.0;
         s->drained_frames = 0;
+        adaptive_feedback(e, s);
     }
     s->pending = false;
     size_t n = (size_t)e->lv[level].count() * state_stride(e->opt.motion_model);

@@ -78,6 +78,8 @@ class Timing(ctypes.Structure):
         ("temporal_seeded", ctypes.c_int32),
         ("temporal_updates", ctypes.c_int32),
         ("velocity_predicted", ctypes.c_int32),
+        ("levels_used", ctypes.c_int32),
+        ("temporal_expired", ctypes.c_int32),
     ]
 
 
@@ -92,6 +94,8 @@ class Initialization(ctypes.Structure):
         ("temporal_fine_correction", ctypes.c_int32),
         ("divergence_patience", ctypes.c_int32),
         ("divergence_min_iterations", ctypes.c_int32),
+        ("adaptive_pyramid", ctypes.c_int32),
+        ("adaptive_acceptance", ctypes.c_float),
         ("reseed_passes", ctypes.c_int32),
     ]
 

@@ -76,9 +76,12 @@ class ICGNWGPUFullPyramid(_NativeEngine):
     the coarsest level is seeded by an exhaustive integer ZNCC search covering
     ``seed_search_radius`` pixels of the finest level.  ``divergence_patience``
     stops unconverged finest-level solves whose change stopped decreasing
-    (icgn.wgsl:103-111).  ``reseed_passes`` (an addition, 0-8) re-solves finest-level
+    (icgn.wgsl:103-111).  ``pyramid_policy="adaptive"`` (temporal, two levels) solves only
+    the finest level after a frame whose merged subsets reach ``adaptive_acceptance`` of the
+    reference-valid ones with no history expiry, and both levels otherwise
+    (full_pyramid_backend.py:1775-1791).  ``reseed_passes`` (an addition, 0-8) re-solves finest-level
     subsets still unaccepted from the warp of their nearest accepted subset.  Not built: the
-    Fourier and CPM-HOG matching initialisers and the adaptive pyramid policy.
+    Fourier and CPM-HOG matching initialisers.
     """
 
     def __init__(self, *, pyramid_levels: int = DEFAULT_FULL_PYRAMID_LEVELS,
@@ -93,13 +96,17 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                  temporal_max_age: int | None = None, temporal_fine_correction: bool = False,
                  temporal_prediction: str = "none", pyramid_policy: str = "fixed", seed_search_radius: int = 32,
                  divergence_patience: int | None = None, divergence_min_iterations: int = 4,
-                 reseed_passes: int = 0, **unsupported) -> None:
+                 reseed_passes: int = 0, adaptive_acceptance: float = 0.95, **unsupported) -> None:
         if unsupported:
             raise ValueError(f"unsupported options for the HIP engine: {sorted(unsupported)}")
         if not isinstance(pyramid_policy, str) or pyramid_policy not in ("fixed", "adaptive"):
             raise ValueError("pyramid_policy must be 'fixed' or 'adaptive'")
-        if pyramid_policy != "fixed":
-            raise ValueError("adaptive pyramid requires temporal initialization, which is not built")
+        if pyramid_policy == "adaptive" and (initialization != "temporal" or pyramid_levels != 2):
+            raise ValueError("adaptive pyramid requires temporal initialization and two levels")
+        if (isinstance(adaptive_acceptance, bool) or not isinstance(adaptive_acceptance, (int, float))
+                or not np.isfinite(adaptive_acceptance) or not 0 < adaptive_acceptance <= 1):
+            raise ValueError("adaptive_acceptance must be finite and in (0, 1]")
+        self.adaptive_acceptance = float(adaptive_acceptance)
         if not isinstance(initialization, str) or initialization not in (
                 "zero", "zncc", "temporal", "fourier", "wgpu-matching"):
             raise ValueError("initialization must be 'zero', 'zncc', 'temporal', 'fourier' or 'wgpu-matching'")
@@ -165,7 +172,8 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                         temporal_fine_correction=int(temporal_fine_correction),
                         divergence_patience=self.divergence_patience or 0,
                         divergence_min_iterations=self.divergence_min_iterations,
-                        reseed_passes=reseed_passes)
+                        reseed_passes=reseed_passes, adaptive_pyramid=int(pyramid_policy == "adaptive"),
+                        adaptive_acceptance=self.adaptive_acceptance)
         if initialization == "zncc":
             self._set_initialization(_native.INIT_ZNCC, self.coarse_search_radius, **sequence)
         elif initialization == "temporal":
@@ -207,6 +215,9 @@ class ICGNWGPUFullPyramid(_NativeEngine):
                                         coarseSearchRadius=self.coarse_search_radius)
         if self.reseed_passes:
             self._pyramid_config.update(reseedPasses=self.reseed_passes)
+        if self.pyramid_policy == "adaptive":
+            self._pyramid_config.update(pyramidPolicy="adaptive", adaptiveAcceptance=self.adaptive_acceptance,
+                                        adaptiveFeedback="ordered-single-slot")
         if self.divergence_patience is not None:
             self._pyramid_config.update(divergencePatience=self.divergence_patience,
                                         divergenceMinIterations=self.divergence_min_iterations,
@@ -251,17 +262,13 @@ class ICGNWGPUFullPyramid(_NativeEngine):
         if self.roi is not None and self.roi.empty:
             return self._empty_roi_result(started, self._config)
         prep_ms = (time.perf_counter() - started) * 1e3
-        result = self._run(frame, started, prep_ms, self._config)
-        result.timing["pyramidLevels"] = float(self.pyramid_levels)
-        return result
+        return self._levels_used(self._run(frame, started, prep_ms, self._config))
 
     @_exclusive
     def register_pair(self, reference: ImageInput, target: ImageInput) -> RegistrationResult:
         """Consecutive-pair mode: the reference pyramid and every level's preparation run on the
         GPU in the same queued sequence as the target's coarse-to-fine solve."""
-        result = self._register_pair(reference, target, self._config)
-        result.timing["pyramidLevels"] = float(self.pyramid_levels)
-        return result
+        return self._levels_used(self._register_pair(reference, target, self._config))
 
     @_exclusive_stream
     def register_stream(self, targets: Iterable[ImageInput], *,
@@ -271,9 +278,21 @@ class ICGNWGPUFullPyramid(_NativeEngine):
             raise ICGNError("Call set_reference() before register_stream().")
         if isinstance(pipeline_depth, bool):
             raise ValueError("pipeline_depth must be a positive integer")
+        if self.pyramid_policy == "adaptive":
+            # Each frame's level count depends on the previous frame's result: retire
+            # every frame before submitting the next (full_pyramid_backend.py:1865-1870).
+            operator.index(pipeline_depth)
+            pipeline_depth = 1
         for result in self._stream(targets, pipeline_depth, self._config):
-            result.timing["pyramidLevels"] = float(self.pyramid_levels)
-            yield result
+            yield self._levels_used(result)
+
+    def _levels_used(self, result: RegistrationResult) -> RegistrationResult:
+        """Levels the frame solved, as timing (and, for the adaptive pyramid, config)."""
+        used = int(result.timing.pop("levelsUsed", 0)) or self.pyramid_levels
+        result.timing["pyramidLevels"] = float(used)
+        if self.pyramid_policy == "adaptive":
+            result.config["pyramidLevelsUsed"] = used
+        return result
 
     def register_many(self, targets: Iterable[ImageInput], *,
                       pipeline_depth: int = DEFAULT_IN_FLIGHT_FRAMES) -> list[RegistrationResult]:

@@ -206,7 +206,10 @@ class _NativeEngine:
                 continue
             plans[k], admitted = self._upload_partial(k, plan)
             if k == 0:
-                valid[admitted // shapes[0][2], admitted % shapes[0][2]] = 1
+                # reference_valid of a partial subset is its admission (wgpu_backend.py:2428-2433).
+                part = np.asarray(plan.partial_indices, np.int64)
+                valid.ravel()[part] = 0
+                valid.ravel()[admitted] = 1
         build_ms = (time.perf_counter() - started) * 1e3
         self.width, self.height = width, height
         self.grid_width, self.grid_height = shapes[0][2], shapes[0][3]
@@ -321,6 +324,9 @@ class _NativeEngine:
             "temporalAcceptedUpdates": float(t.temporal_updates),
             **({"velocityPredictedSubsets": float(t.velocity_predicted)}
                if getattr(self, "temporal_prediction", "none") == "velocity" else {}),
+            **({"temporalExpiredSubsets": float(t.temporal_expired)}
+               if getattr(self, "pyramid_policy", "fixed") == "adaptive" else {}),
+            **({"levelsUsed": float(t.levels_used)} if hasattr(self, "pyramid_policy") else {}),
         }
 
     def _run(self, frame: np.ndarray, started: float, prep_ms: float, config: dict) -> RegistrationResult:
@@ -430,12 +436,14 @@ class _NativeEngine:
                             fallback_radius: int = 0, temporal_max_age: int | None = None,
                             temporal_prediction: int = 0, temporal_fine_correction: int = 0,
                             divergence_patience: int = 0, divergence_min_iterations: int = 4,
+                            adaptive_pyramid: int = 0, adaptive_acceptance: float = 0.95,
                             reseed_passes: int = 0) -> None:
         init = _native.Initialization(kind, int(search_radius), -2.0 if fallback_zncc is None else float(fallback_zncc),
                                       int(fallback_radius),
                                       0xFFFFFFFE if temporal_max_age is None else int(temporal_max_age),
                                       int(temporal_prediction), int(temporal_fine_correction),
-                                      int(divergence_patience), int(divergence_min_iterations), int(reseed_passes))
+                                      int(divergence_patience), int(divergence_min_iterations),
+                                      int(adaptive_pyramid), float(adaptive_acceptance), int(reseed_passes))
         _native.check(self._lib.ldic_set_initialization(self._handle, ctypes.byref(init)))
         self._initialization = init
 

@@ -164,18 +164,23 @@ def transfer_seed(cfg: SeedCfg, source_state, permissions=None, return_weight=Fa
 NEVER = 0xFFFFFFFF
 
 
-def temporal_merge(history, ages, raw, output, use_filtered, max_age):
+def temporal_merge(history, ages, raw, output, use_filtered, max_age, return_expired=False):
     """update_temporal_state (full_pyramid.wgsl:519-565) on float32 (count, stride) arrays and uint64
-    ages; returns the number of merged subsets."""
+    ages; returns the number of merged subsets (and, with ``return_expired``, of entries whose age
+    has just passed ``max_age``).  ``raw is None``: no measurement, every entry ages."""
     P = 12 if history.shape[1] == 16 else 6
-    rs, fs = raw[:, P], output[:, P]
-    upd = ((fs == 1.0) | (fs == 2.0)) & (bool(use_filtered) | (rs == 1.0) | (rs == 2.0))
-    history[upd] = (output if use_filtered else raw)[upd]
+    if raw is None:
+        upd = np.zeros(len(ages), bool)
+    else:
+        rs, fs = raw[:, P], output[:, P]
+        upd = ((fs == 1.0) | (fs == 2.0)) & (bool(use_filtered) | (rs == 1.0) | (rs == 2.0))
+        history[upd] = (output if use_filtered else raw)[upd]
     ages[upd] = 0
     aging = ~upd & (ages != NEVER)
     ages[aging] = np.minimum(ages[aging], 0xFFFFFFFD) + 1
     history[aging & (ages > max_age), P] = -1.0
-    return int(upd.sum())
+    expired = int(np.sum(aging & (ages == max_age + 1)))
+    return (int(upd.sum()), expired) if return_expired else int(upd.sum())
 
 
 NO_SOURCE = 0xFFFFFFFF
@@ -335,6 +340,9 @@ class Level:
         self.roi = roi
         self.roi_words = None
         self.seed_links = None
+        # reference_valid (wgpu_backend.py:2428-2433): numerically admissible subsets, before
+        # the ROI marks the unselected ones -3.
+        self.reference_valid = self.metadata[:, 1] >= 0.5
         if roi is not None and roi.get("effective", False):
             self.metadata[~roi["selected"].ravel(), 1] = -3.0
             self.roi_words = np.zeros((self.grid_width * self.grid_height, 4), dtype=np.float32)
@@ -357,6 +365,8 @@ class Level:
                               sigma=opts["gaussian_sigma"])
             self.partial = (np.asarray(roi["partial_indices"], np.uint32), np.asarray(roi["support_words"], np.uint32),
                             np.asarray(roi["support_boxes"], np.float32))
+            idx = self.partial[0].astype(np.int64)
+            self.reference_valid[idx] = meta[idx, 1] == -4.0
 
 
 class Pipeline:
@@ -373,7 +383,8 @@ class Pipeline:
                  subset_stride=None, spatial_weighting="uniform", gaussian_sigma=None,
                  outlier_policy="replace", median_filter_size=3, median_filter_scope="rejected",
                  lanes=16, initialization="zero", temporal_max_age=None, temporal_prediction="none",
-                 temporal_fine_correction=False, divergence_patience=None, divergence_min_iterations=4):
+                 temporal_fine_correction=False, divergence_patience=None, divergence_min_iterations=4,
+                 pyramid_policy="fixed", adaptive_acceptance=0.95):
         L = pyramid_levels
         self.levels_count = L
         self.iterations = tuple(iterations_per_level or DEFAULT_ITERATIONS[L])
@@ -400,6 +411,13 @@ class Pipeline:
         self.patience = (0 if divergence_patience is None else int(divergence_patience),
                          int(divergence_min_iterations))
         self.last_predicted = 0
+        self.adaptive = pyramid_policy == "adaptive" and self.temporal and L == 2
+        self.adaptive_acceptance = float(adaptive_acceptance)
+        self.next_levels = L
+        self.last_levels = L
+        self.last_expired = 0
+        self.fine_links = None
+        self.population = 0
 
     def set_reference(self, image, rois=None, temporal_links=None):
         images = [np.ascontiguousarray(image, dtype=np.uint8)]
@@ -407,11 +425,17 @@ class Pipeline:
             images.append(reduce(images[-1]))
         self.levels = [Level(img, self.opts, None if rois is None else rois[k]) for k, img in enumerate(images)]
         self.temporal_links = temporal_links
+        fine = self.levels[0]
+        self.population = int(np.sum(fine.reference_valid))
+        self.fine_links = None
+        if fine.roi is not None and fine.roi.get("effective", False):
+            self.fine_links = np.where(np.asarray(fine.roi["selected"]).ravel(), 31, 0).astype(np.uint32)
         self.reset_temporal_state()
         return self
 
     def reset_temporal_state(self):
         """reset_temporal_state (full_pyramid_backend.py:660-677)."""
+        self.next_levels = self.levels_count
         fine = self.levels[0]
         stride = 16 if self.opts["motion_model"] == "quadratic" else 8
         self.history = np.zeros((fine.grid_width * fine.grid_height, stride), np.float32)
@@ -447,18 +471,24 @@ class Pipeline:
         return median(cfg, state, lv.metadata, lv.roi_words)
 
     def register(self, target, return_levels=False):
+        # The adaptive pyramid solves the finest level alone after a good frame
+        # (full_pyramid_backend.py:1436-1569, 1775-1791).
+        Lu = self.next_levels if self.adaptive else self.levels_count
+        self.last_levels = Lu
         targets = [np.ascontiguousarray(target, dtype=np.uint8)]
-        for _ in range(1, self.levels_count):
+        for _ in range(1, Lu):
             targets.append(reduce(targets[-1]))
         outputs = [None] * self.levels_count
         raws = [None] * self.levels_count
         o = self.opts
-        for k in range(self.levels_count - 1, -1, -1):
+        for k in range(Lu - 1, -1, -1):
             lv = self.levels[k]
             coeff = prefilter(targets[k], self.fir) if o["interpolation"] == "bspline" else None
             cfg = self._solve_cfg(k)
-            if k == self.levels_count - 1 and self.temporal:
-                # project_temporal_seed (full_pyramid.wgsl:437-449): finest history -> coarsest grid.
+            if k == Lu - 1 and self.temporal:
+                # project_temporal_seed (full_pyramid.wgsl:437-449): finest history -> the
+                # coarsest grid solved this frame (the finest one itself in an adaptive
+                # finest-only frame, with links "selected or not").
                 fine = self.levels[0]
                 scfg = SeedCfg(fine.grid_width, fine.grid_height, lv.grid_width, lv.grid_width * lv.grid_height,
                                1.0, float(1 << k), float(o["subset_size"]), float(o["subset_stride"]),
@@ -467,16 +497,17 @@ class Pipeline:
                 if self.velocity:
                     source, self.last_predicted = temporal_predict(self.history, self.previous, self.ages,
                                                                    self.previous_ages)
-                seed, active, weight = transfer_seed(scfg, source, self.temporal_links, return_weight=True)
-                links = np.full(lv.grid_width * lv.grid_height, 31, np.uint32) if self.temporal_links is None \
-                    else np.asarray(self.temporal_links, np.uint32).ravel()
+                plinks = self.temporal_links if k == self.levels_count - 1 else self.fine_links
+                seed, active, weight = transfer_seed(scfg, source, plinks, return_weight=True)
+                links = np.full(lv.grid_width * lv.grid_height, 31, np.uint32) if plinks is None \
+                    else np.asarray(plinks, np.uint32).ravel()
                 counted = (weight > 0) & ((links & 16) != 0)
                 seeded = int(np.sum(counted))
                 if self.correct:
                     # project_temporal_seed marks every seed it does not count.
                     seed[~counted, 12 if o["motion_model"] == "quadratic" else 6] = -1.0
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k], seed, active)
-            elif k == self.levels_count - 1:
+            elif k == Lu - 1:
                 seed = active = None
                 state = solve(cfg, lv.reference, lv.metadata, lv.inverse, coeff, targets[k])
             else:
@@ -499,11 +530,17 @@ class Pipeline:
             if self.velocity:
                 temporal_save_previous(self.history, self.ages, raws[0], outputs[0], self.previous,
                                        self.previous_ages)
-            merged = temporal_merge(self.history, self.ages, raws[0], outputs[0], filtered, self.temporal_max_age)
+            merged, expired = temporal_merge(self.history, self.ages, raws[0], outputs[0], filtered,
+                                             self.temporal_max_age, return_expired=True)
             if self.correct:
+                # A frame that skipped the coarse level leaves no coarse measurement.
                 temporal_merge(self.coarse_history, self.coarse_ages, raws[-1], outputs[-1], filtered,
                                self.temporal_max_age)
             self.last_counts = (seeded, merged)
+            self.last_expired = expired if self.adaptive else 0
+            if self.adaptive:
+                good = self.population > 0 and merged / self.population >= self.adaptive_acceptance and not expired
+                self.next_levels = 1 if good else 2
         stride = 16 if o["motion_model"] == "quadratic" else 8
         fine = outputs[0].reshape(self.levels[0].grid_height, self.levels[0].grid_width, stride)
         return (fine, outputs) if return_levels else fine

@@ -121,3 +121,13 @@ def test_reseed_arguments_are_rejected_before_device_creation():
             ICGNWGPUFullPyramid(reseed_passes=value)
     with pytest.raises(ValueError, match="reseed_passes"):
         ICGNWGPUFullPyramid(reseed_passes=2, motion_model="quadratic")
+
+
+def test_adaptive_pyramid_arguments_are_rejected_before_device_creation():
+    """test_latency_options.py:21-35."""
+    for options in ({"pyramid_policy": "unknown"}, {"pyramid_policy": "adaptive"},
+                    {"pyramid_policy": "adaptive", "initialization": "temporal", "pyramid_levels": 1},
+                    {"adaptive_acceptance": float("nan")}, {"adaptive_acceptance": 0},
+                    {"adaptive_acceptance": 1.01}, {"adaptive_acceptance": True}):
+        with pytest.raises(ValueError):
+            ICGNWGPUFullPyramid(**options)

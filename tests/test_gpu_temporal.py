@@ -372,3 +372,44 @@ def test_abort_preserves_early_convergence_and_matches_oracle():
     stats = compare_fields(got, want, 6)
     _say("vs oracle", stats, "iterations equal", float(np.mean(got[..., 7] == want[..., 7])))
     assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
+
+
+def test_adaptive_pyramid_switches_resets_and_is_identical_at_all_depths():
+    """test_latency_options.py:107-144, and the level choices and fields against the oracle."""
+    from helpers import load_gray
+    from live_dic import ICGNWGPUFullPyramid
+    from test_oracle_partial import fractional_shift
+
+    _say("adaptive pyramid, 256^2 sequence")
+    src = load_gray("frame_000.png")[:256, :256].copy()
+    targets = [fractional_shift(src, 0.2 * i, -0.1 * i) for i in range(4)]
+    sequence = [*targets, np.zeros_like(src), targets[-1], targets[-1]]
+    options = dict(initialization="temporal", temporal_prediction="velocity", pyramid_policy="adaptive",
+                   adaptive_acceptance=0.7, temporal_max_age=0, outlier_policy="median",
+                   temporal_fine_correction=True)
+    pipe = oracle.Pipeline(pyramid_levels=2, **options)
+    pipe.set_reference(src)
+    with ICGNWGPUFullPyramid(**options) as eng:
+        eng.set_reference(src)
+        expected = None
+        for depth in (1, 2, 3):
+            eng.reset_temporal_state()
+            outputs = list(eng.register_stream(sequence, pipeline_depth=depth))
+            levels = [r.timing["pyramidLevels"] for r in outputs]
+            _say("depth", depth, "levels", levels, "expired", [r.timing["temporalExpiredSubsets"] for r in outputs])
+            assert levels[:2] == [2, 1] and levels[5] == 2
+            assert outputs[4].timing["temporalExpiredSubsets"] > 0
+            assert all(r.config["pyramidLevelsUsed"] == lv for r, lv in zip(outputs, levels))
+            if expected is None:
+                expected = outputs
+                for r, t in zip(outputs, sequence):
+                    want = pipe.register(t)
+                    assert pipe.last_levels == r.timing["pyramidLevels"]
+                    got = np.concatenate([r.parameters, r.status[..., None], r.iterations[..., None]], -1)
+                    stats = compare_fields(got, want, 6)
+                    assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999, stats
+            for old, new in zip(expected, outputs, strict=True):
+                _same(old, new)
+                assert old.timing["pyramidLevels"] == new.timing["pyramidLevels"]
+        eng.set_reference(src)
+        assert eng.register(targets[0]).timing["pyramidLevels"] == 2

@@ -202,3 +202,30 @@ def test_latency_translation_goldens(case, frames, roi):
         assert np.mean(out[..., 7] == gold[..., 7]) >= 0.98
         if frame == 2:
             assert pipe.last_predicted > 0
+
+
+def test_adaptive_pyramid_rule_on_a_sequence():
+    """The adaptive feedback (full_pyramid_backend.py:1775-1791) on the oracle: a good frame
+    switches the next one to the finest level alone, a corrupted frame (whose expired
+    history is counted) switches back, and a reset restarts with both levels."""
+    from helpers import load_gray
+    from test_oracle_partial import fractional_shift
+
+    src = load_gray("frame_000.png")[:256, :256]
+    targets = [fractional_shift(src, 0.2 * i, -0.1 * i) for i in range(4)]
+    sequence = [*targets, np.zeros_like(src), targets[-1], targets[-1]]
+    pipe = oracle.Pipeline(pyramid_levels=2, initialization="temporal", temporal_prediction="velocity",
+                           pyramid_policy="adaptive", adaptive_acceptance=0.7, temporal_max_age=0,
+                           outlier_policy="median", temporal_fine_correction=True)
+    pipe.set_reference(src)
+    levels, expired = [], []
+    for t in sequence:
+        pipe.register(t)
+        levels.append(pipe.last_levels)
+        expired.append(pipe.last_expired)
+    print(levels, expired)
+    assert levels[:2] == [2, 1] and levels[5] == 2
+    assert expired[4] > 0
+    pipe.reset_temporal_state()
+    pipe.register(targets[0])
+    assert pipe.last_levels == 2
