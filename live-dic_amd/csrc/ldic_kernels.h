@@ -112,8 +112,8 @@ struct PrepArgs {
 };
 
 // Pick the register-cached solver variant for a subset width.
-int solver_pixels_per_lane(int S);
-size_t solver_lds_bytes(int S, int* tile_cap);
+int solver_pixels_per_lane(int S, int model);
+size_t solver_lds_bytes(int S, int model, int* tile_cap);
 
 hipError_t launch_reduce(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
 hipError_t launch_prefilter(const uint8_t* src, int w, int h, const float* fir31, float* coeff, hipStream_t s);

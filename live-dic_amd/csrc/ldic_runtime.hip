@@ -1013,7 +1013,7 @@ int ldic_create(const ldic_options* options, ldic_engine** out_engine) {
     he = hipMemcpy(e->fir, taps, sizeof taps, hipMemcpyHostToDevice);
     if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "FIR upload failed: %s", hipGetErrorString(he)));
     gaussian_taps(e->gauss7);
-    e->solve_lds = solver_lds_bytes(options->subset_size, &e->tile_cap);
+    e->solve_lds = solver_lds_bytes(options->subset_size, options->motion_model, &e->tile_cap);
     *out_engine = e;
     return LDIC_OK;
 }
@@ -1631,7 +1631,7 @@ int batch_build(ldic_batch* b) {
         for (size_t p = 0; p < b->engines.size(); ++p) {
             ldic_engine* e = b->engines[p];
             if (e->levels <= j) continue;
-            const int ppl = solver_pixels_per_lane(e->opt.subset_size);
+            const int ppl = solver_pixels_per_lane(e->opt.subset_size, e->opt.motion_model);
             ldic_batch::Group* g = nullptr;
             for (auto& cand : groups) {
                 const BatchLaunch& l = cand.launch;

@@ -382,6 +382,13 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     const int ox = gx * D, oy = gy * D;
     const float cx0 = cx - 0.5f, cy0 = cy - 0.5f;
 
+    // Uncached sweeps: each lane's first pixel and the per-step increments.
+    const int step_row = kWave / S, step_col = kWave - step_row * S;
+    const int step_off = step_row * a.width + step_col;
+    const int row0 = lane / S, col0 = lane - row0 * S;
+    const int off0 = (oy + row0) * a.width + ox + col0;
+    const float lx0 = (float)col0 - he, ly0 = (float)row0 - he;
+
     // Register-resident reference tile.
     float rf[NP], rgx[NP], rgy[NP], plx[NP], ply[NP], pw[WEIGHTED ? NP : 1];
     if (PPL > 0 && status == 0.0f) {
@@ -526,13 +533,26 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                                    k == full ? tail : 1.0f);
                 }
             } else {
+                // Pixel p = lane + 64 j: the column, row, plane offset and local
+                // coordinates advance by (64 mod S, 64 div S) per step, with one
+                // carry, instead of a division per pixel.
+                int col = col0, off = off0;
+                float lx = lx0, ly = ly0;
                 for (int p = lane; p < SS; p += kWave) {
-                    int row = p / S, col = p - row * S;
-                    float4 r = a.reference[(size_t)(oy + row) * a.width + ox + col];
-                    float lx = (float)col - he, ly = (float)row - he;
+                    const float4 r = a.reference[off];
                     float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
                     if (partial && !((support[p >> 5] >> (p & 31)) & 1u)) iw = 0.0f;
                     accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw, 1.0f);
+                    col += step_col;
+                    off += step_off;
+                    lx += (float)step_col;
+                    ly += (float)step_row;
+                    if (col >= S) {
+                        col -= S;
+                        off += a.width - S;
+                        lx -= (float)S;
+                        ly += 1.0f;
+                    }
                 }
             }
         };
@@ -687,30 +707,24 @@ __global__ __launch_bounds__(256) void k_solve_batched(const SolveArgs* __restri
     }
 }
 
-int solver_pixels_per_lane(int S) {
-    int need = (S * S + kWave - 1) / kWave;
-    if (need <= 4) return 4;
-    if (need <= 8) return 8;
-    if (need <= 16) return 16;
-    if (need <= 27) return 27;
-    return 0;
+// Register-cached reference pixels per lane (0: the reference is re-read from
+// L1/L2 every iteration).  Measured on MI355X at 4K, the uncached sweep is the
+// faster one from subset 21 on for translation and affine (22-32 % at 25-41) and
+// at every size for the quadratic model (1.2-2.8x), where 12 more sums leave no
+// room for a register cache at useful occupancy; only translation/affine
+// subsets of at most 16x16 keep their 4 pixels per lane in registers.
+int solver_pixels_per_lane(int S, int model) {
+    if (model == kQuadratic) return 0;
+    return S * S <= 4 * kWave ? 4 : 0;
 }
 
 // Tile pitch of each register-cached variant (0: pitch = staged width).
-static int solver_pitch(int ppl) {
-    switch (ppl) {
-        case 4: return 32;
-        case 8: return 48;
-        case 16: return 48;
-        case 27: return 64;
-        default: return 0;
-    }
-}
+static int solver_pitch(int ppl) { return ppl == 4 ? 32 : 0; }
 
-size_t solver_lds_bytes(int S, int* tile_cap) {
+size_t solver_lds_bytes(int S, int model, int* tile_cap) {
     // Footprint of a moderately strained subset plus the interpolation halo
     // and the re-staging margin; larger footprints fall back to L2 gathers.
-    int pitch = solver_pitch(solver_pixels_per_lane(S));
+    int pitch = solver_pitch(solver_pixels_per_lane(S, model));
     int cap;
     if (pitch > 0) {
         cap = pitch * (S + 17);
@@ -727,14 +741,11 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     int N = a.partial_list ? a.partial_count : a.grid_w * a.grid_h;
     if (N <= 0) return hipSuccess;
     int cap = 0;
-    size_t lds = solver_lds_bytes(a.S, &cap);
+    size_t lds = solver_lds_bytes(a.S, MODEL, &cap);
     if (cap != a.tile_cap) return hipErrorInvalidValue;
     dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
     switch (ppl) {
         case 4: k_solve<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, 256, lds, s>>>(a); break;
-        case 8: k_solve<MODEL, INTERP, 8, WEIGHTED, 48><<<grid, 256, lds, s>>>(a); break;
-        case 16: k_solve<MODEL, INTERP, 16, WEIGHTED, 48><<<grid, 256, lds, s>>>(a); break;
-        case 27: k_solve<MODEL, INTERP, 27, WEIGHTED, 64><<<grid, 256, lds, s>>>(a); break;
         default: k_solve<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, 256, lds, s>>>(a); break;
     }
     return hipGetLastError();
@@ -742,7 +753,7 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
 
 template <int MODEL, int INTERP>
 static hipError_t launch_w(int weighted, const SolveArgs& a, hipStream_t s) {
-    int ppl = solver_pixels_per_lane(a.S);
+    int ppl = solver_pixels_per_lane(a.S, MODEL);
     return weighted ? launch_ppl<MODEL, INTERP, true>(ppl, a, s) : launch_ppl<MODEL, INTERP, false>(ppl, a, s);
 }
 
@@ -762,9 +773,6 @@ static hipError_t launch_batched_ppl(const BatchLaunch& b, hipStream_t s) {
     dim3 grid(b.grid), block(256);
     switch (b.ppl) {
         case 4: k_solve_batched<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
-        case 8: k_solve_batched<MODEL, INTERP, 8, WEIGHTED, 48><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
-        case 16: k_solve_batched<MODEL, INTERP, 16, WEIGHTED, 48><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
-        case 27: k_solve_batched<MODEL, INTERP, 27, WEIGHTED, 64><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
         default: k_solve_batched<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
     }
     return hipGetLastError();
