@@ -33,7 +33,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes of one fine-level k_solve launch of this workload, measured with rocprofv3
 # FETCH_SIZE/WRITE_SIZE and scaled by a 256 MiB copy calibration (reads 2.0x FETCH_SIZE,
 # writes 1.0x WRITE_SIZE): profiles/k_solve_affine_4k_s31_d8_pmc.md.
-MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 102_686 + 3_986) * 1024
+MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 101_830 + 4_073) * 1024
 SUBSET, STRIDE = 31, 8
 WIDTH, HEIGHT = 3840, 2160
 FRAMES = 60
