@@ -414,53 +414,121 @@ __device__ void median_policy(const MedianArgs& a, int i) {
     for (int k = 0; k < stride; ++k) out[k] = f[k];
 }
 
-__device__ void uod_policy(const MedianArgs& a, int i) {
+// ---------------------------------------------------------------------------
+// Register-only forms of the 3x3 passes.  The neighbour values of one
+// component go into a fixed array padded with +inf, an odd-even transposition
+// network sorts it, and the median of the first `count` entries is picked by
+// selects, so no array is indexed at run time (the generic form above, used
+// for wider windows, keeps its list in scratch memory).  Order statistics are
+// exact, so the results equal the insertion-sorted medians bit for bit.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void sort_network(float (&v)[N]) {
+#pragma unroll
+    for (int pass = 0; pass < N; ++pass)
+#pragma unroll
+        for (int j = pass & 1; j + 1 < N; j += 2) {
+            const float lo = fminf(v[j], v[j + 1]), hi = fmaxf(v[j], v[j + 1]);
+            v[j] = lo;
+            v[j + 1] = hi;
+        }
+}
+
+// Median of the `count` smallest entries of a sorted array (count >= 1).
+template <int N>
+__device__ __forceinline__ float pick_median(const float (&v)[N], int count) {
+    float lo = v[0], hi = v[0];
+#pragma unroll
+    for (int c = 1; c <= N; ++c)
+        if (c == count) {
+            lo = v[(c - 1) / 2];
+            hi = v[c / 2];
+        }
+    return (count & 1) ? hi : 0.5f * (lo + hi);
+}
+
+__device__ constexpr int component_order(int k) { return k == 1 ? 3 : k == 2 ? 1 : k == 3 ? 2 : k; }
+
+struct Neighbours3 {
+    const float* state[8];
+    bool ok[8];
+    float ox[8], oy[8];
+    int count;
+};
+
+// The eight neighbours in the reference's order (dy outer, dx inner), their
+// validity (in the grid, linked under an ROI, accepted) and recentring offsets.
+__device__ __forceinline__ void neighbours3(const MedianArgs& a, int i, bool links, Neighbours3& n) {
+    const int stride = a.model == kQuadratic ? 16 : 8;
+    const int px = i % a.grid_w, py = i / a.grid_w, gh = a.grid_count / a.grid_w;
+    const uint32_t perm = links ? (uint32_t)a.roi_words[(size_t)i * 4] : 0x1FFu;
+    n.count = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const int e = d < 4 ? d : d + 1;
+        const int dx = e % 3 - 1, dy = e / 3 - 1;
+        const int nx = px + dx, ny = py + dy;
+        const bool inside = nx >= 0 && ny >= 0 && nx < a.grid_w && ny < gh && ((perm >> e) & 1u);
+        const float* ns = a.state + (size_t)(inside ? ny * a.grid_w + nx : i) * stride;
+        n.state[d] = ns;
+        n.ok[d] = inside && (a.model == kQuadratic ? ns[12] : ns[6]) >= 0.5f;
+        n.ox[d] = -(float)a.D * dx;
+        n.oy[d] = -(float)a.D * dy;
+        n.count += n.ok[d] ? 1 : 0;
+    }
+}
+
+__device__ void uod_policy3(const MedianArgs& a, int i) {
     const int model = a.model;
     const int stride = model == kQuadratic ? 16 : 8;
-    const int ncomp = model == kQuadratic ? 12 : 6;
     const float* own = a.state + (size_t)i * stride;
     float* out = a.filtered + (size_t)i * stride;
     float f[16];
     for (int k = 0; k < stride; ++k) f[k] = own[k];
     const float own_status = model == kQuadratic ? own[12] : own[6];
     if (own_status >= 0.5f) {
-        const int px = i % a.grid_w, py = i / a.grid_w, gh = a.grid_count / a.grid_w;
-        float nb[8][12];
-        int count = 0;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                int nx = px + dx, ny = py + dy;
-                if (nx < 0 || ny < 0 || nx >= a.grid_w || ny >= gh) continue;
-                if (a.roi_enabled && model != kQuadratic) {
-                    uint32_t perm = (uint32_t)a.roi_words[(size_t)i * 4];
-                    if ((perm & (1u << ((dy + 1) * 3 + dx + 1))) == 0u) continue;
-                }
-                const float* ns = a.state + (size_t)(ny * a.grid_w + nx) * stride;
-                if ((model == kQuadratic ? ns[12] : ns[6]) < 0.5f) continue;
-                for (int k = 0; k < ncomp; ++k) nb[count][k] = recentred(ns, model, k, -(float)a.D * dx, -(float)a.D * dy);
-                ++count;
-            }
-        if (count >= 4) {
-            float med[12], values[9];
-            for (int k = 0; k < ncomp; ++k) med[k] = own[k];
-            int model_count = model == kTranslation ? 2 : ncomp;
-            int median_count = a.policy == kReplace ? model_count : 2;
-            for (int ai = 0; ai < median_count; ++ai) {
-                int comp = kComponentOrder[ai];
-                for (int k = 0; k < count; ++k) values[k] = nb[k][comp];
-                med[comp] = sorted_median<9>(values, count);
-            }
+        Neighbours3 n;
+        neighbours3(a, i, a.roi_enabled && model != kQuadratic, n);
+        if (n.count >= 4) {
+            const int model_count = model == kTranslation ? 2 : (model == kQuadratic ? 12 : 6);
+            const int median_count = a.policy == kReplace ? model_count : 2;
+            float med[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) med[k] = k < stride ? own[k] : 0.0f;
             bool flagged = false;
+#pragma unroll
             for (int ai = 0; ai < 2; ++ai) {
-                int comp = kComponentOrder[ai];
-                for (int k = 0; k < count; ++k) values[k] = fabsf(nb[k][comp] - med[comp]);
-                float spread = sorted_median<9>(values, count);
+                const int comp = component_order(ai);
+                float v[8];
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+                    v[d] = n.ok[d] ? recentred(n.state[d], model, comp, n.ox[d], n.oy[d]) : __builtin_inff();
+                sort_network(v);
+                med[comp] = pick_median(v, n.count);
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+                    v[d] = n.ok[d] ? fabsf(recentred(n.state[d], model, comp, n.ox[d], n.oy[d]) - med[comp]) : __builtin_inff();
+                sort_network(v);
+                const float spread = pick_median(v, n.count);
                 if (fabsf(own[comp] - med[comp]) / (spread + 0.1f) > 2.0f) flagged = true;
             }
             if (flagged) {
-                if (a.policy == kReplace)
-                    for (int ai = 0; ai < model_count; ++ai) f[kComponentOrder[ai]] = med[kComponentOrder[ai]];
+                if (a.policy == kReplace) {
+#pragma unroll
+                    for (int ai = 2; ai < 12; ++ai) {
+                        if (ai >= median_count) break;
+                        const int comp = component_order(ai);
+                        float v[8];
+#pragma unroll
+                        for (int d = 0; d < 8; ++d)
+                            v[d] = n.ok[d] ? recentred(n.state[d], model, comp, n.ox[d], n.oy[d]) : __builtin_inff();
+                        sort_network(v);
+                        med[comp] = pick_median(v, n.count);
+                    }
+#pragma unroll
+                    for (int ai = 0; ai < 12; ++ai)
+                        if (ai < model_count) f[component_order(ai)] = med[component_order(ai)];
+                }
                 if (model == kQuadratic) {
                     f[12] = 2.0f;
                     f[14] = 0.0f;
@@ -474,22 +542,77 @@ __device__ void uod_policy(const MedianArgs& a, int i) {
     for (int k = 0; k < stride; ++k) out[k] = f[k];
 }
 
+// median_policy for a 3x3 window: own value (if accepted) and the accepted,
+// linked neighbours; at least 3 (ROI, rejected subset) or 4 values.
+__device__ void median_policy3(const MedianArgs& a, int i) {
+    const int model = a.model;
+    const int stride = model == kQuadratic ? 16 : 8;
+    const float* own = a.state + (size_t)i * stride;
+    float* out = a.filtered + (size_t)i * stride;
+    float f[16];
+    for (int k = 0; k < stride; ++k) f[k] = own[k];
+    const float own_status = model == kQuadratic ? own[12] : own[6];
+    const float ref_status = a.metadata[(size_t)i * (model == kQuadratic ? 28 : 16) + 1];
+    const bool skip = (own_status >= 0.5f && !a.scope) || (ref_status < 0.5f && ref_status != -4.0f);
+    if (!skip) {
+        Neighbours3 n;
+        neighbours3(a, i, a.roi_enabled, n);
+        const bool own_ok = own_status >= 0.5f;
+        const int count = n.count + (own_ok ? 1 : 0);
+        const int minimum = (a.roi_enabled && own_status < 0.5f) ? 3 : 4;
+        if (count >= minimum) {
+            const int active = model == kTranslation ? 2 : (model == kQuadratic ? 12 : 6);
+            float filtered[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) filtered[k] = k < stride ? own[k] : 0.0f;
+#pragma unroll
+            for (int ai = 0; ai < 12; ++ai) {
+                if (ai >= active) break;
+                const int comp = component_order(ai);
+                float v[9];
+                v[0] = own_ok ? recentred(own, model, comp, 0.0f, 0.0f) : __builtin_inff();
+#pragma unroll
+                for (int d = 0; d < 8; ++d)
+                    v[d + 1] = n.ok[d] ? recentred(n.state[d], model, comp, n.ox[d], n.oy[d]) : __builtin_inff();
+                sort_network(v);
+                filtered[comp] = pick_median(v, count);
+            }
+            for (int k = 0; k < 8; ++k) f[k] = filtered[k];
+            if (model == kQuadratic) {
+                for (int k = 8; k < 12; ++k) f[k] = filtered[k];
+                f[12] = (a.seed_available && own_status < 0.5f) ? 0.0f : own[12];
+            } else {
+                f[6] = (a.seed_available && own_status < 0.5f) ? 0.0f : own[6];
+                f[7] = own[7];
+            }
+        }
+    }
+    for (int k = 0; k < stride; ++k) out[k] = f[k];
+}
+
+// 3x3 passes: registers only.
 __global__ __launch_bounds__(256) void k_median(MedianArgs a) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.grid_count) return;
-    if (a.policy == kMedian) {
-        if (a.size <= 3)
-            median_policy<9>(a, i);
-        else
-            median_policy<81>(a, i);
-    } else {
-        uod_policy(a, i);
-    }
+    if (a.policy == kMedian)
+        median_policy3(a, i);
+    else
+        uod_policy3(a, i);
+}
+
+// Median windows of 5x5 to 9x9 (up to 81 values in scratch memory).
+__global__ __launch_bounds__(256) void k_median_wide(MedianArgs a) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.grid_count) return;
+    median_policy<81>(a, i);
 }
 
 hipError_t launch_median(const MedianArgs& a, hipStream_t s) {
     if (a.grid_count <= 0) return hipSuccess;
-    k_median<<<(a.grid_count + 255) / 256, 256, 0, s>>>(a);
+    if (a.policy == kMedian && a.size > 3)
+        k_median_wide<<<(a.grid_count + 255) / 256, 256, 0, s>>>(a);
+    else
+        k_median<<<(a.grid_count + 255) / 256, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
