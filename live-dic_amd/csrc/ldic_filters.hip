@@ -51,46 +51,91 @@ hipError_t launch_reduce(const uint8_t* src, int w, int h, uint8_t* dst, hipStre
 // Target prefilter: 25-tap FIR (Gaussian sigma=1 combined with the truncated
 // inverse cubic B-spline filter) along x then y, mirror boundaries
 // (shaders/icgn.wgsl:145-214).  Both passes are fused through LDS: the block
-// loads a reflected 88x56 byte footprint, filters its rows into LDS, then
-// filters the columns.  Sums keep the reference tap order without FMA
-// contraction, so each output equals the two-pass float32 evaluation bit for bit.
+// stages its reflected 88x56 byte footprint as bytes (aligned dword loads in
+// the interior, reflected byte loads at the image border), filters the rows
+// into a float tile, then filters the columns.  Sums keep the reference tap
+// order without FMA contraction, so each output equals the two-pass float32
+// evaluation bit for bit.
 // ---------------------------------------------------------------------------
 constexpr int kPfR = 12;
 constexpr int kPfTX = 64, kPfTY = 32;
 constexpr int kPfInW = kPfTX + 2 * kPfR, kPfInH = kPfTY + 2 * kPfR;
+constexpr int kPfInWords = kPfInW / 4;   // 22 dwords of bytes per staged row
+constexpr int kPfRun = 4;                // outputs per thread and pass
+static_assert(kPfInW % 4 == 0 && kPfR % 4 == 0 && kPfRun == 4 && kPfTX % kPfRun == 0 && kPfTY % kPfRun == 0, "tile");
 
 __global__ __launch_bounds__(256) void k_prefilter(const uint8_t* __restrict__ src, int w, int h,
                                                    const float* __restrict__ fir31, float* __restrict__ coeff) {
 #pragma clang fp contract(off)
-    __shared__ float in[kPfInH * kPfInW];
-    __shared__ float mid[kPfInH * kPfTX];
+    __shared__ uint32_t in[kPfInH * kPfInWords];
+    __shared__ float4 mid[kPfInH * kPfTX / 4];
     __shared__ float taps[2 * kPfR + 1];
     const int bx = blockIdx.x * kPfTX, by = blockIdx.y * kPfTY;
     if (threadIdx.x < 2 * kPfR + 1) taps[threadIdx.x] = fir31[threadIdx.x + 3];
-    for (int i = threadIdx.x; i < kPfInH * kPfInW; i += 256) {
-        int ty = i / kPfInW, tx = i - ty * kPfInW;
-        int sx = mirror_clamp(bx + tx - kPfR, w), sy = mirror_clamp(by + ty - kPfR, h);
-        in[i] = (float)src[(size_t)sy * w + sx];
+    const bool interior = bx >= kPfR && bx + kPfTX + kPfR <= w && by >= kPfR && by + kPfTY + kPfR <= h &&
+                          (w & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 3) == 0;
+    if (interior) {
+        // bx - R and every row start are multiples of 4: whole dwords, no reflection.
+        const uint8_t* base = src + (size_t)(by - kPfR) * w + (bx - kPfR);
+        for (int i = threadIdx.x; i < kPfInH * kPfInWords; i += 256) {
+            const int ty = i / kPfInWords, dx = i - ty * kPfInWords;
+            in[i] = *reinterpret_cast<const uint32_t*>(base + (size_t)ty * w + 4 * dx);
+        }
+    } else {
+        uint8_t* in8 = reinterpret_cast<uint8_t*>(in);
+        for (int i = threadIdx.x; i < kPfInH * kPfInW; i += 256) {
+            const int ty = i / kPfInW, tx = i - ty * kPfInW;
+            const int sx = mirror_clamp(bx + tx - kPfR, w), sy = mirror_clamp(by + ty - kPfR, h);
+            in8[i] = src[(size_t)sy * w + sx];
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kPfInH * kPfTX; i += 256) {
-        int ty = i / kPfTX, tx = i - ty * kPfTX;
-        const float* row = in + ty * kPfInW + tx;
-        float r = 0.0f;
+    float tp[2 * kPfR + 1];
 #pragma unroll
-        for (int t = 0; t < 2 * kPfR + 1; ++t) r = r + taps[t] * row[t];
-        mid[i] = r;
+    for (int t = 0; t < 2 * kPfR + 1; ++t) tp[t] = taps[t];
+    // Rows: each thread filters kPfRun consecutive outputs from one register
+    // window of 2R + kPfRun inputs, read from LDS once per run.
+    for (int i = threadIdx.x; i < kPfInH * (kPfTX / kPfRun); i += 256) {
+        const int ty = i / (kPfTX / kPfRun), q = i - ty * (kPfTX / kPfRun);
+        const uint32_t* row = in + ty * kPfInWords + q;
+        float win[2 * kPfR + kPfRun];
+#pragma unroll
+        for (int j = 0; j < (2 * kPfR + kPfRun) / 4; ++j) {
+            const uint32_t v = row[j];
+            win[4 * j + 0] = (float)(v & 0xffu);
+            win[4 * j + 1] = (float)((v >> 8) & 0xffu);
+            win[4 * j + 2] = (float)((v >> 16) & 0xffu);
+            win[4 * j + 3] = (float)(v >> 24);
+        }
+        float r[kPfRun];
+#pragma unroll
+        for (int o = 0; o < kPfRun; ++o) {
+            r[o] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 2 * kPfR + 1; ++t) r[o] = r[o] + tp[t] * win[o + t];
+        }
+        mid[ty * (kPfTX / 4) + q] = make_float4(r[0], r[1], r[2], r[3]);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kPfTY * kPfTX; i += 256) {
-        int ty = i / kPfTX, tx = i - ty * kPfTX;
-        int x = bx + tx, y = by + ty;
-        if (x >= w || y >= h) continue;
-        const float* col = mid + ty * kPfTX + tx;
-        float r = 0.0f;
+    // Columns: the same register window along y.
+    const float* midf = reinterpret_cast<const float*>(mid);
+    for (int i = threadIdx.x; i < (kPfTY / kPfRun) * kPfTX; i += 256) {
+        const int ty = (i / kPfTX) * kPfRun, tx = i - (i / kPfTX) * kPfTX;
+        const int x = bx + tx;
+        if (x >= w) continue;
+        const float* col = midf + ty * kPfTX + tx;
+        float win[2 * kPfR + kPfRun];
 #pragma unroll
-        for (int t = 0; t < 2 * kPfR + 1; ++t) r = r + taps[t] * col[t * kPfTX];
-        coeff[(size_t)y * w + x] = r;
+        for (int t = 0; t < 2 * kPfR + kPfRun; ++t) win[t] = col[t * kPfTX];
+#pragma unroll
+        for (int o = 0; o < kPfRun; ++o) {
+            const int y = by + ty + o;
+            if (y >= h) break;
+            float r = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 2 * kPfR + 1; ++t) r = r + tp[t] * win[o + t];
+            coeff[(size_t)y * w + x] = r;
+        }
     }
 }
 

@@ -38,7 +38,7 @@ def test_reduce_is_bit_exact(native):
 
 def test_prefilter_is_bit_exact(native):
     rng = np.random.default_rng(2)
-    for h, w in ((20, 17), (77, 100), (512, 512)):
+    for h, w in ((20, 17), (77, 100), (512, 512), (301, 999), (300, 1000)):
         _say("prefilter", h, w)
         img = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
         out = np.empty((h, w), np.float32)
