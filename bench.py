@@ -256,6 +256,73 @@ def batched_heterogeneous(frames: int = 20) -> dict:
     }
 
 
+PAIR_FRAMES = 30
+PAIR_SUBSET, PAIR_STRIDE = 25, 4
+PAIR_ITERATIONS, PAIR_TOLERANCE = 30, 1e-3
+
+
+def consecutive_quadratic() -> dict:
+    """Quadratic IC-GN in consecutive-pair mode at 4K: frame k is the reference of frame k + 1,
+    prepared on the GPU in the pair's own sequence (Gaussian, B-spline scans, gradients, FP64
+    subset preparation), then the target prefilter, the solve and the outlier pass.  ROI: the
+    plate rectangle minus 3 circular holes of radius 200-400 px; subset 25, step 4."""
+    from live_dic import ICGNWGPUFullPyramid, _native, synthetic
+
+    lib = _native.library()
+    seq = synthetic.QuadraticPairs(frames=PAIR_FRAMES)
+    mask = synthetic.plate_with_holes()
+    frames = []
+    for k in range(PAIR_FRAMES):
+        buf = _native.DeviceBuffer(WIDTH * HEIGHT)
+        seq.render_device(k, buf)
+        frames.append(buf)
+    eng = ICGNWGPUFullPyramid(pyramid_levels=1, iterations_per_level=(PAIR_ITERATIONS,),
+                              tolerance_per_level=(PAIR_TOLERANCE,), motion_model="quadratic",
+                              subset_size=PAIR_SUBSET, subset_stride=PAIR_STRIDE)
+    try:
+        info = eng.set_reference(seq.render(0), roi_mask=mask)
+        gw, gh = info.grid
+        selected = int(np.count_nonzero(eng.roi.selected))
+        h = eng._handle
+        _native.check(lib.ldic_register_pair_resident(h, 0, frames[0].address, frames[1].address, WIDTH))
+        _native.check(lib.ldic_slot_wait(h, 0, ctypes.byref(_native.Timing())))
+        _native.check(lib.ldic_device_synchronize())
+        pairs = PAIR_FRAMES - 1
+        t0 = time.perf_counter()
+        for k in range(pairs):
+            _native.check(lib.ldic_register_pair_resident(h, 0, frames[k].address, frames[k + 1].address, WIDTH))
+        _native.check(lib.ldic_slot_wait(h, 0, ctypes.byref(_native.Timing())))
+        _native.check(lib.ldic_device_synchronize())
+        per_pair = (time.perf_counter() - t0) / pairs
+        single = []
+        for k in (pairs - 2, pairs - 1):
+            t = _native.Timing()
+            _native.check(lib.ldic_register_pair_resident(h, 0, frames[k].address, frames[k + 1].address, WIDTH))
+            _native.check(lib.ldic_slot_wait(h, 0, ctypes.byref(t)))
+            single.append((t.gpu_ms, t.solve_ms))
+        state = np.empty((gh, gw, 16), np.float32)
+        _native.check(lib.ldic_slot_output(h, 0, state.ctypes.data))
+    finally:
+        eng.close()
+    cx, cy = synthetic.subset_centres(WIDTH, HEIGHT, PAIR_SUBSET, PAIR_STRIDE)
+    u, v = seq.pair_truth(PAIR_FRAMES - 2, cx, cy)
+    ok = state[..., 12] > 0.5
+    err = np.hypot(state[..., 0] - u, state[..., 3] - v)[ok]
+    return {
+        "workload": f"4K quadratic consecutive pairs, plate with 3 holes, subset {PAIR_SUBSET}, step {PAIR_STRIDE}, "
+                    f"{PAIR_FRAMES} frames ({pairs} pairs), {PAIR_ITERATIONS} iterations, tolerance {PAIR_TOLERANCE}",
+        "grid": [gw, gh],
+        "roi_subsets": selected,
+        "pairs_per_s": 1.0 / per_pair,
+        "pair_ms": per_pair * 1e3,
+        "roi_subsets_per_s": selected / per_pair,
+        "last_pair_gpu_ms": single[-1][0],
+        "last_pair_solve_ms": single[-1][1],
+        "accepted_fraction": float(ok.sum()) / max(1, selected),
+        "rmse_px": float(np.sqrt(np.mean(err**2))) if err.size else None,
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -265,7 +332,9 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-learned", action="store_true", help="skip the learned-vs-ZNCC seeding comparison")
     parser.add_argument("--no-batch", action="store_true", help="skip the batched heterogeneous workload")
-    parser.add_argument("--only", choices=("learned", "batch"),
+    parser.add_argument("--no-pairs", action="store_true",
+                        help="skip the quadratic consecutive-pair workload with ROI holes")
+    parser.add_argument("--only", choices=("learned", "batch", "pairs"),
                         help="run only this secondary workload and print its JSON object (for profiling)")
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="only copy 256 MiB with one plain device copy kernel (PMC byte-counter calibration)")
@@ -285,7 +354,8 @@ def main() -> int:
               "sharding design); run with --gpus 1", file=sys.stderr)
         return 2
     if args.only:
-        print(json.dumps(learned_vs_zncc() if args.only == "learned" else batched_heterogeneous()))
+        legs = {"learned": learned_vs_zncc, "batch": batched_heterogeneous, "pairs": consecutive_quadratic}
+        print(json.dumps(legs[args.only]()))
         return 0
 
     from live_dic import ICGNWGPUFullPyramid, _native, synthetic
@@ -392,6 +462,9 @@ def main() -> int:
     if not args.no_batch:
         print("[bench] batched heterogeneous pairs", file=sys.stderr, flush=True)
         line["batched_heterogeneous"] = batched_heterogeneous()
+    if not args.no_pairs:
+        print("[bench] quadratic consecutive pairs with ROI holes", file=sys.stderr, flush=True)
+        line["consecutive_quadratic"] = consecutive_quadratic()
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(seq, 30)

@@ -8,6 +8,9 @@
 
 #include <float.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace ldic {
 
 constexpr int kPrepWaves = 4;
@@ -28,6 +31,18 @@ __host__ __device__ constexpr int tri_col(int e, int P) {
         ++i;
     }
     return i + e;
+}
+
+// Calls f(std::integral_constant<int, E>) for E = 0 .. N-1, so that indices
+// derived from E (the triangle row and column above) are compile-time
+// constants: registers instead of scratch arrays, no runtime index loops.
+template <typename F, int... Es>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Es...>) {
+    (f(std::integral_constant<int, Es>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 template <int P>
@@ -98,13 +113,13 @@ __device__ void moment_pass(const PrepArgs& a, int lane, int ox, int oy, double 
         PixelRef r = pixel(a, p, ox, oy, half, mean);
         double J[12];
         jacobian<JP>(r.gx, r.gy, r.lx, r.ly, J);
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            const int i = tri_row(E0 + e, N), j = tri_col(E0 + e, N);
+        static_for<NE>([&](auto e) {
+            constexpr int E = decltype(e)::value;
+            constexpr int i = tri_row(E0 + E, N), j = tri_col(E0 + E, N);
             // Translation uses the (gx, gy) pair only: indices 0 and 3.
-            const int ii = P == 2 ? 3 * i : i, jj = P == 2 ? 3 * j : j;
-            acc[e] += J[ii] * J[jj] * r.w;
-        }
+            constexpr int ii = P == 2 ? 3 * i : i, jj = P == 2 ? 3 * j : j;
+            acc[E] += J[ii] * J[jj] * r.w;
+        });
         if (FIRST) {
             energy += r.f * r.f * r.w;
 #pragma unroll
@@ -119,15 +134,15 @@ __device__ void moment_pass(const PrepArgs& a, int lane, int ox, int oy, double 
             }
         }
     }
-#pragma unroll
-    for (int e = 0; e < NE; ++e) {
-        double v = wave_sum(acc[e]);
-        const int i = tri_row(E0 + e, N), j = tri_col(E0 + e, N);
+    static_for<NE>([&](auto e) {
+        constexpr int E = decltype(e)::value;
+        constexpr int i = tri_row(E0 + E, N), j = tri_col(E0 + E, N);
+        const double v = wave_sum(acc[E]);
         if (lane == 0) {
             h[i * pitch + j] = v;
             h[j * pitch + i] = v;
         }
-    }
+    });
     if (FIRST) {
         first->energy = wave_sum(energy);
 #pragma unroll
@@ -246,15 +261,15 @@ __global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
         t33 = hxx * idet;
         valid = valid && fmax(fmax(fabs(t00), fabs(t03)), fabs(t33)) <= (double)FLT_MAX;
     } else {
+        // Every entry of the inverse must be finite and representable in float32;
+        // lanes check the P*P entries in parallel.
         bool ok = invert<P>(mat, lane);
-        double peak = 0.0;
-        bool finite = true;
-        for (int k = 0; k < P * P; ++k) {
-            double v = mat[(k / P) * C + P + k % P];
-            finite = finite && isfinite(v);
-            peak = fmax(peak, fabs(v));
+        bool bad = false;
+        for (int k = lane; k < P * P; k += kWave) {
+            const double v = mat[(k / P) * C + P + k % P];
+            bad = bad || !isfinite(v) || fabs(v) > (double)FLT_MAX;
         }
-        valid = ok && finite && energy > 1e-12 && peak <= (double)FLT_MAX;
+        valid = ok && !__any(bad) && energy > 1e-12;
     }
 
     float* meta = a.metadata + (size_t)subset * M;

@@ -251,6 +251,10 @@ class QuadraticPairs:
         return self.texture.render(None if s == 0 else self.field.scaled(s), noise_sigma=1.0,
                                    noise_seed=self.seed * 7919 + k)
 
+    def render_device(self, k: int, out: _native.DeviceBuffer) -> None:
+        s = self.scale(k)
+        self.texture.render_device(out, None if s == 0 else self.field.scaled(s), 1.0, self.seed * 7919 + k)
+
     def pair_truth(self, k: int, X: np.ndarray, Y: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         """Displacement from frame k to frame k+1 at frame-k pixel coordinates (X, Y)."""
         s0, s1 = self.scale(k), self.scale(k + 1)
