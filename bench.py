@@ -320,6 +320,7 @@ def consecutive_quadratic() -> dict:
         "last_pair_solve_ms": single[-1][1],
         "accepted_fraction": float(ok.sum()) / max(1, selected),
         "rmse_px": float(np.sqrt(np.mean(err**2))) if err.size else None,
+        "mean_iterations": float(state[..., 13][ok].mean()) if ok.any() else None,
     }
 
 

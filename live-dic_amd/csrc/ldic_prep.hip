@@ -155,6 +155,86 @@ __device__ void moment_pass(const PrepArgs& a, int lane, int ox, int oy, double 
     }
 }
 
+// Quadratic moments in one pixel pass.  Every Jacobian component is a gradient
+// times one of the monomials {1, x, y, x^2/2, xy, y^2/2}, so each of the 78
+// Hessian entries is c * sum w g_s g_t x^p y^q with p + q <= 4, c in {1, 1/2,
+// 1/4} and (s, t) in {xx, xy, yy}: 45 distinct sums.  J^T f and sum J are the
+// degree <= 2 sums of w f g_s and w g_s (24).  The monomials of the integer
+// subset offsets and the power-of-two factors are exact in float64.
+__host__ __device__ constexpr int mono_index(int p, int q) { return (p + q) * (p + q + 1) / 2 + q; }
+// Component k of the quadratic Jacobian: gradient s(k), monomial exponents
+// (px(k), py(k)) and factor c(k).
+__host__ __device__ constexpr int jac_grad(int k) { return (k / 3) % 2; }
+__host__ __device__ constexpr int jac_mono(int k) { return k % 3 + (k >= 6 ? 3 : 0); }
+__host__ __device__ constexpr int mono_px(int u) { return u == 1 ? 1 : u == 3 ? 2 : u == 4 ? 1 : 0; }
+__host__ __device__ constexpr int mono_py(int u) { return u == 2 ? 1 : u == 4 ? 1 : u == 5 ? 2 : 0; }
+__host__ __device__ constexpr double mono_c(int u) { return (u == 3 || u == 5) ? 0.5 : 1.0; }
+
+__device__ void quadratic_moments(const PrepArgs& a, int lane, int ox, int oy, double half, double mean, double* h,
+                                  int pitch, FirstMoments* first) {
+#pragma clang fp contract(off)
+    double M[3][15], F[2][6], G[2][6], energy = 0.0;
+#pragma unroll
+    for (int m = 0; m < 15; ++m) M[0][m] = M[1][m] = M[2][m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) F[0][m] = F[1][m] = G[0][m] = G[1][m] = 0.0;
+    const int SS = a.S * a.S;
+    for (int p = lane; p < SS; p += kWave) {
+        PixelRef r = pixel(a, p, ox, oy, half, mean);
+        const double x = r.lx, y = r.ly;
+        const double x2 = x * x, xy = x * y, y2 = y * y;
+        const double mono[15] = {1.0,    x,      y,      x2,      xy,      y2,      x2 * x, x2 * y,
+                                 x * y2, y2 * y, x2 * x2, x2 * xy, x2 * y2, xy * y2, y2 * y2};
+        const double gg[3] = {r.gx * r.gx * r.w, r.gx * r.gy * r.w, r.gy * r.gy * r.w};
+        const double wf = r.f * r.w;
+        const double gf[2] = {r.gx * wf, r.gy * wf}, gw[2] = {r.gx * r.w, r.gy * r.w};
+        energy += r.f * wf;
+#pragma unroll
+        for (int st = 0; st < 3; ++st) {
+            M[st][0] += gg[st];
+#pragma unroll
+            for (int m = 1; m < 15; ++m) M[st][m] += gg[st] * mono[m];
+        }
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            F[g][0] += gf[g];
+            G[g][0] += gw[g];
+#pragma unroll
+            for (int m = 1; m < 6; ++m) {
+                F[g][m] += gf[g] * mono[m];
+                G[g][m] += gw[g] * mono[m];
+            }
+        }
+        if (a.D == a.S) {
+            int row = p / a.S, col = p - row * a.S;
+            a.reference[(size_t)(oy + row) * a.width + ox + col].x = (float)r.f;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 15; ++m) M[0][m] = wave_sum(M[0][m]), M[1][m] = wave_sum(M[1][m]), M[2][m] = wave_sum(M[2][m]);
+    first->energy = wave_sum(energy);
+    static_for<12>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int u = jac_mono(k), g = jac_grad(k);
+        constexpr int m = mono_index(mono_px(u), mono_py(u));
+        first->jtf[k] = mono_c(u) * wave_sum(F[g][m]);
+        first->sj[k] = mono_c(u) * wave_sum(G[g][m]);
+    });
+    static_for<78>([&](auto e) {
+        constexpr int E = decltype(e)::value;
+        constexpr int i = tri_row(E, 12), j = tri_col(E, 12);
+        constexpr int ui = jac_mono(i), uj = jac_mono(j);
+        constexpr int st = jac_grad(i) + jac_grad(j);
+        constexpr int m = mono_index(mono_px(ui) + mono_px(uj), mono_py(ui) + mono_py(uj));
+        constexpr double c = mono_c(ui) * mono_c(uj);
+        if (lane == 0) {
+            const double v = c * M[st][m];
+            h[i * pitch + j] = v;
+            h[j * pitch + i] = v;
+        }
+    });
+}
+
 // Column-parallel Gauss-Jordan elimination with partial pivoting on the
 // P x 2P augmented matrix held in LDS.  Returns false for an exactly singular
 // matrix (a zero pivot), which is when LAPACK's LU factorisation fails too.
@@ -229,15 +309,12 @@ __global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
     }
     const double mean = wave_sum(s0) / a.weight_sum;
 
-    // Pass 1 (and 2, 3 for quadratic): energy, J^T f, sum J and the Hessian,
-    // in chunks that bound register use, written straight into the augmented
-    // P x 2P layout used by the inversion.
+    // Pass 1: energy, J^T f, sum J and the Hessian, written straight into the
+    // augmented P x 2P layout used by the inversion.
     constexpr int C = 2 * P;
     FirstMoments fm;
     if (MODEL == kQuadratic) {
-        moment_pass<12, 0, 20, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
-        moment_pass<12, 20, 50, false>(a, lane, ox, oy, half, mean, mat, C, nullptr);
-        moment_pass<12, 50, 78, false>(a, lane, ox, oy, half, mean, mat, C, nullptr);
+        quadratic_moments(a, lane, ox, oy, half, mean, mat, C, &fm);
     } else if (MODEL == kAffine) {
         moment_pass<6, 0, 21, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
     } else {

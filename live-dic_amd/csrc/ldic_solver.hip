@@ -167,9 +167,15 @@ __device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, f
     return inside;
 }
 
-// icgn.wgsl:437-513; false when a pivot falls below 1e-8.
-__device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12]) {
-    float a[6][12];
+// icgn.wgsl:437-513; false when a pivot falls below 1e-8.  The Gauss-Jordan
+// elimination of [M | I] runs column-parallel: lane e < 12 owns column e of the
+// 6 x 12 augmented matrix (the 6 x 6 lifted warp operator M, then the
+// identity), pivot columns are broadcast with readlane, and each lane applies
+// the same row exchange, division and update to its own column.  Every value
+// is formed by the same operations as in a one-lane elimination.
+__device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12], const int lane) {
+    const int e = lane < 12 ? lane : 11;
+    float col[6];
     {
         float u = d[0], ux = d[1], uy = d[2], v = d[3], vx = d[4], vy = d[5];
         float uxx = d[6], uxy = d[7], uyy = d[8], vxx = d[9], vxy = d[10], vyy = d[11];
@@ -184,61 +190,61 @@ __device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12]) {
             {0.5f * vxx, vxy, 0.5f * vyy, vx, 1.0f + vy, v},
             {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f}};
 #pragma unroll
-        for (int r = 0; r < 6; ++r)
+        for (int r = 0; r < 6; ++r) {
+            float x = r == e - 6 ? 1.0f : 0.0f;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                a[r][c] = m[r][c];
-                a[r][6 + c] = r == c ? 1.0f : 0.0f;
-            }
+            for (int c = 0; c < 6; ++c) x = e == c ? m[r][c] : x;
+            col[r] = x;
+        }
     }
 #pragma unroll
-    for (int col = 0; col < 6; ++col) {
-        int pr = col;
-        float ps = fabsf(a[col][col]);
+    for (int c = 0; c < 6; ++c) {
+        float pc[6];
 #pragma unroll
-        for (int cand = col + 1; cand < 6; ++cand) {
-            float cs = fabsf(a[cand][col]);
+        for (int r = 0; r < 6; ++r) pc[r] = readlane_f32(col[r], c);
+        int pr = c;
+        float ps = fabsf(pc[c]);
+#pragma unroll
+        for (int cand = c + 1; cand < 6; ++cand) {
+            float cs = fabsf(pc[cand]);
             if (cs > ps) {
                 pr = cand;
                 ps = cs;
             }
         }
         if (ps < 1e-8f) return false;
-        // Branch-free row exchange keeps the matrix in registers.
 #pragma unroll
-        for (int r = col + 1; r < 6; ++r) {
-            bool sw = r == pr;
-#pragma unroll
-            for (int e = 0; e < 12; ++e) {
-                float x = a[col][e], y = a[r][e];
-                a[col][e] = sw ? y : x;
-                a[r][e] = sw ? x : y;
-            }
+        for (int r = c + 1; r < 6; ++r) {
+            const bool sw = r == pr;
+            float x = col[c], y = col[r];
+            col[c] = sw ? y : x;
+            col[r] = sw ? x : y;
+            x = pc[c];
+            y = pc[r];
+            pc[c] = sw ? y : x;
+            pc[r] = sw ? x : y;
         }
-        float pivot = a[col][col];
-#pragma unroll
-        for (int e = 0; e < 12; ++e) a[col][e] /= pivot;
+        col[c] /= pc[c];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            if (r == col) continue;
-            float f = a[r][col];
-#pragma unroll
-            for (int e = 0; e < 12; ++e) a[r][e] -= f * a[col][e];
+            if (r == c) continue;
+            col[r] -= pc[r] * col[c];
         }
     }
-    float cu[6] = {0.5f * q.C[0], q.C[1], 0.5f * q.C[2], 1.0f + q.L[0], q.L[1], q.T[0]};
-    float cv[6] = {0.5f * q.C[3], q.C[4], 0.5f * q.C[5], q.L[2], 1.0f + q.L[3], q.T[1]};
+    // Lane 6 + k holds column k of M^-1.
+    const float cu[6] = {0.5f * q.C[0], q.C[1], 0.5f * q.C[2], 1.0f + q.L[0], q.L[1], q.T[0]};
+    const float cv[6] = {0.5f * q.C[3], q.C[4], 0.5f * q.C[5], q.L[2], 1.0f + q.L[3], q.T[1]};
+    float su = 0.0f, sv = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        su += cu[r] * col[r];
+        sv += cv[r] * col[r];
+    }
     float uu[6], uv[6];
 #pragma unroll
-    for (int col = 0; col < 6; ++col) {
-        float su = 0.0f, sv = 0.0f;
-#pragma unroll
-        for (int e = 0; e < 6; ++e) {
-            su += cu[e] * a[e][6 + col];
-            sv += cv[e] * a[e][6 + col];
-        }
-        uu[col] = su;
-        uv[col] = sv;
+    for (int k = 0; k < 6; ++k) {
+        uu[k] = readlane_f32(su, 6 + k);
+        uv[k] = readlane_f32(sv, 6 + k);
     }
     q.C[0] = 2.0f * uu[0];
     q.C[1] = uu[1];
@@ -410,6 +416,14 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 if (WEIGHTED) pw[k] = 0.0f;
             }
         }
+    }
+
+    // Quadratic: lane r < 12 keeps row r of H^-1 for the update.
+    float inv_row[MODEL == kQuadratic ? 12 : 1];
+    if (MODEL == kQuadratic) {
+        const float* row = inv + 12 * min(lane, 11);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) inv_row[k] = row[k];
     }
 
     Tile t;
@@ -597,14 +611,13 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             float res[12], d[12];
 #pragma unroll
             for (int k = 0; k < 12; ++k) res[k] = meta[2 + k] - ratio * (rj[k] - mean * meta[14 + k]);
+            // Lane r < 12 forms row r of H^-1 res; the rows are then broadcast.
+            float v = 0.0f;
 #pragma unroll
-            for (int r = 0; r < 12; ++r) {
-                float v = 0.0f;
+            for (int k = 0; k < 12; ++k) v += inv_row[k] * res[k];
 #pragma unroll
-                for (int k = 0; k < 12; ++k) v += inv[12 * r + k] * res[k];
-                d[r] = -v;
-            }
-            if (!compose_quadratic(q, d)) {
+            for (int r = 0; r < 12; ++r) d[r] = -readlane_f32(v, r);
+            if (!compose_quadratic(q, d, lane)) {
                 status = -1.0f;
                 continue;
             }
