@@ -414,6 +414,13 @@ def main() -> int:
     u, v = seq.field.scaled(last / (FRAMES - 1)).displacement(cx, cy)
     ok = out[..., 6] > 0.5
     rmse = float(np.sqrt(np.mean((out[..., 0][ok] - u[ok]) ** 2 + (out[..., 3][ok] - v[ok]) ** 2)))
+    mean_iterations = []  # coarse to fine, accepted subsets of the last frame
+    for level in reversed(range(LEVELS)):
+        lgw, lgh = eng._level_shapes[level][2:4]
+        st = np.empty((lgh, lgw, 8), np.float32)
+        _native.check(lib.ldic_export_level_state(handle, 0, level, st.ctypes.data))
+        acc = st[..., 6] > 0.5
+        mean_iterations.append(float(st[..., 7][acc].mean()) if acc.any() else None)
     eng.close()
 
     avg_solve_ms = solve_total / solve_count if solve_count else float("nan")
@@ -441,6 +448,7 @@ def main() -> int:
             "rmse_px_last_frame": rmse,
             "reference_prep_ms": prep_ms,
             "iterations_per_level": list(ITERATIONS),
+            "mean_iterations_per_level": mean_iterations,
             "tolerance_per_level": list(TOLERANCES),
             "solver_launches_timed": solve_count,
         },
