@@ -672,11 +672,19 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
     }
 }
 
-// Register budget: the unweighted translation/affine B-spline and bilinear
-// variants with up to 16 pixels per lane fit 168 VGPRs without spilling, i.e.
-// three waves per SIMD; the others keep the compiler's own allocation.
-constexpr int solve_min_waves(int model, int interp, int ppl, bool weighted) {
-    return (model != kQuadratic && interp != kKeys && !weighted && ppl > 0 && ppl <= 16) ? 3 : 1;
+// Register budget (waves per SIMD = 512 / VGPRs, allocated in steps of 8):
+// - register-cached translation/affine variants (up to 16 pixels per lane):
+//   3 waves, 168 VGPRs;
+// - uncached translation/affine: 5 waves, 96 VGPRs, as many as the LDS slice
+//   of a subset-31 footprint allows (98 VGPRs would leave 4);
+// - uncached quadratic: 4 waves, 128 VGPRs (the persistent batched kernel
+//   would spill there and keeps its own allocation).
+// None of these spill.  Keys and Gaussian-weighted variants keep the compiler's
+// own allocation.
+constexpr int solve_min_waves(int model, int interp, int ppl, bool weighted, bool batched = false) {
+    if (interp == kKeys || weighted) return 1;
+    if (ppl == 0) return model == kQuadratic ? (batched ? 1 : 4) : 5;
+    return (model != kQuadratic && ppl <= 16) ? 3 : 1;
 }
 
 template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
@@ -704,7 +712,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
 // giving every XCD contiguous runs of them.  Waves never synchronise with each
 // other, so a wave moves to its next item as soon as its subset is done.
 template <int MODEL, int INTERP, int PPL, bool WEIGHTED, int TP>
-__global__ __launch_bounds__(256) void k_solve_batched(const SolveArgs* __restrict__ pairs,
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
+                          amdgpu_waves_per_eu(solve_min_waves(MODEL, INTERP, PPL, WEIGHTED, true)))) void k_solve_batched(
+    const SolveArgs* __restrict__ pairs,
                                                        const int* __restrict__ block_start, int npairs, int total,
                                                        int wave_stride) {
     extern __shared__ float lds[];
