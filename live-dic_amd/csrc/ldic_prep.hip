@@ -283,48 +283,15 @@ __device__ bool invert(double* m, int lane) {
     return !singular;
 }
 
+// Validity, inversion and outputs of one subset whose Hessian sits in the
+// wave's LDS matrix `mat` (P x 2P, left half).
 template <int MODEL>
-__global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
+__device__ void finish_subset(const PrepArgs& a, const int subset, const int lane, double* mat, const double energy,
+                              const double* jtf, const double* sj, const double mean) {
 #pragma clang fp contract(off)
     constexpr int P = MODEL == kQuadratic ? 12 : 6;
     constexpr int M = MODEL == kQuadratic ? 28 : 16;
-    __shared__ double mats[kPrepWaves][P * 2 * P];
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int subset = xcd_block(blockIdx.x, gridDim.x) * kPrepWaves + wave;
-    const int N = a.grid_w * a.grid_h;
-    if (subset >= N) return;
-    double* mat = mats[wave];
-    const int gx = subset % a.grid_w, gy = subset / a.grid_w;
-    const int ox = gx * a.D, oy = gy * a.D;
-    const double half = 0.5 * (double)(a.S - 1);
-    const int SS = a.S * a.S;
-
-    // Pass 0: weighted mean of the raw (or Gaussian-filtered) intensity.
-    double s0 = 0.0;
-    for (int p = lane; p < SS; p += kWave) {
-        int row = p / a.S, col = p - row * a.S;
-        double w = a.weighted ? a.weights[p] : 1.0;
-        s0 += a.intensity[(size_t)(oy + row) * a.width + ox + col] * w;
-    }
-    const double mean = wave_sum(s0) / a.weight_sum;
-
-    // Pass 1: energy, J^T f, sum J and the Hessian, written straight into the
-    // augmented P x 2P layout used by the inversion.
     constexpr int C = 2 * P;
-    FirstMoments fm;
-    if (MODEL == kQuadratic) {
-        quadratic_moments(a, lane, ox, oy, half, mean, mat, C, &fm);
-    } else if (MODEL == kAffine) {
-        moment_pass<6, 0, 21, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
-    } else {
-        moment_pass<2, 0, 3, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
-    }
-    const double energy = fm.energy;
-    const double* jtf = fm.jtf;
-    const double* sj = fm.sj;
-    wave_lds_sync();
-
     bool valid;
     double t00 = 0.0, t03 = 0.0, t33 = 0.0;
     if (MODEL == kTranslation) {
@@ -375,9 +342,190 @@ __global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
     }
 }
 
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_subset_prep(PrepArgs a) {
+#pragma clang fp contract(off)
+    constexpr int P = MODEL == kQuadratic ? 12 : 6;
+    __shared__ double mats[kPrepWaves][P * 2 * P];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int subset = xcd_block(blockIdx.x, gridDim.x) * kPrepWaves + wave;
+    const int N = a.grid_w * a.grid_h;
+    if (subset >= N) return;
+    double* mat = mats[wave];
+    const int gx = subset % a.grid_w, gy = subset / a.grid_w;
+    const int ox = gx * a.D, oy = gy * a.D;
+    const double half = 0.5 * (double)(a.S - 1);
+    const int SS = a.S * a.S;
+
+    // Pass 0: weighted mean of the raw (or Gaussian-filtered) intensity.
+    double s0 = 0.0;
+    for (int p = lane; p < SS; p += kWave) {
+        int row = p / a.S, col = p - row * a.S;
+        double w = a.weighted ? a.weights[p] : 1.0;
+        s0 += a.intensity[(size_t)(oy + row) * a.width + ox + col] * w;
+    }
+    const double mean = wave_sum(s0) / a.weight_sum;
+
+    // Pass 1: energy, J^T f, sum J and the Hessian, written straight into the
+    // augmented P x 2P layout used by the inversion.
+    constexpr int C = 2 * P;
+    FirstMoments fm;
+    if (MODEL == kQuadratic) {
+        quadratic_moments(a, lane, ox, oy, half, mean, mat, C, &fm);
+    } else if (MODEL == kAffine) {
+        moment_pass<6, 0, 21, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
+    } else {
+        moment_pass<2, 0, 3, true>(a, lane, ox, oy, half, mean, mat, C, &fm);
+    }
+    const double energy = fm.energy;
+    const double* jtf = fm.jtf;
+    const double* sj = fm.sj;
+    wave_lds_sync();
+
+    finish_subset<MODEL>(a, subset, lane, mat, energy, jtf, sj, mean);
+}
+
+// Quadratic preparation by separable sums, for uniform weights and step < S.
+// A block takes one subset column and up to kStripRows subset rows.
+//  1. One thread per image row of the strip forms the row-segment sums over
+//     the S columns of the subset column (x = c - (S-1)/2):
+//       q  0..14  sum g_s g_t x^p    (s t = xx, xy, yy; p = 0..4)
+//       q 15..26  sum I g_s x^p, sum g_s x^p   (p = 0..2)
+//       q 27, 28  sum I, sum I^2
+//  2. One thread per (subset, q) sums a quantity down the subset's S rows
+//     times y^e for the powers the subset needs (e <= 4 - p, or 2 - p).
+//  3. One wave per subset turns the 71 moments into the Hessian, J^T f and
+//     sum J (J^T f = sum I J - mean sum J, energy = sum I^2 - mean sum I),
+//     then inverts as the direct kernel does.
+// Overlapping subsets (step 4, size 25) share the row sums, so the pixel work
+// per subset drops from S^2 products of 70 sums to about 5.3 S row products.
+constexpr int kStripRows = 16;
+constexpr int kStripQ = 29;
+constexpr int kStripMoments = 71;
+constexpr int kStripMaxRows = 160;  // image rows per strip (LDS bound)
+
+__host__ __device__ constexpr int strip_vpow(int q) { return q < 15 ? 4 - q % 5 : q < 27 ? 2 - (q - 15) % 3 : 0; }
+// Offset of quantity q's first moment (vertical power 0).
+__host__ __device__ constexpr int strip_offset(int q) {
+    return q < 15 ? (q / 5) * 15 + 5 * (q % 5) - (q % 5) * (q % 5 - 1) / 2
+                  : q < 27 ? 45 + ((q - 15) / 3) * 6 + 3 * ((q - 15) % 3) - ((q - 15) % 3) * ((q - 15) % 3 - 1) / 2
+                           : 69 + (q - 27);
+}
+static_assert(strip_offset(28) + 1 == kStripMoments, "moment count");
+
+__host__ __device__ inline int strip_subset_rows(int S, int D) {
+    const int fit = (kStripMaxRows - S) / D + 1;
+    return fit < 1 ? 1 : (fit > kStripRows ? kStripRows : fit);
+}
+
+__global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows_per_block) {
+    constexpr int P = 12, C = 24;
+    __shared__ double T[kStripQ * kStripMaxRows];
+    __shared__ double mom[kStripRows * kStripMoments];
+    __shared__ double mats[kPrepWaves][P * C];
+    const int strips_y = (a.grid_h + rows_per_block - 1) / rows_per_block;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int gx = blk / strips_y, gy0 = (blk - gx * strips_y) * rows_per_block;
+    if (gx >= a.grid_w) return;
+    const int rows = min(rows_per_block, a.grid_h - gy0);
+    const int S = a.S, D = a.D;
+    const int R = (rows - 1) * D + S;
+    const int ox = gx * D, oy0 = gy0 * D;
+    const double half = 0.5 * (double)(S - 1);
+
+    // 1. Row-segment sums.
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        double t[kStripQ];
+#pragma unroll
+        for (int q = 0; q < kStripQ; ++q) t[q] = 0.0;
+        const size_t row = (size_t)(oy0 + r) * a.width + ox;
+        for (int c = 0; c < S; ++c) {
+            const double I = a.intensity[row + c], gx_ = a.grad_x[row + c], gy_ = a.grad_y[row + c];
+            const double x = (double)c - half, x2 = x * x;
+            const double xp[5] = {1.0, x, x2, x2 * x, x2 * x2};
+            const double gg[3] = {gx_ * gx_, gx_ * gy_, gy_ * gy_};
+            const double gi[2] = {gx_ * I, gy_ * I}, g[2] = {gx_, gy_};
+#pragma unroll
+            for (int st = 0; st < 3; ++st)
+#pragma unroll
+                for (int p = 0; p < 5; ++p) t[st * 5 + p] += gg[st] * xp[p];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    t[15 + k * 3 + p] += gi[k] * xp[p];
+                    t[21 + k * 3 + p] += g[k] * xp[p];
+                }
+            t[27] += I;
+            t[28] += I * I;
+        }
+#pragma unroll
+        for (int q = 0; q < kStripQ; ++q) T[q * kStripMaxRows + r] = t[q];
+    }
+    __syncthreads();
+
+    // 2. Column sums per subset and quantity.
+    for (int i = threadIdx.x; i < rows * kStripQ; i += blockDim.x) {
+        const int j = i / kStripQ, q = i - j * kStripQ;
+        const double* col = T + q * kStripMaxRows + j * D;
+        double m[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int r = 0; r < S; ++r) {
+            const double v = col[r], y = (double)r - half, y2 = y * y;
+            m[0] += v;
+            m[1] += v * y;
+            m[2] += v * y2;
+            m[3] += v * (y2 * y);
+            m[4] += v * (y2 * y2);
+        }
+        const int E = strip_vpow(q), o = j * kStripMoments + strip_offset(q);
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+            if (e <= E) mom[o + e] = m[e];
+    }
+    __syncthreads();
+
+    // 3. One wave per subset: Hessian, first moments, inversion, outputs.
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* mat = mats[wave];
+    for (int j = wave; j < rows; j += kPrepWaves) {
+        const double* m = mom + j * kStripMoments;
+        const double sum_i = m[strip_offset(27)];
+        const double mean = sum_i / a.weight_sum;
+        const double energy = fmax(0.0, m[strip_offset(28)] - mean * sum_i);
+        double jtf[12], sj[12];
+        static_for<12>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int u = jac_mono(k), g = jac_grad(k);
+            constexpr int px = mono_px(u), py = mono_py(u);
+            const double sg = m[strip_offset(21 + g * 3 + px) + py];
+            const double si = m[strip_offset(15 + g * 3 + px) + py];
+            jtf[k] = mono_c(u) * (si - mean * sg);
+            sj[k] = mono_c(u) * sg;
+        });
+        wave_lds_sync();  // the previous subset's inverse has been read
+        for (int e = lane; e < P * P; e += kWave) {
+            const int i = e / P, k = e - i * P;
+            const int ui = jac_mono(i), uk = jac_mono(k);
+            const int st = jac_grad(i) + jac_grad(k);
+            const int px = mono_px(ui) + mono_px(uk), py = mono_py(ui) + mono_py(uk);
+            mat[i * C + k] = mono_c(ui) * mono_c(uk) * m[strip_offset(st * 5 + px) + py];
+        }
+        wave_lds_sync();
+        finish_subset<kQuadratic>(a, (gy0 + j) * a.grid_w + gx, lane, mat, energy, jtf, sj, mean);
+    }
+}
+
 hipError_t launch_subset_prep(const PrepArgs& a, hipStream_t s) {
     int N = a.grid_w * a.grid_h;
     if (N <= 0) return hipSuccess;
+    if (a.model == kQuadratic && !a.weighted && a.D < a.S && a.S <= kStripMaxRows) {
+        const int rows = strip_subset_rows(a.S, a.D);
+        const int strips = a.grid_w * ((a.grid_h + rows - 1) / rows);
+        k_subset_prep_strips<<<strips, 256, 0, s>>>(a, rows);
+        return hipGetLastError();
+    }
     dim3 grid((N + kPrepWaves - 1) / kPrepWaves);
     switch (a.model) {
         case kAffine: k_subset_prep<kAffine><<<grid, 256, 0, s>>>(a); break;
