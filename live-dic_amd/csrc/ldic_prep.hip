@@ -235,22 +235,31 @@ __device__ void quadratic_moments(const PrepArgs& a, int lane, int ox, int oy, d
     });
 }
 
-// Column-parallel Gauss-Jordan elimination with partial pivoting on the
-// P x 2P augmented matrix held in LDS.  Returns false for an exactly singular
-// matrix (a zero pivot), which is when LAPACK's LU factorisation fails too.
+// Column-parallel Gauss-Jordan elimination with partial pivoting of the
+// P x 2P augmented matrix [H | I] whose left half is in LDS (row pitch 2P).
+// Lane e < 2P keeps column e in registers; each pivot column is broadcast
+// with readlane, and every lane applies the row exchange, the division by
+// the pivot and the row updates to its own column.  The result is written
+// back to LDS.  Returns false for an exactly singular matrix (a zero pivot),
+// which is when LAPACK's LU factorisation fails too.
 template <int P>
 __device__ bool invert(double* m, int lane) {
     constexpr int C = 2 * P;
-    // m is P rows x C columns; columns P..2P-1 start as the identity.
-    if (lane < P)
-        for (int r = 0; r < P; ++r) m[r * C + P + lane] = r == lane ? 1.0 : 0.0;
-    wave_lds_sync();
+    const int e = lane < C ? lane : C - 1;
+    double col[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) col[r] = e < P ? m[r * C + e] : (r == e - P ? 1.0 : 0.0);
     bool singular = false;
-    for (int col = 0; col < P; ++col) {
-        int pr = col;
-        double ps = fabs(m[col * C + col]);
-        for (int cand = col + 1; cand < P; ++cand) {
-            double cs = fabs(m[cand * C + col]);
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+        double pc[P];
+#pragma unroll
+        for (int r = 0; r < P; ++r) pc[r] = readlane_f64(col[r], c);
+        int pr = c;
+        double ps = fabs(pc[c]);
+#pragma unroll
+        for (int cand = c + 1; cand < P; ++cand) {
+            const double cs = fabs(pc[cand]);
             if (cs > ps) {
                 pr = cand;
                 ps = cs;
@@ -260,26 +269,28 @@ __device__ bool invert(double* m, int lane) {
             singular = true;
             break;
         }
-        if (pr != col && lane < C) {
-            double x = m[col * C + lane];
-            m[col * C + lane] = m[pr * C + lane];
-            m[pr * C + lane] = x;
-        }
-        wave_lds_sync();
-        double pivot = m[col * C + col];
-        double factors[P];
 #pragma unroll
-        for (int r = 0; r < P; ++r) factors[r] = m[r * C + col];
-        wave_lds_sync();
-        if (lane < C) {
-            double top = m[col * C + lane] / pivot;
-            m[col * C + lane] = top;
-#pragma unroll
-            for (int r = 0; r < P; ++r)
-                if (r != col) m[r * C + lane] -= factors[r] * top;
+        for (int r = c + 1; r < P; ++r) {
+            const bool sw = r == pr;
+            double x = col[c], y = col[r];
+            col[c] = sw ? y : x;
+            col[r] = sw ? x : y;
+            x = pc[c];
+            y = pc[r];
+            pc[c] = sw ? y : x;
+            pc[r] = sw ? x : y;
         }
-        wave_lds_sync();
+        const double top = col[c] / pc[c];
+        col[c] = top;
+#pragma unroll
+        for (int r = 0; r < P; ++r)
+            if (r != c) col[r] -= pc[r] * top;
     }
+    wave_lds_sync();
+    if (lane < C)
+#pragma unroll
+        for (int r = 0; r < P; ++r) m[r * C + lane] = col[r];
+    wave_lds_sync();
     return !singular;
 }
 
@@ -419,9 +430,9 @@ __host__ __device__ inline int strip_subset_rows(int S, int D) {
     return fit < 1 ? 1 : (fit > kStripRows ? kStripRows : fit);
 }
 
-__global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows_per_block) {
+__global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows_per_block, int pitch) {
     constexpr int P = 12, C = 24;
-    __shared__ double T[kStripQ * kStripMaxRows];
+    extern __shared__ double T[];  // kStripQ x pitch row sums
     __shared__ double mom[kStripRows * kStripMoments];
     __shared__ double mats[kPrepWaves][P * C];
     const int strips_y = (a.grid_h + rows_per_block - 1) / rows_per_block;
@@ -440,8 +451,7 @@ __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows
 #pragma unroll
         for (int q = 0; q < kStripQ; ++q) t[q] = 0.0;
         const size_t row = (size_t)(oy0 + r) * a.width + ox;
-        for (int c = 0; c < S; ++c) {
-            const double I = a.intensity[row + c], gx_ = a.grad_x[row + c], gy_ = a.grad_y[row + c];
+        auto add = [&](double I, double gx_, double gy_, int c) {
             const double x = (double)c - half, x2 = x * x;
             const double xp[5] = {1.0, x, x2, x2 * x, x2 * x2};
             const double gg[3] = {gx_ * gx_, gx_ * gy_, gy_ * gy_};
@@ -459,17 +469,33 @@ __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows
                 }
             t[27] += I;
             t[28] += I * I;
-        }
+        };
+        // Five columns' loads are issued together before their products.
+        constexpr int B = 5;
+        int c = 0;
+        for (; c + B <= S; c += B) {
+            double I[B], gxv[B], gyv[B];
 #pragma unroll
-        for (int q = 0; q < kStripQ; ++q) T[q * kStripMaxRows + r] = t[q];
+            for (int b = 0; b < B; ++b) {
+                I[b] = a.intensity[row + c + b];
+                gxv[b] = a.grad_x[row + c + b];
+                gyv[b] = a.grad_y[row + c + b];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) add(I[b], gxv[b], gyv[b], c + b);
+        }
+        for (; c < S; ++c) add(a.intensity[row + c], a.grad_x[row + c], a.grad_y[row + c], c);
+#pragma unroll
+        for (int q = 0; q < kStripQ; ++q) T[q * pitch + r] = t[q];
     }
     __syncthreads();
 
     // 2. Column sums per subset and quantity.
     for (int i = threadIdx.x; i < rows * kStripQ; i += blockDim.x) {
         const int j = i / kStripQ, q = i - j * kStripQ;
-        const double* col = T + q * kStripMaxRows + j * D;
+        const double* col = T + q * pitch + j * D;
         double m[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 5
         for (int r = 0; r < S; ++r) {
             const double v = col[r], y = (double)r - half, y2 = y * y;
             m[0] += v;
@@ -523,7 +549,8 @@ hipError_t launch_subset_prep(const PrepArgs& a, hipStream_t s) {
     if (a.model == kQuadratic && !a.weighted && a.D < a.S && a.S <= kStripMaxRows) {
         const int rows = strip_subset_rows(a.S, a.D);
         const int strips = a.grid_w * ((a.grid_h + rows - 1) / rows);
-        k_subset_prep_strips<<<strips, 256, 0, s>>>(a, rows);
+        const int pitch = (rows - 1) * a.D + a.S;
+        k_subset_prep_strips<<<strips, 256, sizeof(double) * kStripQ * pitch, s>>>(a, rows, pitch);
         return hipGetLastError();
     }
     dim3 grid((N + kPrepWaves - 1) / kPrepWaves);
