@@ -9,7 +9,11 @@ enum Model : int { kAffine = 0, kTranslation = 1, kQuadratic = 2 };
 enum Interp : int { kBspline = 0, kLinear = 1, kKeys = 2 };
 enum Policy : int { kFlag = 0, kReplace = 1, kDisabled = 2, kMedian = 3 };
 
-constexpr int kWavesPerBlock = 4;   // subsets (waves) per solver workgroup
+constexpr int kWavesPerBlock = 4;   // subsets (waves) per persistent batched-solver workgroup
+// Subsets (waves) per k_solve workgroup.  One: a wave that converges early
+// frees its LDS slice and registers at once instead of holding them until the
+// slowest wave of its block has finished.
+constexpr int kSolveBlockWaves = 1;
 
 inline int param_count(int model) { return model == kQuadratic ? 12 : 6; }
 inline int state_stride(int model) { return model == kQuadratic ? 16 : 8; }

@@ -693,7 +693,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
     extern __shared__ float lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int item = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    const int item = xcd_block(blockIdx.x, gridDim.x) * kSolveBlockWaves + wave;
     if (a.partial_list) {
         // Partial-support pass over the list of partial subsets.
         if (item >= a.partial_count) return;
@@ -753,10 +753,10 @@ size_t solver_lds_bytes(int S, int model, int* tile_cap) {
         cap = pitch * (S + 17);
     } else {
         cap = (S + 12) * (S + 12);
-        if (cap > 4096) cap = 4096;  // 64 KB of LDS per four-wave block
+        if (cap > 4096) cap = 4096;  // 16 KB of LDS per wave
     }
     *tile_cap = cap;
-    return (size_t)cap * sizeof(float) * kWavesPerBlock;
+    return (size_t)cap * sizeof(float) * kSolveBlockWaves;
 }
 
 template <int MODEL, int INTERP, bool WEIGHTED>
@@ -766,10 +766,10 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     int cap = 0;
     size_t lds = solver_lds_bytes(a.S, MODEL, &cap);
     if (cap != a.tile_cap) return hipErrorInvalidValue;
-    dim3 grid((N + kWavesPerBlock - 1) / kWavesPerBlock);
+    dim3 grid((N + kSolveBlockWaves - 1) / kSolveBlockWaves), block(kWave * kSolveBlockWaves);
     switch (ppl) {
-        case 4: k_solve<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, 256, lds, s>>>(a); break;
-        default: k_solve<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, 256, lds, s>>>(a); break;
+        case 4: k_solve<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, block, lds, s>>>(a); break;
+        default: k_solve<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, block, lds, s>>>(a); break;
     }
     return hipGetLastError();
 }
