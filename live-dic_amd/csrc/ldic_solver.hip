@@ -363,7 +363,9 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             for (int k = 0; k < 6; ++k) q.C[k] = sd[6 + k];
     }
     const float meta1 = meta[1];
-    const bool partial = ordinal >= 0;
+    // Partial-support passes run the weighted variants only, so the per-pixel
+    // support test compiles away in the unweighted ones.
+    const bool partial = WEIGHTED && ordinal >= 0;
     float status = partial ? ((enabled && meta1 == -4.0f) ? 0.0f : -1.0f)
                            : ((enabled && meta1 >= 0.5f) ? 0.0f : (a.roi_enabled ? fminf(-1.0f, meta1) : -1.0f));
     float iters = 0.0f;
@@ -694,7 +696,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = xcd_block(blockIdx.x, gridDim.x) * kSolveBlockWaves + wave;
-    if (a.partial_list) {
+    if (WEIGHTED && a.partial_list) {
         // Partial-support pass over the list of partial subsets.
         if (item >= a.partial_count) return;
         solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, (int)a.partial_list[item], lane, lds + wave * a.tile_cap,
@@ -763,6 +765,7 @@ template <int MODEL, int INTERP, bool WEIGHTED>
 static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     int N = a.partial_list ? a.partial_count : a.grid_w * a.grid_h;
     if (N <= 0) return hipSuccess;
+    if (a.partial_list && !WEIGHTED) return hipErrorInvalidValue;  // partial passes use the weighted variant
     int cap = 0;
     size_t lds = solver_lds_bytes(a.S, MODEL, &cap);
     if (cap != a.tile_cap) return hipErrorInvalidValue;
