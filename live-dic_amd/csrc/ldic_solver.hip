@@ -552,6 +552,30 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 // Pixel p = lane + 64 j: the column, row, plane offset and local
                 // coordinates advance by (64 mod S, 64 div S) per step, with one
                 // carry, instead of a division per pixel.
+                if (MODEL != kQuadratic && !WEIGHTED && S >= 30 && S <= 32) {
+                    // One subset row per half-wave: lane 32h + c takes column c of
+                    // rows h, h + 2, ... (columns c >= S idle).  The taps a half-wave
+                    // reads then lie in one tile row and fall in distinct LDS banks;
+                    // with rows packed across the wave, the pixels of the next row
+                    // read one tile pitch further on and collide with a bank of
+                    // the current row.  For S = 30..32 this costs at most 7 % more
+                    // lane slots.
+                    const int c = lane & 31;
+                    const bool on = c < S;
+                    int r = lane >> 5;
+                    int off = (oy + r) * a.width + ox + (on ? c : 0);
+                    const float lxc = (float)c - he;
+                    float ly = (float)r - he;
+                    for (; r < S; r += 2) {
+                        if (on) {
+                            const float4 rv = a.reference[off];
+                            accumulate(sampler, lxc, ly, rv.x - ref_mean, rv.y, rv.z, 1.0f, 1.0f);
+                        }
+                        off += 2 * a.width;
+                        ly += 2.0f;
+                    }
+                    return;
+                }
                 int col = col0, off = off0;
                 float lx = lx0, ly = ly0;
                 for (int p = lane; p < SS; p += kWave) {
