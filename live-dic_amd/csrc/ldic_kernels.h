@@ -117,6 +117,8 @@ struct PrepArgs {
 
 // Pick the register-cached solver variant for a subset width.
 int solver_pixels_per_lane(int S, int model);
+// Compile-time LDS row pitch of the solver variant (0: the staged width).
+int solver_tile_pitch(int S, int model, int interp, int weighted);
 size_t solver_lds_bytes(int S, int model, int* tile_cap);
 
 hipError_t launch_reduce(const uint8_t* src, int w, int h, uint8_t* dst, hipStream_t s);
@@ -128,7 +130,7 @@ hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a,
 // block_start (npairs + 1 prefix sums of ceil(grid / 4)) are device arrays;
 // wave_stride is the LDS floats per wave (the largest tile_cap of the pairs).
 struct BatchLaunch {
-    int model, interp, weighted, ppl;
+    int model, interp, weighted, ppl, pitch;
     const SolveArgs* pairs;
     const int* block_start;
     int npairs, total, grid;

@@ -1632,11 +1632,13 @@ int batch_build(ldic_batch* b) {
             ldic_engine* e = b->engines[p];
             if (e->levels <= j) continue;
             const int ppl = solver_pixels_per_lane(e->opt.subset_size, e->opt.motion_model);
+            const int pitch = solver_tile_pitch(e->opt.subset_size, e->opt.motion_model, e->opt.interpolation,
+                                                e->opt.spatial_weighting);
             ldic_batch::Group* g = nullptr;
             for (auto& cand : groups) {
                 const BatchLaunch& l = cand.launch;
                 if (l.model == e->opt.motion_model && l.interp == e->opt.interpolation &&
-                    l.weighted == e->opt.spatial_weighting && l.ppl == ppl)
+                    l.weighted == e->opt.spatial_weighting && l.ppl == ppl && l.pitch == pitch)
                     g = &cand;
             }
             if (!g) {
@@ -1646,6 +1648,7 @@ int batch_build(ldic_batch* b) {
                 g->launch.interp = e->opt.interpolation;
                 g->launch.weighted = e->opt.spatial_weighting;
                 g->launch.ppl = ppl;
+                g->launch.pitch = pitch;
             }
             g->members.push_back((int)p);
         }

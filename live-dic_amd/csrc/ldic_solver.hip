@@ -478,6 +478,11 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                     nt.h = ry1 - ry0 + 1;
                 }
                 if (fits(nt)) {
+                    // Every lane computed the same box: keep it in scalar registers.
+                    nt.x0 = __builtin_amdgcn_readfirstlane(nt.x0);
+                    nt.y0 = __builtin_amdgcn_readfirstlane(nt.y0);
+                    nt.w = __builtin_amdgcn_readfirstlane(nt.w);
+                    nt.h = __builtin_amdgcn_readfirstlane(nt.h);
                     nt.valid = true;
                     t = nt;
                     stage_tile<INTERP, TP>(a, tile, t, lane);
@@ -596,8 +601,71 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 }
             }
         };
+        // One subset row per half-wave, B-spline taps from the LDS tile: the same
+        // values and operations as the row branch of `sweep` with `tile_sample`,
+        // scheduled for fewer instructions per pixel.
+        // - The floor split uses v_fract_f32 and v_cvt_flr_i32_f32 (exact for the
+        //   positive coordinates inside the tile).
+        // - With a compile-time pitch the 16 taps are immediate offsets from one
+        //   LDS address, formed by one 24-bit multiply-add from (ix, iy).
+        // - The trip count is wave-uniform: rows 2k + h for k < S/2 in both
+        //   half-waves, then row S - 1 in the first half-wave when S is odd.
+        auto row_sweep_tile = [&]() {
+            const int pitch = TP > 0 ? TP : t.w;
+            const int c = lane & 31, h = lane >> 5;
+            const bool on = c < S;
+            const float lxc = (float)c - he;
+            const float axl = fmaf(ux, lxc, q.T[0]);
+            const float* base = tile - (__mul24(t.y0 + 1, pitch) + t.x0 + 1);
+            const float4* rp = a.reference + ((size_t)(oy + h) * a.width + ox + (on ? c : 0));
+            const size_t rstep = 2 * (size_t)a.width;
+            float ly = (float)h - he;
+            auto pixel = [&]() {
+                const float4 rv = *rp;
+                const float px = cx0 + fmaf(q.L[1], ly, axl);
+                const float py = cy0 + fmaf(q.L[2], lxc, fmaf(vy, ly, q.T[1]));
+                int ix, iy;
+                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(ix) : "v"(px));
+                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iy) : "v"(py));
+                const float tx = __builtin_amdgcn_fractf(px), ty = __builtin_amdgcn_fractf(py);
+                float wx[4], wy[4];
+                bspline_weights6(tx, wx);
+                bspline_weights6(ty, wy);
+                const float* b = base + (__mul24(iy, pitch) + ix);
+                float cur;
+#pragma unroll
+                for (int yy = 0; yy < 4; ++yy) {
+                    const float* r = b + yy * pitch;
+                    float row = wx[0] * r[0];
+#pragma unroll
+                    for (int xx = 1; xx < 4; ++xx) row += wx[xx] * r[xx];
+                    cur = yy == 0 ? wy[0] * row : cur + wy[yy] * row;
+                }
+                const float f = rv.x - ref_mean;
+                const float gxg = rv.y * cur, gyg = rv.z * cur;
+                sc += cur;
+                sc2 += cur * cur;
+                src += f * cur;
+                sj[0] += gxg;
+                sj[3] += gyg;
+                if (MODEL != kTranslation) {
+                    sj[1] += gxg * lxc;
+                    sj[2] += gxg * ly;
+                    sj[4] += gyg * lxc;
+                    sj[5] += gyg * ly;
+                }
+            };
+            for (int k = 0; k < (S >> 1); ++k) {
+                if (on) pixel();
+                rp += rstep;
+                ly += 2.0f;
+            }
+            if ((S & 1) && on && h == 0) pixel();
+        };
         // The sampling path is chosen once per iteration (wave-uniform).
-        if (use_tile)
+        if (MODEL != kQuadratic && !WEIGHTED && INTERP == kBspline && S >= 30 && S <= 32 && use_tile)
+            row_sweep_tile();
+        else if (use_tile)
             sweep([&](float px, float py) { return tile_sample<INTERP, TP>(tile, t, px, py); });
         else
             sweep([&](float px, float py) { return global_sample<INTERP>(a, px, py); });
@@ -770,6 +838,17 @@ int solver_pixels_per_lane(int S, int model) {
 // Tile pitch of each register-cached variant (0: pitch = staged width).
 static int solver_pitch(int ppl) { return ppl == 4 ? 32 : 0; }
 
+// Row pitch of the uncached translation/affine B-spline variant for subsets
+// 30-32.  Their tile slice, (S + 12)^2 floats, then holds 40-44 rows of 44: the
+// zero-strain footprint with its re-staging margin is S + 9 wide and high.
+constexpr int kRowSweepPitch = 44;
+
+int solver_tile_pitch(int S, int model, int interp, int weighted) {
+    const int ppl = solver_pixels_per_lane(S, model);
+    if (ppl > 0) return solver_pitch(ppl);
+    return (model != kQuadratic && interp == kBspline && !weighted && S >= 30 && S <= 32) ? kRowSweepPitch : 0;
+}
+
 size_t solver_lds_bytes(int S, int model, int* tile_cap) {
     // Footprint of a moderately strained subset plus the interpolation halo
     // and the re-staging margin; larger footprints fall back to L2 gathers.
@@ -794,6 +873,13 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     size_t lds = solver_lds_bytes(a.S, MODEL, &cap);
     if (cap != a.tile_cap) return hipErrorInvalidValue;
     dim3 grid((N + kSolveBlockWaves - 1) / kSolveBlockWaves), block(kWave * kSolveBlockWaves);
+    if constexpr (MODEL != kQuadratic && INTERP == kBspline && !WEIGHTED) {
+        // Uncached subsets 30-32: fixed row pitch for the row sweep's taps.
+        if (solver_tile_pitch(a.S, MODEL, INTERP, WEIGHTED) == kRowSweepPitch) {
+            k_solve<MODEL, INTERP, 0, WEIGHTED, kRowSweepPitch><<<grid, block, lds, s>>>(a);
+            return hipGetLastError();
+        }
+    }
     switch (ppl) {
         case 4: k_solve<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, block, lds, s>>>(a); break;
         default: k_solve<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, block, lds, s>>>(a); break;
@@ -821,6 +907,14 @@ template <int MODEL, int INTERP, bool WEIGHTED>
 static hipError_t launch_batched_ppl(const BatchLaunch& b, hipStream_t s) {
     if (b.total <= 0) return hipSuccess;
     dim3 grid(b.grid), block(256);
+    if constexpr (MODEL != kQuadratic && INTERP == kBspline && !WEIGHTED) {
+        if (b.ppl == 0 && b.pitch == kRowSweepPitch) {
+            k_solve_batched<MODEL, INTERP, 0, WEIGHTED, kRowSweepPitch><<<grid, block, b.lds, s>>>(
+                b.pairs, b.block_start, b.npairs, b.total, b.wave_stride);
+            return hipGetLastError();
+        }
+    }
+    if (b.pitch != solver_pitch(b.ppl)) return hipErrorInvalidValue;
     switch (b.ppl) {
         case 4: k_solve_batched<MODEL, INTERP, 4, WEIGHTED, 32><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
         default: k_solve_batched<MODEL, INTERP, 0, WEIGHTED, 0><<<grid, block, b.lds, s>>>(b.pairs, b.block_start, b.npairs, b.total, b.wave_stride); break;
