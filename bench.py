@@ -330,6 +330,11 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=120)
     parser.add_argument("--warmup", type=int, default=10)
     parser.add_argument("--drain", type=int, default=32, help="frames queued between host waits")
+    parser.add_argument("--slots", type=int, default=2,
+                        help="engine slots (HIP streams) that consecutive frames alternate over")
+    parser.add_argument("--alone-frames", type=int, default=FRAMES - 1,
+                        help="frames queued on one slot after the timed region, for the solver's duration "
+                             "without a concurrent frame (the default covers frames 1-59 once)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-learned", action="store_true", help="skip the learned-vs-ZNCC seeding comparison")
     parser.add_argument("--no-batch", action="store_true", help="skip the batched heterogeneous workload")
@@ -380,36 +385,42 @@ def main() -> int:
     gw, gh = info.grid
     n_fine = gw * gh
     handle = eng._handle
+    if args.slots > 1:
+        _native.check(lib.ldic_set_slots(handle, args.slots))
 
-    def run(steps: int, start: int) -> tuple[float, int]:
-        """Queue frames on one stream; the host waits every `drain` frames.  Returns the
+    def run(steps: int, start: int, slots: int) -> tuple[float, int]:
+        """Queue frames round-robin on the first `slots` slots' streams (frames are
+        independent with a fixed reference, so one frame's small kernels and grid tails
+        overlap another's solves); the host waits every `drain` frames.  Returns the
         summed finest-level solver time (HIP events around each launch) and its count."""
         total, count = 0.0, 0
         for i in range(steps):
             k = 1 + (start + i) % (FRAMES - 1)
-            _native.check(lib.ldic_register_resident(handle, 0, frames[k].address, WIDTH))
+            _native.check(lib.ldic_register_resident(handle, (start + i) % slots, frames[k].address, WIDTH))
             if (i + 1) % args.drain == 0 or i + 1 == steps:
-                t = _native.Timing()
-                _native.check(lib.ldic_slot_wait(handle, 0, ctypes.byref(t)))
-                total += t.solve_ms
-                count += t.frames
+                for slot in range(slots):
+                    t = _native.Timing()
+                    _native.check(lib.ldic_slot_wait(handle, slot, ctypes.byref(t)))
+                    total += t.solve_ms
+                    count += t.frames
         return total, count
 
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     if args.warmup:
-        run(args.warmup, 0)
+        run(args.warmup, 0, args.slots)
     _native.check(lib.ldic_device_synchronize())
     print(f"[bench] timing {args.steps} frames", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solve_total, solve_count = run(args.steps, args.warmup)
+    solve_total, solve_count = run(args.steps, args.warmup, args.slots)
     _native.check(lib.ldic_device_synchronize())
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
 
     # Accuracy of the last timed frame against the analytic field.
     last = 1 + (args.warmup + args.steps - 1) % (FRAMES - 1)
+    last_slot = (args.warmup + args.steps - 1) % args.slots
     out = np.empty((gh, gw, 8), np.float32)
-    _native.check(lib.ldic_slot_output(handle, 0, out.ctypes.data))
+    _native.check(lib.ldic_slot_output(handle, last_slot, out.ctypes.data))
     cx, cy = synthetic.subset_centres(WIDTH, HEIGHT, SUBSET, STRIDE)
     u, v = seq.field.scaled(last / (FRAMES - 1)).displacement(cx, cy)
     ok = out[..., 6] > 0.5
@@ -418,14 +429,17 @@ def main() -> int:
     for level in reversed(range(LEVELS)):
         lgw, lgh = eng._level_shapes[level][2:4]
         st = np.empty((lgh, lgw, 8), np.float32)
-        _native.check(lib.ldic_export_level_state(handle, 0, level, st.ctypes.data))
+        _native.check(lib.ldic_export_level_state(handle, last_slot, level, st.ctypes.data))
         acc = st[..., 6] > 0.5
         mean_iterations.append(float(st[..., 7][acc].mean()) if acc.any() else None)
+    # The same kernel with no concurrent frame (one slot), after the timed region.
+    alone_total, alone_count = run(args.alone_frames, 0, 1) if args.alone_frames > 0 else (0.0, 0)
     eng.close()
 
     avg_solve_ms = solve_total / solve_count if solve_count else float("nan")
     traffic_bytes = n_fine * subset_bytes(SUBSET)
     achieved = traffic_bytes / (avg_solve_ms * 1e-3) / 1e9
+    alone_ms = alone_total / alone_count if alone_count else None
     line = {
         "metric": METRIC,
         "value": fps,
@@ -451,6 +465,7 @@ def main() -> int:
             "mean_iterations_per_level": mean_iterations,
             "tolerance_per_level": list(TOLERANCES),
             "solver_launches_timed": solve_count,
+            "slots": args.slots,
         },
         "roofline": {
             "bound": "hbm",
@@ -462,6 +477,10 @@ def main() -> int:
             "kernel": "k_solve<affine, bspline> (finest level)",
             "kernel_avg_ms": avg_solve_ms,
             "algorithmic_bytes_per_launch": traffic_bytes,
+            # The timed region runs two frames concurrently, so each launch shares the GPU;
+            # the same kernel on one stream, measured after the timed region:
+            "kernel_avg_ms_alone": alone_ms,
+            "frac_alone": traffic_bytes / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if alone_ms else None,
         },
         "cpu_baseline": None,
     }

@@ -310,8 +310,33 @@ __device__ __forceinline__ float increment_change(const float* d, float he) {
 // Stage the target footprint [x0, x0+w) x [y0, y0+h) of the wave's LDS slice.
 template <int INTERP, int TP>
 __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, const Tile& t, int lane) {
-    const int n = t.w * t.h;
     const int pitch = TP > 0 ? TP : t.w;
+    auto load = [&](size_t g) -> float { return INTERP == kBspline ? a.coeff[g] : (float)a.target[g]; };
+    if (t.w <= 2 * 32) {
+        // Boxes up to 64 wide: half-wave h copies rows h, h + 2, ... (then the last
+        // row in half-wave 0 when the height is odd); lane c copies columns c and
+        // c + 32, so a row costs two address increments instead of a division and
+        // 64-bit index arithmetic per element.
+        const int c = lane & 31, h = lane >> 5;
+        const bool lo = c < t.w, hi = c + 32 < t.w;
+        const size_t step = 2 * (size_t)a.width;
+        size_t g = (size_t)(t.y0 + h) * a.width + (t.x0 + c);
+        float* d = tile + (h * pitch + c);
+#pragma unroll 4
+        for (int k = 0; k < (t.h >> 1); ++k) {
+            if (lo) d[0] = load(g);
+            if (hi) d[32] = load(g + 32);
+            g += step;
+            d += 2 * pitch;
+        }
+        if ((t.h & 1) && h == 0) {
+            if (lo) d[0] = load(g);
+            if (hi) d[32] = load(g + 32);
+        }
+        wave_lds_sync();
+        return;
+    }
+    const int n = t.w * t.h;
     const float inv_w = 1.0f / (float)t.w;
     for (int i = lane; i < n; i += kWave) {
         int r = (int)(((float)i + 0.5f) * inv_w);
