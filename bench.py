@@ -32,8 +32,10 @@ METRIC = "4K frames/s + IC-GN subsets/s; learned vs ZNCC seed at ≤0.01 px RMS 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes of one fine-level k_solve launch of this workload, measured with rocprofv3
 # FETCH_SIZE/WRITE_SIZE and scaled by a 256 MiB copy calibration (reads 2.0x FETCH_SIZE,
-# writes 1.0x WRITE_SIZE): profiles/k_solve_affine_4k_s31_d8_pmc.md.
-MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 101_830 + 4_073) * 1024
+# writes 1.0x WRITE_SIZE), and its L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS):
+# profiles/k_solve_affine_4k_s31_d8_pmc.md.
+MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_898 + 4_191) * 1024
+MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.317e7 / (4.317e7 + 1.661e6)
 SUBSET, STRIDE = 31, 8
 WIDTH, HEIGHT = 3840, 2160
 FRAMES = 60
@@ -474,6 +476,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": MEASURED_FINE_SOLVE_TRAFFIC_BYTES,
+            "l2_hit_rate": MEASURED_FINE_SOLVE_L2_HIT_RATE,
             "kernel": "k_solve<affine, bspline> (finest level)",
             "kernel_avg_ms": avg_solve_ms,
             "algorithmic_bytes_per_launch": traffic_bytes,
