@@ -44,14 +44,22 @@ struct Tile {
     bool valid;
 };
 
+// Floor split of a positive coordinate: v_cvt_flr_i32_f32 and v_fract_f32 give
+// (int)floor(x) and x - floor(x) exactly.
+__device__ __forceinline__ void floor_split(float x, int& i, float& f) {
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    f = __builtin_amdgcn_fractf(x);
+}
+
 // TP > 0: compile-time row pitch, so the 16 taps are immediate offsets from one
 // per-pixel LDS address; TP == 0: the pitch is the staged box width.
 template <int INTERP, int TP>
 __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, float px, float py) {
     const int pitch = TP > 0 ? TP : t.w;
-    float fx = floorf(px), fy = floorf(py);
-    int ix = (int)fx, iy = (int)fy;
-    float tx = px - fx, ty = py - fy;
+    int ix, iy;
+    float tx, ty;
+    floor_split(px, ix, tx);
+    floor_split(py, iy, ty);
     if (INTERP == kLinear) {
         const float* b = tile + (__mul24(iy - t.y0, pitch) + (ix - t.x0));
         float top = b[0] * (1.0f - tx) + b[1] * tx;
@@ -650,9 +658,9 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 const float px = cx0 + fmaf(q.L[1], ly, axl);
                 const float py = cy0 + fmaf(q.L[2], lxc, fmaf(vy, ly, q.T[1]));
                 int ix, iy;
-                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(ix) : "v"(px));
-                asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iy) : "v"(py));
-                const float tx = __builtin_amdgcn_fractf(px), ty = __builtin_amdgcn_fractf(py);
+                float tx, ty;
+                floor_split(px, ix, tx);
+                floor_split(py, iy, ty);
                 float wx[4], wy[4];
                 bspline_weights6(tx, wx);
                 bspline_weights6(ty, wy);
