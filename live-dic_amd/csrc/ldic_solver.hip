@@ -804,13 +804,13 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
 //   3 waves, 168 VGPRs;
 // - uncached translation/affine: 5 waves, 96 VGPRs, as many as the LDS slice
 //   of a subset-31 footprint allows (98 VGPRs would leave 4);
-// - uncached quadratic: 4 waves, 128 VGPRs (the persistent batched kernel
-//   would spill there and keeps its own allocation).
+// - uncached quadratic: 4 waves, 128 VGPRs, also in the persistent batched
+//   kernel.
 // None of these spill.  Keys and Gaussian-weighted variants keep the compiler's
 // own allocation.
 constexpr int solve_min_waves(int model, int interp, int ppl, bool weighted, bool batched = false) {
     if (interp == kKeys || weighted) return 1;
-    if (ppl == 0) return model == kQuadratic ? (batched ? 1 : 4) : 5;
+    if (ppl == 0) return model == kQuadratic ? 4 : 5;
     return (model != kQuadratic && ppl <= 16) ? 3 : 1;
 }
 
@@ -890,7 +890,10 @@ size_t solver_lds_bytes(int S, int model, int* tile_cap) {
     if (pitch > 0) {
         cap = pitch * (S + 17);
     } else {
-        cap = (S + 12) * (S + 12);
+        // Up to subset 32, room for strain beyond the margin; above it the
+        // footprint with its margin (S + 9 square) only, so a 41 x 41 subset's
+        // 10 KB slice leaves room for 4 waves per SIMD (16 per CU).
+        cap = S <= 32 ? (S + 12) * (S + 12) : (S + 9) * (S + 9);
         if (cap > 4096) cap = 4096;  // 16 KB of LDS per wave
     }
     *tile_cap = cap;
