@@ -1,12 +1,15 @@
 // Inverse-compositional Gauss-Newton subset solver for gfx950.
 //
-// One wave64 owns one subset (four independent waves per workgroup, no block
-// barriers).  The reference tile (f, gx, gy) of the subset is loaded once into
-// VGPRs; the target footprint around the current warp is staged in the wave's
-// LDS slice and re-staged only when the warp leaves it, so the per-iteration
-// B-spline gathers (16 per pixel) hit LDS instead of L2.  Sums are reduced with
-// DPP inside the wave and every lane evaluates the (wave-uniform) increment,
-// composition and convergence test, so no lane waits on another.
+// One wave64 owns one subset: one wave per workgroup in k_solve, four
+// independent waves per workgroup in the persistent k_solve_batched, no block
+// barriers.  Translation/affine subsets up to 16x16 keep their reference tile
+// (f, gx, gy) in VGPRs; larger ones and every quadratic subset re-read it from
+// L1/L2 each iteration.  The target footprint around the current warp is
+// staged in the wave's LDS slice and re-staged only when the warp leaves it, so
+// the per-iteration B-spline gathers (16 per pixel) hit LDS instead of L2.
+// Sums are reduced with DPP inside the wave and every lane evaluates the
+// (wave-uniform) increment, composition and convergence test, so no lane waits
+// on another.
 //
 // Equations: shaders/icgn.wgsl:216-958 (sampling 230-309, increments 321-384,
 // compositions 386-549, change metrics 551-593, solve loop 595-958).
