@@ -652,7 +652,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             const bool on = c < S;
             const float lxc = (float)c - he;
             const float axl = fmaf(ux, lxc, q.T[0]);
-            const float* base = tile - (__mul24(t.y0 + 1, pitch) + t.x0 + 1);
+            const int bias = __mul24(t.y0 + 1, pitch) + t.x0 + 1;  // tile offset of (ix, iy) = iy * pitch + ix - bias
             const float4* rp = a.reference + ((size_t)(oy + h) * a.width + ox + (on ? c : 0));
             const size_t rstep = 2 * (size_t)a.width;
             float ly = (float)h - he;
@@ -667,7 +667,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 float wx[4], wy[4];
                 bspline_weights6(tx, wx);
                 bspline_weights6(ty, wy);
-                const float* b = base + (__mul24(iy, pitch) + ix);
+                const float* b = tile + (__mul24(iy, pitch) + ix - bias);
                 float cur;
 #pragma unroll
                 for (int yy = 0; yy < 4; ++yy) {
