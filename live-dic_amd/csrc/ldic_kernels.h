@@ -9,7 +9,7 @@ enum Model : int { kAffine = 0, kTranslation = 1, kQuadratic = 2 };
 enum Interp : int { kBspline = 0, kLinear = 1, kKeys = 2 };
 enum Policy : int { kFlag = 0, kReplace = 1, kDisabled = 2, kMedian = 3 };
 
-constexpr int kWavesPerBlock = 4;   // subsets (waves) per persistent batched-solver workgroup
+constexpr int kWavesPerBlock = 4;   // most subsets (waves) per persistent batched-solver workgroup
 // Subsets (waves) per k_solve workgroup.  One: a wave that converges early
 // frees its LDS slice and registers at once instead of holding them until the
 // slowest wave of its block has finished.
@@ -131,6 +131,7 @@ hipError_t launch_solve(int model, int interp, int weighted, const SolveArgs& a,
 // wave_stride is the LDS floats per wave (the largest tile_cap of the pairs).
 struct BatchLaunch {
     int model, interp, weighted, ppl, pitch;
+    int waves;  // subsets (waves) per workgroup, 1, 2 or 4; block_start counts blocks of this many
     const SolveArgs* pairs;
     const int* block_start;
     int npairs, total, grid;

@@ -1533,7 +1533,10 @@ struct ldic_batch {
     struct Group {
         BatchLaunch launch{};
         std::vector<int> members;
-        std::vector<int> candidates;   // grid sizes tried by the autotuner
+        // Block prefix sums for 1, 2 and 4 subsets per workgroup (index log2(waves)).
+        int* block_start[3] = {nullptr, nullptr, nullptr};
+        int total[3] = {0, 0, 0};
+        std::vector<std::pair<int, int>> candidates;   // (waves per workgroup, grid) tried by the autotuner
         std::vector<float> timings;
     };
     std::vector<ldic_engine*> engines;
@@ -1621,6 +1624,18 @@ int batch_enqueue(ldic_batch* b) {
     return LDIC_OK;
 }
 
+// Point a group's launch at the block prefix sums of `waves` subsets per
+// workgroup, with `grid` persistent workgroups.
+void batch_shape(ldic_batch::Group& g, int waves, int grid) {
+    const int k = waves == 1 ? 0 : waves == 2 ? 1 : 2;
+    BatchLaunch& l = g.launch;
+    l.waves = waves;
+    l.block_start = g.block_start[k];
+    l.total = g.total[k];
+    l.lds = (size_t)l.wave_stride * sizeof(float) * waves;
+    l.grid = grid;
+}
+
 int batch_build(ldic_batch* b) {
     b->max_levels = 0;
     for (ldic_engine* e : b->engines) b->max_levels = std::max(b->max_levels, e->levels);
@@ -1654,35 +1669,46 @@ int batch_build(ldic_batch* b) {
         }
         for (ldic_batch::Group& g : groups) {
             std::vector<SolveArgs> args;
-            std::vector<int> start(1, 0);
+            std::vector<int> start[3];
+            for (auto& v : start) v.assign(1, 0);
             int cap = 0;
             for (int p : g.members) {
                 ldic_engine* e = b->engines[p];
                 SlotLevel& sl = e->slots[0]->lv[j];
                 const bool seeded = j < e->levels - 1 || e->init.kind == LDIC_INIT_ZNCC;
                 args.push_back(solve_args(e, j, sl, seeded ? sl.seed : nullptr, seeded ? sl.active : nullptr, 0));
-                start.push_back(start.back() + (e->lv[j].count() + kWavesPerBlock - 1) / kWavesPerBlock);
+                for (int k = 0; k < 3; ++k) {
+                    const int w = 1 << k;
+                    start[k].push_back(start[k].back() + (e->lv[j].count() + w - 1) / w);
+                }
                 cap = std::max(cap, e->tile_cap);
             }
             SolveArgs* d_args = nullptr;
-            int* d_start = nullptr;
             CHECK(b->arena.alloc(&d_args, args.size()));
-            CHECK(b->arena.alloc(&d_start, start.size()));
             HIP_CHECK(hipMemcpy(d_args, args.data(), sizeof(SolveArgs) * args.size(), hipMemcpyHostToDevice));
-            HIP_CHECK(hipMemcpy(d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+            for (int k = 0; k < 3; ++k) {
+                CHECK(b->arena.alloc(&g.block_start[k], start[k].size()));
+                HIP_CHECK(hipMemcpy(g.block_start[k], start[k].data(), sizeof(int) * start[k].size(),
+                                    hipMemcpyHostToDevice));
+                g.total[k] = start[k].back();
+            }
             BatchLaunch& l = g.launch;
             l.pairs = d_args;
-            l.block_start = d_start;
             l.npairs = (int)args.size();
-            l.total = start.back();
             l.wave_stride = cap;
-            l.lds = (size_t)cap * sizeof(float) * kWavesPerBlock;
-            l.grid = l.total;
+            batch_shape(g, kWavesPerBlock, g.total[2]);
+            // Candidates: 1, 2 or 4 waves per workgroup, each with grids of 1-8 full
+            // waves of 4-wave-equivalent workgroups per CU and the plain grid.
             g.candidates.clear();
             const int cus = b->engines[0]->prop.multiProcessorCount;
-            for (int per : {1, 2, 3, 4, 8})
-                if (cus * per < l.total) g.candidates.push_back(cus * per);
-            g.candidates.push_back(l.total);
+            for (int k = 2; k >= 0; --k) {
+                const int w = 1 << k;
+                for (int per : {1, 2, 3, 4, 8}) {
+                    const int grid = cus * per * (kWavesPerBlock / w);
+                    if (grid < g.total[k]) g.candidates.emplace_back(w, grid);
+                }
+                g.candidates.emplace_back(w, g.total[k]);
+            }
         }
     }
     return LDIC_OK;
@@ -1698,13 +1724,12 @@ int batch_tune(ldic_batch* b) {
         for (ldic_batch::Group& g : b->steps[j]) {
             g.timings.clear();
             float best = 1e30f;
-            int best_grid = g.launch.total;
-            for (int grid : g.candidates) {
-                BatchLaunch l = g.launch;
-                l.grid = grid;
-                HIP_CHECK(launch_solve_batched(l, b->stream));  // warm
+            std::pair<int, int> best_shape(g.launch.waves, g.launch.grid);
+            for (const auto& cand : g.candidates) {
+                batch_shape(g, cand.first, cand.second);
+                HIP_CHECK(launch_solve_batched(g.launch, b->stream));  // warm
                 HIP_CHECK(hipEventRecord(b->ev0, b->stream));
-                HIP_CHECK(launch_solve_batched(l, b->stream));
+                HIP_CHECK(launch_solve_batched(g.launch, b->stream));
                 HIP_CHECK(hipEventRecord(b->ev1, b->stream));
                 HIP_CHECK(hipEventSynchronize(b->ev1));
                 float ms = 0.0f;
@@ -1712,10 +1737,10 @@ int batch_tune(ldic_batch* b) {
                 g.timings.push_back(ms);
                 if (ms < best) {
                     best = ms;
-                    best_grid = grid;
+                    best_shape = cand;
                 }
             }
-            g.launch.grid = best_grid;
+            batch_shape(g, best_shape.first, best_shape.second);
         }
     }
     return LDIC_OK;
@@ -1823,18 +1848,19 @@ int ldic_batch_describe(ldic_batch* b, char* buffer, int32_t capacity) {
     std::string text = "[";
     for (int j = b->max_levels - 1; j >= 0; --j)
         for (const ldic_batch::Group& g : b->steps[j]) {
-            char item[512];
+            char item[4096];
             std::string times;
             for (size_t i = 0; i < g.timings.size(); ++i) {
-                char t[48];
-                std::snprintf(t, sizeof t, "%s[%d,%.4f]", i ? "," : "", g.candidates[i], g.timings[i]);
+                char t[64];
+                std::snprintf(t, sizeof t, "%s[%d,%d,%.4f]", i ? "," : "", g.candidates[i].first,
+                              g.candidates[i].second, g.timings[i]);
                 times += t;
             }
             std::snprintf(item, sizeof item,
-                          "%s{\"level\":%d,\"model\":%d,\"ppl\":%d,\"pairs\":%d,\"blocks\":%d,\"grid\":%d,"
-                          "\"candidates_ms\":[%s]}",
-                          text.size() > 1 ? "," : "", j, g.launch.model, g.launch.ppl, g.launch.npairs, g.launch.total,
-                          g.launch.grid, times.c_str());
+                          "%s{\"level\":%d,\"model\":%d,\"ppl\":%d,\"pairs\":%d,\"waves\":%d,\"blocks\":%d,"
+                          "\"grid\":%d,\"candidates_ms\":[%s]}",
+                          text.size() > 1 ? "," : "", j, g.launch.model, g.launch.ppl, g.launch.npairs, g.launch.waves,
+                          g.launch.total, g.launch.grid, times.c_str());
             text += item;
         }
     text += "]";

@@ -837,7 +837,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
 
 // Persistent solve over the subset grids of several image pairs (one variant
 // of model, interpolant, register bucket and weighting).  Work item i is a
-// block of kWavesPerBlock subsets of pair p, block_start[p] <= i <
+// block of (workgroup waves) subsets of pair p, block_start[p] <= i <
 // block_start[p + 1]; each workgroup strides over the items, the XCD mapping
 // giving every XCD contiguous runs of them.  Waves never synchronise with each
 // other, so a wave moves to its next item as soon as its subset is done.
@@ -850,12 +850,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 256),
     extern __shared__ float lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int waves = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
     float* tile = lds + wave * wave_stride;
     for (int item = xcd_block(blockIdx.x, gridDim.x); item < total; item += gridDim.x) {
         int p = 0;
         while (p + 1 < npairs && block_start[p + 1] <= item) ++p;
         const SolveArgs a = pairs[p];
-        const int subset = (item - block_start[p]) * kWavesPerBlock + wave;
+        const int subset = (item - block_start[p]) * waves + wave;
         if (subset < a.grid_w * a.grid_h) solve_subset<MODEL, INTERP, PPL, WEIGHTED, TP>(a, subset, lane, tile);
     }
 }
@@ -945,7 +946,8 @@ static hipError_t launch_i(int interp, int weighted, const SolveArgs& a, hipStre
 template <int MODEL, int INTERP, bool WEIGHTED>
 static hipError_t launch_batched_ppl(const BatchLaunch& b, hipStream_t s) {
     if (b.total <= 0) return hipSuccess;
-    dim3 grid(b.grid), block(256);
+    if (b.waves != 1 && b.waves != 2 && b.waves != 4) return hipErrorInvalidValue;
+    dim3 grid(b.grid), block(kWave * b.waves);
     if constexpr (MODEL != kQuadratic && INTERP == kBspline && !WEIGHTED) {
         if (b.ppl == 0 && b.pitch == kRowSweepPitch) {
             k_solve_batched<MODEL, INTERP, 0, WEIGHTED, kRowSweepPitch><<<grid, block, b.lds, s>>>(
