@@ -86,6 +86,7 @@ LARGE_LEVELS = 4                       # ZNCC path: 1/8-resolution search covers
 LARGE_SEARCH_RADIUS = 504              # finest-level pixels (coarsest-level radius 64)
 LARGE_ITERATIONS = (12, 8, 8, 16)
 LARGE_RESEED = 2                        # neighbour re-seeding passes after the finest solve, both paths
+LARGE_SLOTS = 2                         # frames in flight (engine slots), both paths
 
 
 def _large_targets(count: int, seed: int = 5150):
@@ -141,41 +142,56 @@ def learned_vs_zncc(reps: int = 3) -> dict:
                           "U[-10,10] deg, strains U[-3%,3%] about the centre, noise 2; affine IC-GN subset 31 "
                           "step 8; reference prepared once; targets resident in HBM"}
 
-    def timed(submit, handle, out_shape):
+    def timed(submits, handle, out_shape):
+        """submits[k](target) queues one frame on slot k.  Accuracy from slot 0 frame by frame;
+        throughput with consecutive frames alternating over the slots (frames are independent
+        with a fixed reference, so one frame's small kernels overlap another's solves)."""
         states = []
         for tgt, _ in targets:                          # warm-up and accuracy
-            submit(tgt)
+            submits[0](tgt)
             _native.check(lib.ldic_slot_wait(handle, 0, None))
             st = np.empty(out_shape, np.float32)
             _native.check(lib.ldic_slot_output(handle, 0, st.ctypes.data))
             states.append(st)
+        for k in range(1, len(submits)):                # warm-up of the other slots
+            submits[k](targets[0][0])
+            _native.check(lib.ldic_slot_wait(handle, k, None))
         _native.check(lib.ldic_device_synchronize())
         t0 = time.perf_counter()
+        i = 0
         for _ in range(reps):
             for tgt, _ in targets:
-                submit(tgt)
-        _native.check(lib.ldic_slot_wait(handle, 0, None))
+                submits[i % len(submits)](tgt)
+                i += 1
+        for k in range(len(submits)):
+            _native.check(lib.ldic_slot_wait(handle, k, None))
         _native.check(lib.ldic_device_synchronize())
         rate = reps * len(targets) / (time.perf_counter() - t0)
-        return {"frames_per_s": rate, **_accuracy(states, targets)}
+        return {"frames_per_s": rate, "slots": len(submits), **_accuracy(states, targets)}
 
     zeng = ICGNWGPUFullPyramid(pyramid_levels=LARGE_LEVELS, iterations_per_level=LARGE_ITERATIONS,
                                subset_size=SUBSET, subset_stride=STRIDE, initialization="zncc",
                                seed_search_radius=LARGE_SEARCH_RADIUS, reseed_passes=LARGE_RESEED)
     info = zeng.set_reference(host)
     gw, gh = info.grid
-    zncc = timed(lambda t: _native.check(lib.ldic_register_resident(zeng._handle, 0, t.address, WIDTH)),
-                 zeng._handle, (gh, gw, 8))
+    _native.check(lib.ldic_set_slots(zeng._handle, LARGE_SLOTS))
+    zncc = timed([lambda t, k=k: _native.check(lib.ldic_register_resident(zeng._handle, k, t.address, WIDTH))
+                  for k in range(LARGE_SLOTS)], zeng._handle, (gh, gw, 8))
     zeng.close()
 
     # The finest-level solve uses the ZNCC path's finest-level controls (16 iterations, 5e-3).
+    # Each slot has its own network instance (activation buffers) and seeder on its stream.
     leng = ICGNWGPU(subset_size=SUBSET, subset_stride=STRIDE)
     leng.set_reference(host, iterations=LARGE_ITERATIONS[-1], tolerance=5e-3)
-    net = seednet.SeedNet()
-    net.load(seednet.WEIGHTS_PATH)
-    seeder = seednet.LearnedSeeder(leng, net, reseed_passes=LARGE_RESEED)
-    seeder.set_reference(ref.address)
-    learned = timed(lambda t: seeder.submit(t.address), leng._handle, (gh, gw, 8))
+    _native.check(lib.ldic_set_slots(leng._handle, LARGE_SLOTS))
+    seeders = []
+    for k in range(LARGE_SLOTS):
+        net = seednet.SeedNet()
+        net.load(seednet.WEIGHTS_PATH)
+        seeders.append(seednet.LearnedSeeder(leng, net, slot=k, reseed_passes=LARGE_RESEED))
+        seeders[-1].set_reference(ref.address)
+    seeder = seeders[0]
+    learned = timed([lambda t, s=s: s.submit(t.address) for s in seeders], leng._handle, (gh, gw, 8))
     leng.close()
     zncc["path"] = f"{LARGE_LEVELS}-level pyramid {list(LARGE_ITERATIONS)}, coarsest-level ZNCC search " \
                    f"+-{LARGE_SEARCH_RADIUS // 8 + 1} px, {LARGE_RESEED} neighbour re-seeding passes"
