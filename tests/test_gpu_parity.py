@@ -133,7 +133,12 @@ def test_small_affine_accuracy_against_analytic_field(small_pair):
     assert rmse <= 0.01
 
 
-def test_uhd_golden_statuses_and_oracle():
+def test_uhd_goldens_and_oracle():
+    """The tensile pair at UHD, 16/16, one level, against the WGSL capture (before.npz) and the oracle.
+
+    The capture's "median" policy filtered every subset (the present median_filter_scope="all"), so
+    the engine runs with that scope for the golden comparison; the raw solve (outlier pass disabled)
+    is compared with the oracle's."""
     from live_dic import ICGNWGPUFullPyramid
 
     ref = load_gray("uhd_reference.png")
@@ -141,24 +146,31 @@ def test_uhd_golden_statuses_and_oracle():
     goldens = np.load(oracle_golden_path(), allow_pickle=False)
     for model in ("affine", "quadratic"):
         _say("uhd golden", model, "l1")
-        with ICGNWGPUFullPyramid(pyramid_levels=1, motion_model=model, outlier_policy="median") as eng:
-            eng.set_reference(ref)
-            res = eng.register(tgt)
         P = 12 if model == "quadratic" else 6
-        raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+        fields = {}
+        for policy, scope in (("median", "all"), ("disabled", "rejected")):
+            with ICGNWGPUFullPyramid(pyramid_levels=1, motion_model=model, outlier_policy=policy,
+                                     median_filter_scope=scope) as eng:
+                eng.set_reference(ref)
+                res = eng.register(tgt)
+            fields[policy] = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]],
+                                            axis=-1)
         key = f"{model}-l1-none-zero-f1"
         golden = np.concatenate([goldens[key + "-parameters"], goldens[key + "-status"][..., None],
                                  goldens[key + "-iterations"][..., None]], axis=-1)
-        stats = compare_fields(raw, golden, P)
+        stats = compare_fields(fields["median"], golden, P)
         print("vs WGSL golden", stats)
         assert stats["status_equal"] >= 0.999
-        want = _oracle_pipeline(ref, pyramid_levels=1, motion_model=model, outlier_policy="median").register(tgt)
-        stats = compare_fields(raw, want[..., : P + 2], P)
-        print("vs oracle", stats)
+        # Quadratic: 1.4 % of subsets stop one iteration apart at the 5e-3 tolerance in any two
+        # summation orders, and the all-subsets median spreads each to its neighbours.
+        assert stats["disp_within_1e3"] >= (0.999 if model == "affine" else 0.98)
+        want = _oracle_pipeline(ref, pyramid_levels=1, motion_model=model, outlier_policy="disabled").register(tgt)
+        stats = compare_fields(fields["disabled"], want[..., : P + 2], P)
+        print("raw vs oracle", stats)
         assert stats["status_equal"] >= 0.999
-        # The quadratic solve at the default 5e-3 tolerance is sensitive to summation
-        # order: the oracle itself with 64 instead of 16 lanes keeps |du| <= 1e-3 on
-        # 99.895 % of accepted subsets (and 99.904 % with 32 lanes).
+        # The quadratic solve at the default 5e-3 tolerance is sensitive to summation order: the
+        # oracle itself with 64 instead of 16 lanes keeps |du| <= 1e-3 on 99.895 % of accepted
+        # subsets (tests/test_oracle.py::test_quadratic_uhd_summation_order_sensitivity).
         assert stats["disp_within_1e3"] >= (0.999 if model == "affine" else 0.998)
 
 
@@ -231,7 +243,7 @@ def _uhd_mask():
                                                 ("translation-l1-partial-zero", "translation", 1),
                                                 ("affine-l2-partial-zero", "affine", 2)])
 def test_uhd_roi_goldens(key, model, levels):
-    """The partial cases of this capture have no partial subsets: at 16/16 every window cut by the
+    """ROI goldens of the capture (median over every subset, as above).  The partial cases of this capture have no partial subsets: at 16/16 every window cut by the
     mask's edges has its even centre on an edge, so they pin that the partial policy without
     admissible partial windows gives the full-policy field."""
     from live_dic import ICGNWGPUFullPyramid
@@ -240,7 +252,8 @@ def test_uhd_roi_goldens(key, model, levels):
     tgt = load_gray("uhd_frame_040.png") if model != "translation" else np.roll(ref, (-1, 2), (0, 1))
     goldens = np.load(oracle_golden_path(), allow_pickle=False)
     _say("uhd roi golden", key)
-    with ICGNWGPUFullPyramid(pyramid_levels=levels, motion_model=model, outlier_policy="median") as eng:
+    with ICGNWGPUFullPyramid(pyramid_levels=levels, motion_model=model, outlier_policy="median",
+                             median_filter_scope="all") as eng:
         eng.set_reference(ref, roi_mask=_uhd_mask(), roi_policy=key.split("-")[2])
         res = eng.register(tgt)
     raw = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
@@ -250,10 +263,8 @@ def test_uhd_roi_goldens(key, model, levels):
     print("vs WGSL golden", stats)
     assert np.array_equal(res.status == -3, golden[..., 6] == -3)
     assert stats["status_equal"] >= 0.999
-    if model == "translation":
-        # One level: the capture and the oracle differ by up to 0.018 px on 3 % of subsets (the CPU
-        # oracle test of the same case meets 0.95); two levels agree to 1e-3 on >= 99 %.
-        assert stats["disp_within_1e3"] >= (0.99 if levels == 2 else 0.95)
+    # Two-level affine: a coarse-level status that flips moves the seeds of its fine subsets.
+    assert stats["disp_within_1e3"] >= (0.998 if (model, levels) == ("affine", 2) else 0.999)
 
 
 def test_consecutive_pair_mode_matches_set_reference_then_register():

@@ -179,11 +179,12 @@ def test_temporal_translation_goldens(case, levels):
     ref = load_gray("uhd_reference.png")
     goldens = np.load(GOLDEN / "uhd_goldens.npz", allow_pickle=False)
     frames = [np.roll(ref, (-i, 2 * i), (0, 1)) for i in range(3)]
+    # The capture's "median" policy filtered every subset (median_filter_scope="all").
     pipe = oracle.Pipeline(pyramid_levels=levels, motion_model="translation", outlier_policy="median",
-                           initialization="temporal")
+                           median_filter_scope="all", initialization="temporal")
     pipe.set_reference(ref)
     with ICGNWGPUFullPyramid(pyramid_levels=levels, motion_model="translation", outlier_policy="median",
-                             initialization="temporal") as eng:
+                             median_filter_scope="all", initialization="temporal") as eng:
         eng.set_reference(ref)
         results = list(eng.register_stream(frames, pipeline_depth=2))
     for frame, res in enumerate(results):
@@ -198,7 +199,7 @@ def test_temporal_translation_goldens(case, levels):
                                    goldens[f"{case}-f{frame}-iterations"][..., None]], axis=-1)
             stats = compare_fields(got, gold, 6)
             _say("frame", frame, "vs WGSL golden", stats)
-            assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.99
+            assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
 
 
 LATENCY = dict(initialization="temporal", temporal_prediction="velocity", temporal_fine_correction=True,
@@ -227,10 +228,11 @@ def test_latency_translation_goldens(case, frames, roi):
         links = seed_permissions(rois[0]["plan"], rois[1]["plan"], source_scale=1, destination_scale=2, size=16,
                                  stride=16, reference_mask=mask)
     frames_in = [np.roll(ref, (-i, 2 * i), (0, 1)) for i in range(3)]
-    pipe = oracle.Pipeline(pyramid_levels=2, motion_model="translation", outlier_policy="median", **LATENCY)
+    pipe = oracle.Pipeline(pyramid_levels=2, motion_model="translation", outlier_policy="median",
+                           median_filter_scope="all", **LATENCY)
     pipe.set_reference(ref, rois=rois, temporal_links=links)
     with ICGNWGPUFullPyramid(pyramid_levels=2, motion_model="translation", outlier_policy="median",
-                             **LATENCY) as eng:
+                             median_filter_scope="all", **LATENCY) as eng:
         eng.set_reference(ref, **({} if mask is None else dict(roi_mask=mask, roi_policy=roi)))
         results = list(eng.register_stream(frames_in, pipeline_depth=3))
     for frame, res in enumerate(results):
@@ -247,7 +249,7 @@ def test_latency_translation_goldens(case, frames, roi):
                                    goldens[f"{key}-iterations"][..., None]], axis=-1)
             stats = compare_fields(got, gold, 6)
             _say("frame", frame, "vs WGSL golden", stats)
-            assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.99
+            assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
     assert results[-1].config["temporalPrediction"] == "velocity"
     assert results[-1].config["temporalFineCorrection"] is True
     assert results[-1].config["divergencePatience"] == 3

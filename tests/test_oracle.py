@@ -158,35 +158,117 @@ def _golden(goldens, key):
                            goldens[key + "-iterations"][..., None]], axis=-1)
 
 
+# The capture (before.npz, qualify_binding_budget.py:46-50) ran outlier_policy="median" before the
+# reference added median_filter_scope; its "median" then filtered every subset, which is the
+# present median_filter_scope="all".  With that scope the oracle reproduces the capture's
+# parameters (below); with the present default ("rejected") it differs by the 3x3 median's
+# smoothing, about 0.02 px RMS at 16/16, while statuses and iteration counts stay identical.
+GOLDEN_OPTIONS = dict(outlier_policy="median", median_filter_scope="all")
+
+
 def test_uhd_translation_golden(uhd):
     """WGSL golden (before.npz): translation l1, np.roll(reference, (-1, 2)) -> truth (2, -1)."""
     ref, _, goldens = uhd
-    out = oracle.Pipeline(pyramid_levels=1, motion_model="translation", outlier_policy="median").set_reference(
+    out = oracle.Pipeline(pyramid_levels=1, motion_model="translation", **GOLDEN_OPTIONS).set_reference(
         ref).register(np.roll(ref, (-1, 2), (0, 1)))
     stats = compare_fields(out, _golden(goldens, "translation-l1-none-zero-f1"), 6)
     print(stats)
     assert stats["status_equal"] == 1.0
-    assert stats["disp_within_1e3"] >= 0.95
+    assert stats["disp_within_1e3"] == 1.0 and stats["param_rmse"] < 1e-5
 
 
-def test_uhd_affine_golden_statuses(uhd):
-    """WGSL golden affine l1 and l2 on the tensile pair: identical statuses.  Parameters
-    differ from this capture by ~0.02 px RMS, which summation order cannot explain (it
-    moves them by 1e-5); the capture predates the reference snapshot's source."""
+def test_uhd_affine_golden(uhd):
+    """WGSL golden affine l1 on the tensile pair: identical statuses, |du| <= 1e-3 px on >= 99.9 %
+    of the accepted subsets."""
     ref, tgt, goldens = uhd
-    out = oracle.Pipeline(pyramid_levels=1, outlier_policy="median").set_reference(ref).register(tgt)
+    out = oracle.Pipeline(pyramid_levels=1, **GOLDEN_OPTIONS).set_reference(ref).register(tgt)
     stats = compare_fields(out, _golden(goldens, "affine-l1-none-zero-f1"), 6)
     print(stats)
     assert stats["status_equal"] == 1.0
-    assert stats["param_rmse"] < 0.02
+    assert stats["disp_within_1e3"] >= 0.999 and stats["param_rmse"] < 1e-4
+
+
+def test_uhd_affine_golden_needs_the_capture_scope(uhd):
+    """The present default scope ("rejected") leaves accepted subsets unfiltered: same statuses and
+    iterations as the capture, parameters 0.01-0.02 px away.  Pins that the scope, not the
+    solver arithmetic, separates the two."""
+    ref, tgt, goldens = uhd
+    out = oracle.Pipeline(pyramid_levels=1, outlier_policy="median").set_reference(ref).register(tgt)
+    gold = _golden(goldens, "affine-l1-none-zero-f1")
+    stats = compare_fields(out, gold, 6)
+    assert stats["status_equal"] == 1.0 and np.mean(out[..., 7] == gold[..., 7]) > 0.998
+    assert stats["disp_within_1e3"] < 0.2
 
 
 @pytest.mark.slow
 def test_uhd_quadratic_and_pyramid_goldens(uhd):
+    """Affine l2/l3 at the 99.9 % bar.  Quadratic l1: statuses >= 99.9 %, but |du| <= 1e-3 on only
+    98.6 %: 1.4 % of its subsets stop one iteration apart at the 5e-3 tolerance (summation order
+    of the 12x12 update), and the all-subsets median spreads each to its 3x3 neighbourhood.
+    Where the neighbourhood's iteration counts are identical, 99.89 % agree."""
     ref, tgt, goldens = uhd
-    for key, kw, P in (("quadratic-l1-none-zero-f1", dict(pyramid_levels=1, motion_model="quadratic"), 12),
-                       ("affine-l2-none-zero-f1", dict(pyramid_levels=2), 6)):
-        out = oracle.Pipeline(outlier_policy="median", **kw).set_reference(ref).register(tgt)
-        stats = compare_fields(out, _golden(goldens, key), P)
+    for key, kw, P, bar in (("affine-l2-none-zero-f1", dict(pyramid_levels=2), 6, 0.999),
+                            ("affine-l3-none-zero-f1", dict(pyramid_levels=3), 6, 0.998),
+                            ("quadratic-l1-none-zero-f1", dict(pyramid_levels=1, motion_model="quadratic"), 12,
+                             0.985)):
+        out = oracle.Pipeline(**GOLDEN_OPTIONS, **kw).set_reference(ref).register(tgt)
+        gold = _golden(goldens, key)
+        stats = compare_fields(out, gold, P)
         print(key, stats)
-        assert stats["status_equal"] >= 0.995
+        assert stats["status_equal"] >= 0.999
+        assert stats["disp_within_1e3"] >= bar
+        if P == 12:
+            same = out[..., 13] == gold[..., 13]
+            pad = np.pad(same, 1, constant_values=True)
+            calm = np.ones_like(same)
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    calm &= pad[1 + dy:1 + dy + same.shape[0], 1 + dx:1 + dx + same.shape[1]]
+            both = calm & (out[..., 12] > 0.5) & (gold[..., 12] > 0.5)
+            du = np.maximum(np.abs(out[..., 0] - gold[..., 0]), np.abs(out[..., 3] - gold[..., 3]))[both]
+            assert np.mean(du <= 1e-3) >= 0.998
+
+
+def _uhd_mask():
+    """ROI of the capture (benchmarks/qualify_binding_budget.py:30-32)."""
+    mask = np.zeros((2160, 3840), dtype=bool)
+    mask[160:2000, 160:3680] = True
+    mask[880:1200, 1800:2040] = False
+    return mask
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("key,model,levels,bar", [("affine-l1-full-zero", "affine", 1, 0.999),
+                                                  ("translation-l2-full-zero", "translation", 2, 0.999),
+                                                  ("translation-l1-partial-zero", "translation", 1, 0.999),
+                                                  ("affine-l2-partial-zero", "affine", 2, 0.998)])
+def test_uhd_roi_goldens(uhd, key, model, levels, bar):
+    """ROI goldens of the capture with the host planner's tables: the -3 pattern, statuses and
+    parameters."""
+    from helpers import oracle_rois
+
+    ref, tgt, goldens = uhd
+    rois = oracle_rois(16, 16, _uhd_mask(), levels, policy=key.split("-")[2])
+    target = tgt if model != "translation" else np.roll(ref, (-1, 2), (0, 1))
+    out = oracle.Pipeline(pyramid_levels=levels, motion_model=model, **GOLDEN_OPTIONS).set_reference(
+        ref, rois=rois).register(target)
+    gold = _golden(goldens, f"{key}-f1")
+    stats = compare_fields(out, gold, 6)
+    print(key, stats)
+    assert np.array_equal(out[..., 6] == -3, gold[..., 6] == -3)
+    assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= bar
+
+
+@pytest.mark.slow
+def test_quadratic_uhd_summation_order_sensitivity(uhd):
+    """The quadratic tensile solve at the reference's 5e-3 tolerance cannot meet |du| <= 1e-3 on
+    99.9 % between two legitimate summation orders: the oracle with the reference's 16 lanes against
+    the same oracle with 64 lanes (a wave's strided order) stays below the bar, which is why the
+    GPU-vs-oracle quadratic comparison at this tolerance is held to 99.8 %."""
+    ref, tgt, _ = uhd
+    runs = {lanes: oracle.Pipeline(pyramid_levels=1, motion_model="quadratic", outlier_policy="disabled",
+                                   lanes=lanes).set_reference(ref).register(tgt) for lanes in (16, 64)}
+    stats = compare_fields(runs[64], runs[16], 12)
+    print(stats)
+    assert stats["status_equal"] >= 0.999
+    assert 0.997 <= stats["disp_within_1e3"] < 0.999

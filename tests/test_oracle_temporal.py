@@ -61,7 +61,7 @@ def test_temporal_translation_goldens(case, levels):
     ref = load_gray("uhd_reference.png")
     goldens = np.load(GOLDEN / "uhd_goldens.npz", allow_pickle=False)
     pipe = oracle.Pipeline(pyramid_levels=levels, motion_model="translation", outlier_policy="median",
-                           initialization="temporal")
+                           median_filter_scope="all", initialization="temporal")
     pipe.set_reference(ref)
     for frame in range(3):
         out = pipe.register(np.roll(ref, (-frame, 2 * frame), (0, 1)))
@@ -72,16 +72,18 @@ def test_temporal_translation_goldens(case, levels):
         stats = compare_fields(out, gold, 6)
         print(case, frame, stats, pipe.last_counts)
         assert pipe.last_counts[0] > 0
-        assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.99
+        assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
 
 
-def test_temporal_affine_golden_statuses():
-    """affine-l2-none-temporal frame 1 (the tensile pair seeded by the identity frame): statuses."""
+def test_temporal_affine_golden():
+    """affine-l2-none-temporal frame 1 (the tensile pair seeded by the identity frame): statuses and
+    parameters.  The capture's "median" policy filtered every subset (median_filter_scope="all")."""
     from helpers import GOLDEN, compare_fields, load_gray
 
     ref = load_gray("uhd_reference.png")
     goldens = np.load(GOLDEN / "uhd_goldens.npz", allow_pickle=False)
-    pipe = oracle.Pipeline(pyramid_levels=2, outlier_policy="median", initialization="temporal")
+    pipe = oracle.Pipeline(pyramid_levels=2, outlier_policy="median", median_filter_scope="all",
+                           initialization="temporal")
     pipe.set_reference(ref)
     pipe.register(ref)
     out = pipe.register(load_gray("uhd_frame_040.png"))
@@ -90,7 +92,7 @@ def test_temporal_affine_golden_statuses():
                            goldens[f"{key}-iterations"][..., None]], axis=-1)
     stats = compare_fields(out, gold, 6)
     print(stats)
-    assert stats["status_equal"] >= 0.999 and stats["param_rmse"] < 0.02
+    assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
 
 
 # -- latency profile: velocity prediction, fine correction, divergence patience ------------------
@@ -186,7 +188,7 @@ def test_latency_translation_goldens(case, frames, roi):
         links = seed_permissions(rois[0]["plan"], rois[1]["plan"], source_scale=1, destination_scale=2, size=16,
                                  stride=16, reference_mask=mask)
     pipe = oracle.Pipeline(pyramid_levels=2, motion_model="translation", outlier_policy="median",
-                           initialization="temporal", temporal_prediction="velocity", temporal_fine_correction=True,
+                           median_filter_scope="all", initialization="temporal", temporal_prediction="velocity", temporal_fine_correction=True,
                            divergence_patience=3, divergence_min_iterations=4)
     pipe.set_reference(ref, rois=rois, temporal_links=links)
     for frame in range(3):
@@ -198,7 +200,7 @@ def test_latency_translation_goldens(case, frames, roi):
                                goldens[f"{key}-iterations"][..., None]], axis=-1)
         stats = compare_fields(out, gold, 6)
         print(key, stats, pipe.last_counts, pipe.last_predicted)
-        assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.99
+        assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
         assert np.mean(out[..., 7] == gold[..., 7]) >= 0.98
         if frame == 2:
             assert pipe.last_predicted > 0
