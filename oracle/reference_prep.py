@@ -109,12 +109,6 @@ def reference_arrays(
     raw image for direct interpolants), B-spline derivative gradients, per-subset
     anchored mean, energy, J^T f, sum J, H = sum w J J^T and its inverse.
     """
-    h, w = image.shape
-    S = int(subset_size)
-    D = S if subset_stride is None else int(subset_stride)
-    half = 0.5 * (S - 1)
-    gw, gh = grid_shape(w, h, S, D)
-    sigma = 0.25 * S if gaussian_sigma is None else float(gaussian_sigma)
     if interpolation == "bspline":
         intensity = gaussian_sigma_one(image)
         source = bspline_coefficients(intensity)
@@ -127,7 +121,21 @@ def reference_arrays(
     if interpolation == "bspline":
         gx = (0.5 * (p[:-2, 2:] - p[:-2, :-2]) + 4.0 * gx + 0.5 * (p[2:, 2:] - p[2:, :-2])) / 6.0
         gy = (0.5 * (p[2:, :-2] - p[:-2, :-2]) + 4.0 * gy + 0.5 * (p[2:, 2:] - p[:-2, 2:])) / 6.0
+    return subset_arrays(intensity, gx, gy, motion_model=motion_model, subset_size=subset_size,
+                         subset_stride=subset_stride, spatial_weighting=spatial_weighting,
+                         gaussian_sigma=gaussian_sigma)
 
+
+def subset_arrays(intensity, gx, gy, *, motion_model="affine", subset_size=16, subset_stride=None,
+                  spatial_weighting="uniform", gaussian_sigma=None):
+    """The per-subset part of ``reference_arrays`` (wgpu_backend.py:650-823) for given float64
+    intensity and gradient planes: the reference plane, inverse Hessians and metadata."""
+    h, w = intensity.shape
+    S = int(subset_size)
+    D = S if subset_stride is None else int(subset_stride)
+    half = 0.5 * (S - 1)
+    gw, gh = grid_shape(w, h, S, D)
+    sigma = 0.25 * S if gaussian_sigma is None else float(gaussian_sigma)
     ly, lx = np.meshgrid(np.arange(S, dtype=np.float64) - half, np.arange(S, dtype=np.float64) - half,
                          indexing="ij")
     lx = lx.ravel()

@@ -7,12 +7,19 @@ Reads (as data, never executes) files of the reference checkout:
 - DIC/LucasKanadeICGN/benchmarks/results/binding-budget-20261031/before.npz, the WGSL fields
   captured by benchmarks/qualify_binding_budget.py (ICGNWGPUFullPyramid, outlier_policy="median",
   16/16, B-spline; ROI mask [160:2000,160:3680] minus [880:1200,1800:2040]).
-Writes tests/golden/*.png (copied bytes) and tests/golden/uhd_goldens.npz (selected cases).
+- DIC/LucasKanadeICGN/assets/uhd_tensile/machine_truth.csv and manifest.json: the analytic
+  stretches of frame 40 and the manifest's regression_limits;
+- DIC/LucasKanadeICGN/benchmarks/results/browser-gpu-reference-20261101/affine-uhd.json: the
+  reference's published accuracy of the frame-40 affine field (2 levels, 16/16, median).
+Writes tests/golden/*.png (copied bytes), tests/golden/uhd_goldens.npz (selected cases) and
+tests/golden/uhd_tensile_truth.json.
 The assets and golden fields are GPL-2.0-or-later data of the reference project.
 """
 
 from __future__ import annotations
 
+import csv
+import json
 import shutil
 from pathlib import Path
 
@@ -75,6 +82,25 @@ def main() -> None:
                 fields[name] = data[name]
     np.savez_compressed(OUT / "uhd_goldens.npz", **fields)
     print("wrote", len(fields), "arrays")
+    tensile = SRC / "assets/uhd_tensile"
+    with (tensile / "machine_truth.csv").open(newline="", encoding="utf-8") as stream:
+        rows = {int(row["frame"]): row for row in csv.DictReader(stream)}
+    manifest = json.loads((tensile / "manifest.json").read_text(encoding="utf-8"))
+    truth = {
+        "source": "assets/uhd_tensile/machine_truth.csv (frame 40) and manifest.json",
+        "forward_map": manifest["forward_map"],
+        "frame_040": {key: float(rows[40][key]) for key in ("stretch_x", "stretch_y")},
+        "regression_limits": manifest["regression_limits"],
+    }
+    report = json.loads((SRC / "benchmarks/results/browser-gpu-reference-20261101/affine-uhd.json")
+                        .read_text(encoding="utf-8"))
+    truth["published_frame_040_affine"] = {
+        "source": "benchmarks/results/browser-gpu-reference-20261101/affine-uhd.json",
+        "engine_options": report["engineOptions"],
+        "accepted": report["quality"]["cpuAccepted"],
+        "analytical": report["analytical"],
+    }
+    (OUT / "uhd_tensile_truth.json").write_text(json.dumps(truth, indent=2) + "\n", encoding="utf-8")
 
 
 if __name__ == "__main__":

@@ -59,3 +59,47 @@ def oracle_rois(size: int, stride: int, mask: np.ndarray, levels: int, *, policy
                                                reference_mask=mask)
         out.append(d)
     return out
+
+
+def bilinear_shift(source: np.ndarray, dx: float, dy: float) -> np.ndarray:
+    """Target of a known translation (dx, dy): each target pixel samples the source bilinearly at
+    (x - dx, y - dy) with mirrored borders (period 2(n-1)), rounded to uint8.  This is the image
+    formation of the reference's fractional-shift fixture (python/tests/uhd_partial_cases.py:24-52)."""
+    h, w = source.shape
+
+    def mirror(i, n):
+        period = 2 * (n - 1)
+        i = np.mod(i, period)
+        return np.where(i < n, i, period - i)
+
+    x = np.arange(w, dtype=np.float64) - dx
+    x0 = np.floor(x).astype(np.int64)
+    fx = x - x0
+    xa, xb = mirror(x0, w), mirror(x0 + 1, w)
+    out = np.empty_like(source)
+    src = source.astype(np.float64)
+    for start in range(0, h, 256):
+        y = np.arange(start, min(start + 256, h), dtype=np.float64) - dy
+        y0 = np.floor(y).astype(np.int64)
+        fy = (y - y0)[:, None]
+        ya, yb = mirror(y0, h), mirror(y0 + 1, h)
+        top = src[ya][:, xa] * (1 - fx) + src[ya][:, xb] * fx
+        bottom = src[yb][:, xa] * (1 - fx) + src[yb][:, xb] * fx
+        out[start:start + len(y)] = np.clip(np.rint(top * (1 - fy) + bottom * fy), 0, 255).astype(np.uint8)
+    return out
+
+
+def tensile_truth() -> dict:
+    import json
+
+    return json.loads((GOLDEN / "uhd_tensile_truth.json").read_text(encoding="utf-8"))
+
+
+def interior_centres(shape: tuple[int, int], size: int, stride: int, width: int, height: int, margin: int):
+    """Subset centres (x, y) and the interior mask of a margin in pixels
+    (python/tests/test_uhd_fixtures.py:43-49)."""
+    gy, gx = np.indices(shape)
+    x = gx * stride + (size - 1) / 2
+    y = gy * stride + (size - 1) / 2
+    inside = (x >= margin) & (x < width - margin) & (y >= margin) & (y < height - margin)
+    return inside, x, y
