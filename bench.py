@@ -6,9 +6,13 @@ U[80, 200], 30 grey background, noise sigma 1 grey; 126,563 blobs, the density o
 the 512x512 pair, because 1.2 M summed blobs saturate the image at 255), a 60-frame sequence with
 u_x = s(2 + 0.002x + 1.5 sin(2 pi y/2160)), u_y = s(-1 + 0.001y + 0.8 cos(2 pi x/3840)),
 s = k/59; affine IC-GN, subset 31, step 8 (127,359 subsets), B-spline, FP32,
-two-level pyramid, GPU reference preparation.  Frames are rendered on the GPU
-and stay resident in HBM; the timed region covers the per-frame work only
-(pyramid, prefilters, solves, outlier passes, seed transfer).
+two-level pyramid, GPU reference preparation.
+
+`value` is measured at the reference's product boundary (LK/BENCHMARK.md:11-15): resident,
+contiguous uint8 NumPy frames in host memory go through
+ICGNWGPUFullPyramid.register_stream(pipeline_depth=2) and come back as materialised NumPy fields
+and statistics; every upload and readback is inside the timed region.  The same frames resident in
+HBM (`resident` key) time the device work alone.
 
 Prints one JSON line (see the repository's measurement contract).
 """
@@ -75,7 +79,8 @@ def cpu_baseline(seq, frame_index: int, budget_s: float = 25.0) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": f"{frames} full 4K frame(s) (frame {frame_index} of the 60-frame sequence, both pyramid levels) "
-                  f"through the C oracle with {threads} OpenMP threads; reference prepared beforehand",
+                  f"through the C oracle with {threads} OpenMP threads, NumPy frame to NumPy field; reference "
+                  "prepared beforehand",
     }
 
 
@@ -342,6 +347,93 @@ def consecutive_quadratic() -> dict:
     }
 
 
+# The reference's own CPU figure for this path (its WGSL on Mesa lavapipe through wgpu-py, which
+# cannot run on this machine): docs/how-to/run-on-cpu.rst:140-165.
+PUBLISHED_REFERENCE_CPU = {
+    "value": 4.56,
+    "unit": "frames/s",
+    "ms_per_frame_median": 218.2,
+    "cores": 8,
+    "kind": "reference (published, not re-measured)",
+    "hardware": "Intel Core Ultra 5 228V, Mesa 25.2.8 llvmpipe, wgpu-py 0.32.0",
+    "workload": "3840x2160 rigid translation, translation model 16/16, B-spline, median, 2 levels, noise 1; "
+                "NumPy image to NumPy field",
+    "source": "LK/docs/how-to/run-on-cpu.rst:140-165",
+}
+
+SMALL_SUBSET, SMALL_STRIDE = 21, 8
+
+
+def small_pair_line(frames: int = 400, warmup: int = 20) -> dict:
+    """The 512x512 pair (subset 21, step 8, 20 iterations, tolerance 1e-4, one level): GPU frames/s
+    at the NumPy boundary through register_stream(pipeline_depth=2), accuracy against the analytic
+    field, and the C oracle on the host at the same boundary."""
+    import oracle
+    from live_dic import ICGNWGPU, synthetic
+
+    ref, tgt, fld = synthetic.small_affine_pair()
+    with ICGNWGPU(subset_size=SMALL_SUBSET, subset_stride=SMALL_STRIDE) as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-4)
+        for _ in eng.register_stream([tgt] * warmup, pipeline_depth=2):
+            pass
+        t0 = time.perf_counter()
+        for res in eng.register_stream([tgt] * frames, pipeline_depth=2):
+            last = res
+        gpu_rate = frames / (time.perf_counter() - t0)
+    cx, cy = synthetic.subset_centres(512, 512, SMALL_SUBSET, SMALL_STRIDE)
+    u, v = fld.displacement(cx, cy)
+    ok = last.status == 1
+    rmse = float(np.sqrt(np.mean((last.parameters[..., 0][ok] - u[ok]) ** 2 + (last.parameters[..., 3][ok] - v[ok]) ** 2)))
+    pipe = oracle.Pipeline(pyramid_levels=1, iterations_per_level=(20,), tolerance_per_level=(1e-4,),
+                           subset_size=SMALL_SUBSET, subset_stride=SMALL_STRIDE).set_reference(ref)
+    pipe.register(tgt)
+    count, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 3.0:
+        pipe.register(tgt)
+        count += 1
+    cpu_rate = count / (time.perf_counter() - t0)
+    return {
+        "workload": "512x512 speckle pair, u = (0.3, -0.7) + 0.5 % x-strain, noise 1; affine subset 21 step 8 "
+                    "(3,844 subsets), 20 iterations, tolerance 1e-4, one level, B-spline",
+        "frames_per_s": gpu_rate,
+        "subsets_per_s": gpu_rate * last.status.size,
+        "boundary": "host NumPy uint8 -> NumPy field, register_stream(pipeline_depth=2)",
+        "accepted": int(ok.sum()),
+        "rmse_px": rmse,
+        "cpu_oracle_frames_per_s": cpu_rate,
+        "cpu_oracle_threads": oracle.cpu_threads(),
+    }
+
+
+def numpy_boundary(eng, host_frames, steps: int, warmup: int, depth: int = 2) -> dict:
+    """register_stream over host NumPy frames: frames/s and the median per-frame split of the
+    library's timing keys (the reference's call-time split, performance.rst:516-527)."""
+    order = [host_frames[1 + i % (FRAMES - 1)] for i in range(warmup + steps)]
+    for _ in eng.register_stream(order[:warmup], pipeline_depth=depth):
+        pass
+    timings = []
+    last = None
+    t0 = time.perf_counter()
+    for res in eng.register_stream(order[warmup:], pipeline_depth=depth):
+        timings.append(res.timing)
+        last = res
+    elapsed = time.perf_counter() - t0
+
+    def median(key):
+        values = [t[key] for t in timings if key in t]
+        return float(np.median(values)) if values else None
+
+    return {
+        "frames_per_s": steps / elapsed,
+        "ms_per_frame": elapsed / steps * 1e3,
+        "pipeline_depth": depth,
+        "split_ms_median": {key: median(key) for key in (
+            "inputPreparationWallMs", "uploadWallMs", "gpuMs", "solveGpuMs", "completionWaitWallMs",
+            "resultAgeWallMs")},
+        "last": last,
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -452,6 +544,26 @@ def main() -> int:
         mean_iterations.append(float(st[..., 7][acc].mean()) if acc.any() else None)
     # The same kernel with no concurrent frame (one slot), after the timed region.
     alone_total, alone_count = run(args.alone_frames, 0, 1) if args.alone_frames > 0 else (0.0, 0)
+    resident_fps, resident_elapsed = fps, elapsed
+
+    # The headline: the same frames as host NumPy arrays through register_stream.
+    print("[bench] NumPy boundary: register_stream(pipeline_depth=2) over host frames", file=sys.stderr,
+          flush=True)
+    host_frames = []
+    for buf in frames:
+        arr = np.empty((HEIGHT, WIDTH), np.uint8)
+        buf.download(arr)
+        host_frames.append(arr)
+    boundary = numpy_boundary(eng, host_frames, args.steps, args.warmup, depth=args.slots)
+    fps = boundary["frames_per_s"]
+    elapsed = args.steps / fps
+    last_result = boundary.pop("last")
+    last_numpy = 1 + (args.warmup + args.steps - 1) % (FRAMES - 1)
+    u2, v2 = seq.field.scaled(last_numpy / (FRAMES - 1)).displacement(cx, cy)
+    ok2 = last_result.accepted_mask
+    boundary["rmse_px_last_frame"] = float(np.sqrt(np.mean(
+        (last_result.parameters[..., 0][ok2] - u2[ok2]) ** 2 + (last_result.parameters[..., 3][ok2] - v2[ok2]) ** 2)))
+    boundary["accepted_fraction_last_frame"] = float(ok2.mean())
     eng.close()
 
     avg_solve_ms = solve_total / solve_count if solve_count else float("nan")
@@ -470,14 +582,16 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (GPU-rendered analytic speckle, seeded)",
+        "data": "synthetic (GPU-rendered analytic speckle, seeded; frames held as host NumPy uint8 arrays)",
         "config": {
             "workload": "4K fixed-reference affine IC-GN sequence, 3840x2160, subset 31, step 8, "
                         "2-level pyramid, B-spline, 60 frames",
+            "boundary": "host NumPy uint8 frame -> materialised NumPy field and statistics through "
+                        "ICGNWGPUFullPyramid.register_stream(pipeline_depth=2); uploads and readbacks timed",
             "subsets_per_frame": n_fine,
             "subsets_per_s": fps * n_fine,
-            "accepted_fraction_last_frame": float(ok.mean()),
-            "rmse_px_last_frame": rmse,
+            "accepted_fraction_last_frame": boundary["accepted_fraction_last_frame"],
+            "rmse_px_last_frame": boundary["rmse_px_last_frame"],
             "reference_prep_ms": prep_ms,
             "iterations_per_level": list(ITERATIONS),
             "mean_iterations_per_level": mean_iterations,
@@ -502,6 +616,16 @@ def main() -> int:
             "frac_alone": traffic_bytes / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if alone_ms else None,
         },
         "cpu_baseline": None,
+        "numpy_boundary": boundary,
+        "resident": {
+            "what": "the same frames resident in HBM (ldic_register_resident), device work only",
+            "frames_per_s": resident_fps,
+            "ms_per_frame": resident_elapsed / args.steps * 1e3,
+            "subsets_per_s": resident_fps * n_fine,
+            "accepted_fraction_last_frame": float(ok.mean()),
+            "rmse_px_last_frame": rmse,
+            "slots": args.slots,
+        },
     }
     if not args.no_learned:
         print("[bench] learned vs ZNCC seeding at large displacement", file=sys.stderr, flush=True)
@@ -515,6 +639,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print("[bench] CPU oracle baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(seq, 30)
+        line["cpu_baseline"]["published_reference"] = PUBLISHED_REFERENCE_CPU
+        print("[bench] 512x512 pair (GPU and CPU oracle)", file=sys.stderr, flush=True)
+        line["small_pair_512"] = small_pair_line()
     print(json.dumps(line))
     return 0
 

@@ -72,6 +72,15 @@ typedef struct ldic_timing {
     int32_t velocity_predicted; /* velocity prediction: finest subsets extrapolated (last frame) */
     int32_t levels_used;       /* pyramid levels the last frame solved (the adaptive pyramid may use 1) */
     int32_t temporal_expired;  /* adaptive pyramid: finest history entries that expired (last frame) */
+    /* Summary of the finest field copied into out_state, computed on the device with it
+     * (the statistics of RegistrationResult, engine.py:105-193): subsets with status > 0.5
+     * and > 1.5, and float64 sums of tx, ty and iterations over the former.  accepted and
+     * replaced are -1 when the call returned no field. */
+    int64_t accepted;
+    int64_t replaced;
+    double sum_translation_x;
+    double sum_translation_y;
+    double sum_iterations;
 } ldic_timing;
 
 /* Device selection and kernel set-up: ICGNWGPU.__init__ device/pipeline

@@ -148,6 +148,12 @@ hipError_t launch_temporal_merge(int count, int model, const float* raw, const f
                                  uint32_t max_age, float* history, uint32_t* ages, uint32_t* accepted_count,
                                  hipStream_t s, uint32_t* expired_count = nullptr);
 hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s);
+// Five float64 values: accepted count, replaced count, and the sums of tx, ty and
+// iterations over the accepted subsets of a packed state field.  `out` holds
+// kSummaryDoubles doubles (the summary, then per-workgroup partials).
+constexpr int kSummaryMaxBlocks = 256;
+constexpr int kSummaryDoubles = 8 + 5 * kSummaryMaxBlocks;
+hipError_t launch_field_summary(const float* state, int count, int model, double* out, hipStream_t s);
 // Velocity prediction (full_pyramid.wgsl:83-148): extrapolate the history from
 // the previous accepted measurement; save the measurement before the merge.
 hipError_t launch_temporal_predict(int count, int model, const float* history, const float* previous,

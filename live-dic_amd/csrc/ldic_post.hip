@@ -2,6 +2,8 @@
 // pass and the final affine ZNCC.  One thread (or one wave for ZNCC) per
 // subset; the state grid is a few MB, so these are latency-bound side passes.
 #include "ldic_device.h"
+#include <algorithm>
+
 #include "ldic_kernels.h"
 
 namespace ldic {
@@ -288,6 +290,68 @@ hipError_t launch_temporal_merge(int count, int model, const float* raw, const f
 hipError_t launch_fill_u32(int count, uint32_t value, uint32_t* p, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     k_fill_u32<<<(count + 255) / 256, 256, 0, s>>>(count, value, p);
+    return hipGetLastError();
+}
+
+// Summary of a packed state field for the host result: accepted (status > 0.5) and
+// replaced (status > 1.5) counts, and float64 sums of tx, ty and iterations over the
+// accepted subsets (the statistics RegistrationResult reports, engine.py:105-193).
+// Stage 1: each workgroup sums one contiguous run of subsets (threads strided, then a
+// fixed-order LDS tree) into its partial; stage 2: one workgroup sums the partials in
+// the same fixed order.  The result does not depend on scheduling.
+constexpr int kSummaryThreads = 256;
+
+__device__ void summary_tree(double (*sums)[kSummaryThreads], double* out) {
+    const int t = threadIdx.x;
+    __syncthreads();
+    for (int half = kSummaryThreads / 2; half > 0; half >>= 1) {
+        if (t < half)
+            for (int k = 0; k < 5; ++k) sums[k][t] += sums[k][t + half];
+        __syncthreads();
+    }
+    if (t < 5) out[t] = sums[t][0];
+}
+
+__global__ void __launch_bounds__(kSummaryThreads) k_field_summary_partial(const float* state, int count, int stride,
+                                                                           int P, int per_block, double* partial) {
+    __shared__ double sums[5][kSummaryThreads];
+    double acc = 0.0, rep = 0.0, tx = 0.0, ty = 0.0, it = 0.0;
+    const int begin = blockIdx.x * per_block, end = min(count, begin + per_block);
+    for (int i = begin + threadIdx.x; i < end; i += kSummaryThreads) {
+        const float* q = state + (size_t)i * stride;
+        const float status = q[P];
+        if (status > 0.5f) {
+            acc += 1.0;
+            tx += (double)q[0];
+            ty += (double)q[3];
+            it += (double)q[P + 1];
+        }
+        if (status > 1.5f) rep += 1.0;
+    }
+    const int t = threadIdx.x;
+    sums[0][t] = acc; sums[1][t] = rep; sums[2][t] = tx; sums[3][t] = ty; sums[4][t] = it;
+    summary_tree(sums, partial + 5 * (size_t)blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kSummaryThreads) k_field_summary_final(const double* partial, int blocks,
+                                                                         double* out) {
+    __shared__ double sums[5][kSummaryThreads];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 5; ++k) {
+        double v = 0.0;
+        for (int b = t; b < blocks; b += kSummaryThreads) v += partial[5 * (size_t)b + k];
+        sums[k][t] = v;
+    }
+    summary_tree(sums, out);
+}
+
+hipError_t launch_field_summary(const float* state, int count, int model, double* out, hipStream_t s) {
+    // out[0..4] receives the summary; out[8 .. 8 + 5 * kSummaryMaxBlocks) holds the partials.
+    const int per_block = std::max(kSummaryThreads, (count + kSummaryMaxBlocks - 1) / kSummaryMaxBlocks);
+    const int blocks = std::max(1, (count + per_block - 1) / per_block);
+    k_field_summary_partial<<<blocks, kSummaryThreads, 0, s>>>(state, count, state_stride(model), param_count(model),
+                                                               per_block, out + 8);
+    k_field_summary_final<<<1, kSummaryThreads, 0, s>>>(out + 8, blocks, out);
     return hipGetLastError();
 }
 

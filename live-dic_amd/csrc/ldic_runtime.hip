@@ -10,10 +10,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ldic_kernels.h"
@@ -141,6 +144,9 @@ struct Slot {
     float* pinned_zncc = nullptr;
     uint32_t* counts = nullptr;         // temporal: seeded coarsest subsets, merged finest subsets
     uint32_t* pinned_counts = nullptr;
+    double* summary = nullptr;          // launch_field_summary of the finest field read back
+    double* pinned_summary = nullptr;
+    bool summary_ready = false;         // the pending frame's readback carries a summary
     int levels_used = 0;                // pyramid levels of the slot's last frame
     bool adaptive_frame = false;        // it ran under the adaptive pyramid
     bool feedback_pending = false;      // its counts have not yet set the next frame's levels
@@ -259,6 +265,8 @@ int create_slot(ldic_engine* e, Slot** out) {
     CHECK(s->arena.alloc_pinned(&s->pinned_valid, (size_t)e->lv[0].count()));
     CHECK(s->arena.alloc_pinned(&s->pinned_out, (size_t)output_floats(e)));
     CHECK(s->arena.alloc_pinned(&s->pinned_zncc, (size_t)e->lv[0].count()));
+    CHECK(s->arena.alloc(&s->summary, (size_t)kSummaryDoubles));
+    CHECK(s->arena.alloc_pinned(&s->pinned_summary, 8));
     *out = s;
     return LDIC_OK;
 }
@@ -673,15 +681,110 @@ int validate_initialization(const ldic_engine* e, const ldic_initialization* in)
     return LDIC_OK;
 }
 
+// Host staging copies (caller buffer <-> pinned staging) of whole frames and fields
+// are split over a small pool of persistent threads: one thread's memcpy reaches a
+// fraction of the host's memory bandwidth, and these copies sit on the host path
+// between two frames in flight.
+class HostCopier {
+  public:
+    static HostCopier& get() {
+        static HostCopier pool;
+        return pool;
+    }
+    // Rows of `row_bytes` from src (pitch src_pitch) to dst (pitch dst_pitch).
+    void copy2d(void* dst, size_t dst_pitch, const void* src, size_t src_pitch, size_t row_bytes, size_t rows) {
+        const size_t bytes = row_bytes * rows;
+        const int parts = workers_.empty() || bytes < kMinParallelBytes ? 1 : (int)workers_.size() + 1;
+        if (parts == 1) {
+            run(dst, dst_pitch, src, src_pitch, row_bytes, 0, rows);
+            return;
+        }
+        std::unique_lock<std::mutex> lock(job_mutex_);  // one job at a time
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = Job{dst, dst_pitch, src, src_pitch, row_bytes, rows, parts};
+            remaining_ = parts - 1;
+            ++generation_;
+        }
+        cv_.notify_all();
+        run_part(job_, 0);
+        std::unique_lock<std::mutex> g(m_);
+        done_cv_.wait(g, [&] { return remaining_ == 0; });
+    }
+
+  private:
+    struct Job {
+        void* dst; size_t dst_pitch; const void* src; size_t src_pitch; size_t row_bytes, rows; int parts;
+    };
+    static constexpr size_t kMinParallelBytes = 1u << 20;
+
+    HostCopier() {
+        unsigned hw = std::thread::hardware_concurrency();
+        int n = hw >= 16 ? 7 : hw >= 8 ? 3 : hw >= 4 ? 1 : 0;
+        for (int i = 0; i < n; ++i) workers_.emplace_back([this, i] { loop(i + 1); });
+    }
+    ~HostCopier() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    static void run(void* dst, size_t dst_pitch, const void* src, size_t src_pitch, size_t row_bytes, size_t r0,
+                    size_t r1) {
+        auto* d = static_cast<uint8_t*>(dst);
+        auto* q = static_cast<const uint8_t*>(src);
+        if (dst_pitch == row_bytes && src_pitch == row_bytes) {
+            std::memcpy(d + r0 * row_bytes, q + r0 * row_bytes, (r1 - r0) * row_bytes);
+            return;
+        }
+        for (size_t r = r0; r < r1; ++r) std::memcpy(d + r * dst_pitch, q + r * src_pitch, row_bytes);
+    }
+    static void run_part(const Job& j, int part) {
+        // Contiguous copies split by bytes, pitched ones by rows.
+        if (j.dst_pitch == j.row_bytes && j.src_pitch == j.row_bytes) {
+            const size_t total = j.row_bytes * j.rows;
+            const size_t chunk = (total / j.parts + 4095) & ~(size_t)4095;
+            const size_t b0 = std::min(total, chunk * part), b1 = std::min(total, b0 + chunk);
+            if (b1 > b0)
+                std::memcpy(static_cast<uint8_t*>(j.dst) + b0, static_cast<const uint8_t*>(j.src) + b0, b1 - b0);
+            return;
+        }
+        const size_t per = (j.rows + j.parts - 1) / j.parts;
+        const size_t r0 = std::min(j.rows, per * part), r1 = std::min(j.rows, r0 + per);
+        run(j.dst, j.dst_pitch, j.src, j.src_pitch, j.row_bytes, r0, r1);
+    }
+    void loop(int part) {
+        uint64_t seen = 0;
+        for (;;) {
+            Job job;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || generation_ != seen; });
+                if (stop_) return;
+                seen = generation_;
+                job = job_;
+            }
+            if (part < job.parts) run_part(job, part);
+            std::lock_guard<std::mutex> g(m_);
+            if (part < job.parts && --remaining_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex job_mutex_, m_;
+    std::condition_variable cv_, done_cv_;
+    Job job_{};
+    int remaining_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+};
+
 int copy_in(ldic_engine* e, Slot* s, const uint8_t* target, int64_t row_stride) {
     const Level& L = e->lv[0];
     double t0 = now_ms();
-    if (row_stride == L.width) {
-        std::memcpy(s->pinned_in, target, (size_t)L.width * L.height);
-    } else {
-        for (int y = 0; y < L.height; ++y)
-            std::memcpy(s->pinned_in + (size_t)y * L.width, target + (size_t)y * row_stride, (size_t)L.width);
-    }
+    HostCopier::get().copy2d(s->pinned_in, (size_t)L.width, target, (size_t)row_stride, (size_t)L.width,
+                             (size_t)L.height);
     HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
     HIP_CHECK(hipMemcpyAsync(s->lv[0].target, s->pinned_in, (size_t)L.width * L.height, hipMemcpyHostToDevice,
                              s->stream));
@@ -721,8 +824,17 @@ int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wal
     CHECK(drain_ring(s));
     s->pending = false;
     adaptive_feedback(e, s);
-    if (out_state) std::memcpy(out_state, s->pinned_out, sizeof(float) * (size_t)output_floats(e));
+    if (out_state) {
+        const size_t bytes = sizeof(float) * (size_t)output_floats(e);
+        HostCopier::get().copy2d(out_state, bytes, s->pinned_out, bytes, bytes, 1);
+    }
     if (t) {
+        const bool summary = s->summary_ready && out_state;
+        t->accepted = summary ? (int64_t)s->pinned_summary[0] : -1;
+        t->replaced = summary ? (int64_t)s->pinned_summary[1] : -1;
+        t->sum_translation_x = summary ? s->pinned_summary[2] : 0.0;
+        t->sum_translation_y = summary ? s->pinned_summary[3] : 0.0;
+        t->sum_iterations = summary ? s->pinned_summary[4] : 0.0;
         float gpu = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&gpu, s->ev_begin, s->ev_end));
         t->upload_ms = s->upload_ms;
@@ -739,6 +851,19 @@ int finish(ldic_engine* e, Slot* s, float* out_state, ldic_timing* t, double wal
     }
     s->drained_solve_ms = 0.0;
     s->drained_frames = 0;
+    s->summary_ready = false;
+    return LDIC_OK;
+}
+
+// Queue the readback of a finest field into the slot's pinned staging, with its
+// summary (launch_field_summary) so the host result needs no pass over the field.
+int enqueue_readback(ldic_engine* e, Slot* s, const float* out) {
+    const int n = e->lv[0].count();
+    HIP_CHECK(launch_field_summary(out, n, e->opt.motion_model, s->summary, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)output_floats(e), hipMemcpyDeviceToHost,
+                             s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->pinned_summary, s->summary, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    s->summary_ready = true;
     return LDIC_OK;
 }
 
@@ -746,9 +871,7 @@ int submit_frame(ldic_engine* e, Slot* s, const uint8_t* target, int64_t row_str
     if (s->pending) return set_error(LDIC_ERR_STATE, "slot still holds an unretired frame");
     CHECK(copy_in(e, s, target, row_stride));
     CHECK(enqueue_frame(e, s));
-    const float* out = level_output(e, s->lv[0]);
-    HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)output_floats(e), hipMemcpyDeviceToHost,
-                             s->stream));
+    CHECK(enqueue_readback(e, s, level_output(e, s->lv[0])));
     HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
     s->pending = true;
     s->resident = false;
@@ -1179,7 +1302,7 @@ int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stri
     const float* out = level_output(e, sl);
     if (include_zncc)
         HIP_CHECK(hipMemcpyAsync(s->pinned_zncc, s->zncc, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s->stream));
-    HIP_CHECK(hipMemcpyAsync(s->pinned_out, out, sizeof(float) * (size_t)n * stride, hipMemcpyDeviceToHost, s->stream));
+    CHECK(enqueue_readback(e, s, out));
     HIP_CHECK(hipEventRecord(s->ev_end, s->stream));
     s->pending = true;
     s->resident = false;

@@ -80,6 +80,11 @@ class Timing(ctypes.Structure):
         ("velocity_predicted", ctypes.c_int32),
         ("levels_used", ctypes.c_int32),
         ("temporal_expired", ctypes.c_int32),
+        ("accepted", ctypes.c_int64),
+        ("replaced", ctypes.c_int64),
+        ("sum_translation_x", ctypes.c_double),
+        ("sum_translation_y", ctypes.c_double),
+        ("sum_iterations", ctypes.c_double),
     ]
 
 

@@ -1,4 +1,11 @@
-"""Result and input contract of the native supervisor (reference: live_dic/engine.py:56-362, 778-828)."""
+"""Result and input contract of the native supervisor.
+
+The public shapes follow the reference (live_dic/engine.py:56-362 for the result
+fields and archive keys, :778-828 for the accepted image inputs and their error
+messages); the code is this package's own.  A result is a set of zero-copy views
+of one packed float32 field ``(gh, gw, 8|16)`` as the device writes it, plus the
+summary statistics the device computes beside it.
+"""
 
 from __future__ import annotations
 
@@ -7,7 +14,7 @@ import json
 import os
 from collections.abc import Mapping, Sequence
 from dataclasses import dataclass
-from typing import Any, Final, TypeAlias
+from typing import Any, Final, NamedTuple, TypeAlias
 
 import numpy as np
 import numpy.typing as npt
@@ -24,14 +31,45 @@ MIN_FRAME_DIMENSION: Final = 16
 PARAMETER_ORDER: Final = AFFINE_PARAMETER_ORDER
 ImageInput: TypeAlias = Any
 
+# Status codes of the packed field: > 0.5 accepted (1) or a displacement outlier (2).
+_ACCEPTED_ABOVE: Final = 0.5
+_OUTLIER_ABOVE: Final = 1.5
+
 
 class ICGNError(RuntimeError):
     """Raised when the native runtime or the IC-GN computation fails."""
 
 
+class FieldSummary(NamedTuple):
+    """Counts and float64 sums of one packed field: subsets with status > 0.5 and > 1.5, and
+    the sums of tx, ty and iterations over the former."""
+
+    accepted: int
+    replaced: int
+    sum_tx: float
+    sum_ty: float
+    sum_iterations: float
+
+    @classmethod
+    def of(cls, packed: np.ndarray, params: int) -> FieldSummary:
+        """Host computation, for fields that did not come from the device with a summary."""
+        status = packed[..., params]
+        keep = status > _ACCEPTED_ABOVE
+        picked = packed[keep]
+        return cls(int(picked.shape[0]), int(np.count_nonzero(status > _OUTLIER_ABOVE)),
+                   float(picked[:, 0].sum(dtype=np.float64)), float(picked[:, 3].sum(dtype=np.float64)),
+                   float(picked[:, params + 1].sum(dtype=np.float64)))
+
+    def means(self) -> tuple[tuple[float, float] | None, float | None]:
+        if self.accepted == 0:
+            return None, None
+        n = float(self.accepted)
+        return (self.sum_tx / n, self.sum_ty / n), self.sum_iterations / n
+
+
 @dataclass(frozen=True, slots=True)
 class ReferenceInfo:
-    """Dimensions and timings of reference preparation (engine.py:82-103)."""
+    """Dimensions and timings of one reference preparation (fields: engine.py:82-103)."""
 
     preparation_wall_ms: float
     runtime_build_wall_ms: float
@@ -50,17 +88,16 @@ class ReferenceInfo:
 
     @property
     def roi_selected(self) -> np.ndarray | None:
-        return None if self.roi is None else self.roi.selected
+        return self.roi.selected if self.roi is not None else None
 
 
 @dataclass(slots=True)
 class RegistrationResult:
-    """Dense field over the subset grid (engine.py:106-273).
+    """Dense field over the subset grid (fields and properties: engine.py:106-273).
 
-    ``parameters[..., :]`` follows ``[tx, du/dx, du/dy, ty, dv/dx, dv/dy]``;
-    the quadratic model appends the six second derivatives.  Status is 1 for
-    accepted, 2 for displacement outliers, 0 for non-converged and negative for
-    rejected subsets.  All arrays are views of one packed float32 buffer.
+    ``parameters[..., :]`` is ``[tx, du/dx, du/dy, ty, dv/dx, dv/dy]``, followed for the
+    quadratic model by the six second derivatives.  Status 1 is accepted, 2 a displacement
+    outlier, 0 not converged, negative rejected.
     """
 
     parameters: np.ndarray
@@ -82,48 +119,58 @@ class RegistrationResult:
 
     @property
     def zncc(self) -> np.ndarray | None:
-        """Optional final normalised correlation; NaN means unavailable."""
+        """Final normalised correlation when it was requested; NaN where undefined."""
         return self._zncc
 
     @property
     def roi_selected(self) -> np.ndarray | None:
-        return None if self.roi is None else self.roi.selected
+        return self.roi.selected if self.roi is not None else None
+
+    def _selected_count(self) -> int:
+        return self.status.size if self.roi is None else self.roi.selected_count
 
     @property
     def populations(self) -> dict[str, int | float | None]:
-        selected = self.status.size if self.roi is None else self.roi.selected_count
+        total = self.status.size
+        chosen = self._selected_count()
+        partial = self.roi.partial_indices if self.roi is not None else None
         admissible = None
         if self.reference_valid is not None:
-            valid = self.reference_valid if self.roi is None else self.reference_valid & self.roi.selected
-            admissible = int(np.count_nonzero(valid))
+            usable = self.reference_valid if self.roi is None else np.logical_and(self.reference_valid,
+                                                                                self.roi.selected)
+            admissible = int(np.count_nonzero(usable))
         return {
-            "gridTotal": self.status.size,
-            "roiSelected": selected,
-            "roiExcluded": self.status.size - selected,
-            "roiPartial": 0,
+            "gridTotal": total,
+            "roiSelected": chosen,
+            "roiExcluded": total - chosen,
+            "roiPartial": 0 if partial is None else int(np.size(partial)),
             "referenceAdmissible": admissible,
             "accepted": self.accepted,
-            "rejectedSelected": selected - self.accepted,
-            "acceptedFractionSelected": self.accepted / selected if selected else None,
+            "rejectedSelected": chosen - self.accepted,
+            "acceptedFractionSelected": (self.accepted / chosen) if chosen else None,
         }
 
     @property
     def displacement(self) -> np.ndarray:
-        """(tx, ty) as a zero-copy view."""
+        """(tx, ty) per subset, a view of the parameters."""
         return self.parameters[..., 0:4:3]
 
     @property
     def accepted_mask(self) -> np.ndarray:
-        mask = self.status > 0.5
-        return mask if self.roi is None else mask & self.roi.selected
+        """Accepted and flagged subsets (status > 0.5) inside the ROI selection."""
+        keep = self.status > _ACCEPTED_ABOVE
+        if self.roi is not None:
+            keep &= self.roi.selected
+        return keep
 
     @property
     def flagged(self) -> int:
+        """Displacement outliers found by the outlier pass (status 2); same as ``replaced``."""
         return self.replaced
 
     @property
     def outlier_mask(self) -> np.ndarray:
-        return self.status > 1.5
+        return self.status > _OUTLIER_ABOVE
 
     @property
     def replaced_mask(self) -> np.ndarray:
@@ -134,118 +181,124 @@ class RegistrationResult:
         return parameter_order(str(self.config.get("motionModel", "affine")))
 
     def rmse(self, truth: Sequence[float]) -> float | None:
-        """Vector RMSE of accepted displacements against a constant (dx, dy)."""
-        expected = np.asarray(truth, dtype=np.float64)
-        if expected.shape != (2,):
+        """Root mean square of the displacement error vector against a constant (dx, dy)."""
+        target = np.asarray(truth, dtype=np.float64)
+        if target.shape != (2,):
             raise ValueError("truth must contain exactly two values: (dx, dy)")
-        mask = self.accepted_mask
-        if not np.any(mask):
+        picked = self.displacement[self.accepted_mask]
+        if picked.shape[0] == 0:
             return None
-        error = self.displacement[mask].astype(np.float64) - expected
-        return float(np.sqrt(np.mean(np.sum(error * error, axis=1))))
+        squared = np.square(picked.astype(np.float64) - target).sum(axis=1)
+        return float(np.sqrt(squared.mean()))
+
+    def archive(self) -> dict[str, np.ndarray]:
+        """Arrays of the self-contained field archive (keys: engine.py:180-193)."""
+        out = {
+            "parameters": self.parameters,
+            "displacement": self.displacement,
+            "status": self.status,
+            "iterations": self.iterations,
+            "parameter_order": np.asarray(self.parameter_order),
+            "config_json": np.asarray(json.dumps(dict(self.config), sort_keys=True, allow_nan=False)),
+            "populations_json": np.asarray(json.dumps(self.populations, allow_nan=False)),
+        }
+        if self.roi is not None:
+            out.update(self.roi.archive_fields())
+        if self.reference_valid is not None:
+            out["reference_valid"] = self.reference_valid
+        return out
 
     def save_npz(self, path: str | os.PathLike[str]) -> None:
-        """Self-contained field archive, including the static ROI (engine.py:180-193)."""
-        np.savez_compressed(
-            path,
-            parameters=self.parameters,
-            displacement=self.displacement,
-            status=self.status,
-            iterations=self.iterations,
-            parameter_order=np.asarray(self.parameter_order),
-            config_json=np.asarray(json.dumps(dict(self.config), sort_keys=True, allow_nan=False)),
-            populations_json=np.asarray(json.dumps(self.populations, allow_nan=False)),
-            **({} if self.roi is None else self.roi.archive_fields()),
-            **({} if self.reference_valid is None else {"reference_valid": self.reference_valid}),
-        )
+        """Write :meth:`archive` as one compressed NPZ file."""
+        np.savez_compressed(path, **self.archive())
 
     @classmethod
-    def from_packed(cls, raw, *, grid: tuple[int, int], motion_model: str, **fields) -> RegistrationResult:
-        """Views of one packed state buffer (gh, gw, 8|16) as a result."""
+    def from_packed(cls, raw, *, grid: tuple[int, int], motion_model: str, summary: FieldSummary | None = None,
+                    **fields) -> RegistrationResult:
+        """Views of one packed state buffer (gh, gw, 8|16).  ``summary`` carries the device's
+        statistics of the same field; without it they are computed here."""
         gw, gh = grid
-        order = parameter_order(motion_model)
-        stride = 16 if len(order) == 12 else 8
-        packed = np.frombuffer(raw, dtype=np.float32).reshape(gh, gw, stride)
-        P = len(order)
-        parameters = packed[..., :P]
-        status = packed[..., P]
-        iterations = packed[..., P + 1]
-        accepted_mask = status > 0.5
-        accepted = int(np.count_nonzero(accepted_mask))
-        mean_translation = mean_iterations = None
-        if accepted:
-            mean_translation = (
-                float(np.mean(parameters[..., 0][accepted_mask], dtype=np.float64)),
-                float(np.mean(parameters[..., 3][accepted_mask], dtype=np.float64)),
-            )
-            mean_iterations = float(np.mean(iterations[accepted_mask], dtype=np.float64))
-        return cls(parameters=parameters, status=status, iterations=iterations, accepted=accepted,
-                   replaced=int(np.count_nonzero(status > 1.5)), mean_translation=mean_translation,
+        params = len(parameter_order(motion_model))
+        packed = np.frombuffer(raw, dtype=np.float32).reshape(gh, gw, 16 if params == 12 else 8)
+        stats = summary if summary is not None else FieldSummary.of(packed, params)
+        mean_translation, mean_iterations = stats.means()
+        return cls(parameters=packed[..., :params], status=packed[..., params], iterations=packed[..., params + 1],
+                   accepted=stats.accepted, replaced=stats.replaced, mean_translation=mean_translation,
                    mean_iterations=mean_iterations, **fields)
 
     @classmethod
     def from_packed_browser_result(cls, raw: bytes, payload: Mapping[str, Any]) -> RegistrationResult:
-        """Interleaved little-endian float32 state plus its JSON header (engine.py:275-327)."""
-        grid = tuple(int(v) for v in payload["grid"])
+        """The browser engine's interleaved little-endian float32 state and its JSON header
+        (engine.py:275-327)."""
+        gw, gh = (int(v) for v in payload["grid"])
         config = dict(payload.get("config", {}))
         model = str(config.get("motionModel", "affine"))
         stride = int(payload.get("stateStride", 16 if model == "quadratic" else 8))
-        expected = grid[0] * grid[1] * stride * 4
-        if len(raw) != expected:
-            raise ICGNError(f"Browser binary result size mismatch: expected {expected} bytes, received {len(raw)}")
-        return cls.from_packed(
-            bytearray(raw), grid=grid, motion_model=model,
-            timing={k: float(v) for k, v in payload.get("timing", {}).items() if v is not None},
-            config=config, width=int(payload["width"]), height=int(payload["height"]),
-            subset_size=int(payload["subsetSize"]),
-            subset_stride=int(payload.get("subsetStride", payload["subsetSize"])))
+        want = gw * gh * stride * 4
+        if len(raw) != want:
+            raise ICGNError(f"Browser binary result size mismatch: expected {want} bytes, received {len(raw)}")
+        size = int(payload["subsetSize"])
+        timing = {k: float(v) for k, v in payload.get("timing", {}).items() if v is not None}
+        return cls.from_packed(bytearray(raw), grid=(gw, gh), motion_model=model, timing=timing, config=config,
+                               width=int(payload["width"]), height=int(payload["height"]), subset_size=size,
+                               subset_stride=int(payload.get("subsetStride", size)))
+
+
+# -- image inputs (engine.py:778-828) ------------------------------------------------------------
+def _from_pil(picture) -> np.ndarray:
+    """Float and 32-bit integer modes go through the array rules; others convert to "L"."""
+    if picture.mode == "F" or picture.mode.startswith("I"):
+        return _from_array(np.asarray(picture))
+    return np.asarray(picture.convert("L"), dtype=np.uint8)
+
+
+def _from_array(source: np.ndarray) -> np.ndarray:
+    if source.ndim not in (2, 3):
+        raise ValueError("Expected a 2D, RGB, or RGBA image.")
+    if source.ndim == 3 and source.shape[2] not in (3, 4):
+        raise ValueError("Expected an RGB or RGBA image with 3 or 4 channels.")
+    kind = source.dtype
+    if np.issubdtype(kind, np.floating):
+        finite = np.nan_to_num(source, nan=0.0, posinf=255.0, neginf=0.0)
+        if finite.size and float(finite.max()) <= 1.0:  # normalised intensities
+            finite = finite * 255.0
+        values = np.clip(finite, 0, 255).astype(np.uint8)
+    elif kind == np.uint8:
+        values = source
+    elif np.issubdtype(kind, np.integer):
+        if source.size and (int(source.min()) < 0 or int(source.max()) > 255):
+            raise ValueError(
+                "Integer image values must be in [0, 255]. Convert higher-bit-depth images explicitly "
+                "to uint8 with the same calibrated intensity range for the reference and every target."
+            )
+        values = source.astype(np.uint8)
+    else:
+        raise ValueError(f"Unsupported image dtype {kind}.")
+    if values.ndim == 2:
+        return values
+    if Image is None:
+        raise ValueError("Pillow is required to convert colour images")
+    return np.asarray(Image.fromarray(values).convert("L"), dtype=np.uint8)
 
 
 def _as_uint8_gray(image: ImageInput) -> np.ndarray:
-    """Contiguous 8-bit grayscale frame from an array, PIL image, path or bytes (engine.py:778-828)."""
-    if isinstance(image, (str, os.PathLike, bytes)) or (Image is not None and isinstance(image, Image.Image)):
+    """Contiguous 8-bit grayscale frame from an array, a PIL image, a path or encoded bytes."""
+    is_pil = Image is not None and isinstance(image, Image.Image)
+    if isinstance(image, (str, os.PathLike, bytes)) or is_pil:
         if Image is None:
             raise ValueError("Pillow is required to read image files")
-        opened = image if isinstance(image, Image.Image) else Image.open(
-            io.BytesIO(image) if isinstance(image, bytes) else image)
-        try:
-            if opened.mode == "F" or opened.mode.startswith("I"):
-                return _as_uint8_gray(np.asarray(opened))
-            array = np.asarray(opened.convert("L"), dtype=np.uint8)
-        finally:
-            if opened is not image:
-                opened.close()
-    else:
-        source = np.asarray(image)
-        if source.ndim not in (2, 3):
-            raise ValueError("Expected a 2D, RGB, or RGBA image.")
-        if source.ndim == 3 and source.shape[2] not in (3, 4):
-            raise ValueError("Expected an RGB or RGBA image with 3 or 4 channels.")
-        if np.issubdtype(source.dtype, np.floating):
-            source = np.nan_to_num(source, nan=0.0, posinf=255.0, neginf=0.0)
-            scale = 255.0 if source.size and float(np.max(source)) <= 1.0 else 1.0
-            source = np.clip(source * scale, 0, 255).astype(np.uint8)
-        elif source.dtype != np.uint8:
-            if not np.issubdtype(source.dtype, np.integer):
-                raise ValueError(f"Unsupported image dtype {source.dtype}.")
-            if source.size and (int(np.min(source)) < 0 or int(np.max(source)) > 255):
-                raise ValueError(
-                    "Integer image values must be in [0, 255]. Convert higher-bit-depth images explicitly "
-                    "to uint8 with the same calibrated intensity range for the reference and every target."
-                )
-            source = source.astype(np.uint8)
-        if source.ndim == 2:
-            array = source
+        if is_pil:
+            frame = _from_pil(image)
         else:
-            if Image is None:
-                raise ValueError("Pillow is required to convert colour images")
-            array = np.asarray(Image.fromarray(source).convert("L"), dtype=np.uint8)
-    height, width = array.shape
-    if height < MIN_FRAME_DIMENSION or width < MIN_FRAME_DIMENSION:
+            with Image.open(io.BytesIO(image) if isinstance(image, bytes) else image) as opened:
+                frame = _from_pil(opened)
+    else:
+        frame = _from_array(np.asarray(image))
+    if min(frame.shape) < MIN_FRAME_DIMENSION:
         raise ValueError(
-            f"Image dimensions must each be at least {MIN_FRAME_DIMENSION} pixels; received {array.shape}."
+            f"Image dimensions must each be at least {MIN_FRAME_DIMENSION} pixels; received {frame.shape}."
         )
-    return np.ascontiguousarray(array)
+    return np.ascontiguousarray(frame)
 
 
 def empty_timing() -> dict[str, float]:

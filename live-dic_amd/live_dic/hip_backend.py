@@ -21,7 +21,7 @@ from typing import Any
 import numpy as np
 
 from . import _native
-from .engine import ICGNError, ImageInput, ReferenceInfo, RegistrationResult, _as_uint8_gray
+from .engine import FieldSummary, ICGNError, ImageInput, ReferenceInfo, RegistrationResult, _as_uint8_gray
 from .interpolation import INTERPOLATION_CODES, normalize_interpolation
 from .motion import MOTION_MODEL_CODES, normalize_motion_model, parameter_count, state_float_count
 from .outlier import (
@@ -277,10 +277,19 @@ class _NativeEngine:
                              f"received {frame.shape[1]}x{frame.shape[0]}.")
         return frame
 
-    def _state_buffer(self) -> bytearray:
-        return bytearray(self.grid_width * self.grid_height * state_float_count(self.motion_model) * 4)
+    def _state_buffer(self) -> np.ndarray:
+        """Uninitialised writable bytes for one packed field (the device fills all of it)."""
+        return np.empty(self.grid_width * self.grid_height * state_float_count(self.motion_model) * 4, np.uint8)
 
-    def _result(self, raw, timing: dict, config: dict) -> RegistrationResult:
+    @staticmethod
+    def _summary(t: _native.Timing) -> FieldSummary | None:
+        """The device's summary of the field a call returned (None if it returned none)."""
+        if t.accepted < 0:
+            return None
+        return FieldSummary(int(t.accepted), int(t.replaced), t.sum_translation_x, t.sum_translation_y,
+                            t.sum_iterations)
+
+    def _result(self, raw, timing: dict, config: dict, summary: FieldSummary | None = None) -> RegistrationResult:
         full_config = {
             **config,
             **self._pyramid_config,
@@ -294,7 +303,8 @@ class _NativeEngine:
             **self._option_config(),
         }
         return RegistrationResult.from_packed(
-            raw, grid=(self.grid_width, self.grid_height), motion_model=self.motion_model, timing=timing,
+            raw, grid=(self.grid_width, self.grid_height), motion_model=self.motion_model, summary=summary,
+            timing=timing,
             config=full_config, width=self.width, height=self.height, subset_size=self.subset_size,
             subset_stride=self.subset_stride, roi=self.roi, reference_valid=self.reference_valid)
 
@@ -302,7 +312,7 @@ class _NativeEngine:
         """Finite dense field without device work when the ROI selects nothing (:2562-2572)."""
         raw = np.zeros((self.grid_height, self.grid_width, state_float_count(self.motion_model)), np.float32)
         raw[..., parameter_count(self.motion_model)] = -3.0
-        result = self._result(bytearray(raw.tobytes()), {"queueSubmissions": 0.0}, config)
+        result = self._result(raw.reshape(-1).view(np.uint8), {"queueSubmissions": 0.0}, config)
         elapsed = (time.perf_counter() - started) * 1e3
         result.timing = {**result.timing, "totalWallMs": elapsed, "resultAgeWallMs": elapsed}
         return result
@@ -336,7 +346,7 @@ class _NativeEngine:
         _native.check(self._lib.ldic_register(self._handle, frame.ctypes.data, frame.strides[0],
                                               ctypes.addressof(out), ctypes.byref(t)))
         built = time.perf_counter()
-        result = self._result(raw, self._timing_dict(t, started, prep_ms), config)
+        result = self._result(raw, self._timing_dict(t, started, prep_ms), config, self._summary(t))
         result.timing["resultBuildWallMs"] = (time.perf_counter() - built) * 1e3
         return result
 
@@ -366,7 +376,7 @@ class _NativeEngine:
             t = _native.Timing()
             _native.check(self._lib.ldic_retire(self._handle, slot, ctypes.addressof(out), ctypes.byref(t)))
             free.append(slot)
-            return self._result(raw, self._timing_dict(t, started, prep_ms), config)
+            return self._result(raw, self._timing_dict(t, started, prep_ms), config, self._summary(t))
 
         for target in targets:
             if not free:
@@ -397,7 +407,7 @@ class _NativeEngine:
                                                    ctypes.byref(t)))
         self.reference_valid = valid.astype(bool)
         self.reference_valid.setflags(write=False)
-        return self._result(raw, self._timing_dict(t, started, prep_ms), config)
+        return self._result(raw, self._timing_dict(t, started, prep_ms), config, self._summary(t))
 
     def _set_level_configs(self, configs: list[dict]) -> None:
         for k, c in enumerate(configs):
@@ -572,7 +582,7 @@ class ICGNWGPU(_NativeEngine):
         _native.check(self._lib.ldic_register_seeded(
             self._handle, frame.ctypes.data, frame.strides[0], packed.ctypes.data, active.ctypes.data,
             1 if include_zncc else 0, ctypes.addressof(out), zncc.ctypes.data, ctypes.byref(t)))
-        result = self._result(raw, self._timing_dict(t, started, prep_ms), config)
+        result = self._result(raw, self._timing_dict(t, started, prep_ms), config, self._summary(t))
         if include_zncc:
             zncc.setflags(write=False)
             result._zncc = zncc

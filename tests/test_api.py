@@ -131,3 +131,20 @@ def test_adaptive_pyramid_arguments_are_rejected_before_device_creation():
                     {"adaptive_acceptance": 1.01}, {"adaptive_acceptance": True}):
         with pytest.raises(ValueError):
             ICGNWGPUFullPyramid(**options)
+
+
+def test_field_summary_host_statistics():
+    """FieldSummary.of: counts of status > 0.5 and > 1.5 and float64 sums over the accepted."""
+    from live_dic.engine import FieldSummary
+
+    rng = np.random.default_rng(8)
+    packed = rng.normal(size=(7, 9, 8)).astype(np.float32)
+    packed[..., 6] = rng.choice([-3.0, -1.0, 0.0, 1.0, 2.0], size=(7, 9))
+    s = FieldSummary.of(packed, 6)
+    ok = packed[..., 6] > 0.5
+    assert s.accepted == ok.sum() and s.replaced == (packed[..., 6] > 1.5).sum()
+    (mx, my), mi = s.means()
+    assert mx == pytest.approx(packed[..., 0][ok].astype(np.float64).mean(), rel=1e-15)
+    assert my == pytest.approx(packed[..., 3][ok].astype(np.float64).mean(), rel=1e-15)
+    assert mi == pytest.approx(packed[..., 7][ok].astype(np.float64).mean(), rel=1e-15)
+    assert FieldSummary(0, 0, 0.0, 0.0, 0.0).means() == (None, None)

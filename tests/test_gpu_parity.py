@@ -297,3 +297,24 @@ def test_consecutive_pair_mode_matches_set_reference_then_register():
     assert ok.sum() > 0.9 * paired.roi.selected.sum()
     assert np.all(paired.status[~paired.roi.selected] == -3)
     assert rmse <= 0.01
+
+
+def test_device_field_summary_matches_host_statistics(small_pair):
+    """The accepted/replaced counts and means a result carries come from the device's summary of
+    the field; they equal the same statistics taken on the host from the returned field."""
+    from live_dic import ICGNWGPUFullPyramid
+    from live_dic.engine import FieldSummary
+
+    ref, tgt, _ = small_pair
+    _say("device field summary 512 affine 21/8, replace")
+    with ICGNWGPUFullPyramid(pyramid_levels=2, subset_size=21, subset_stride=8, outlier_policy="flag") as eng:
+        eng.set_reference(ref)
+        results = [eng.register(tgt)] + list(eng.register_stream([tgt, ref, tgt], pipeline_depth=2))
+    for res in results:
+        packed = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], -1)
+        host = FieldSummary.of(packed, 6)
+        assert (res.accepted, res.replaced) == (host.accepted, host.replaced)
+        (mx, my), mi = host.means()
+        assert abs(res.mean_translation[0] - mx) <= 1e-12 * max(1.0, abs(mx))
+        assert abs(res.mean_translation[1] - my) <= 1e-12 * max(1.0, abs(my))
+        assert abs(res.mean_iterations - mi) <= 1e-12 * mi
