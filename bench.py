@@ -209,35 +209,23 @@ def learned_vs_zncc(reps: int = 3) -> dict:
     return result
 
 
-# Batched heterogeneous workload: 16 pairs of mixed sizes, subset sizes, steps and warp models,
-# random affine fields (t U[-5, 5] px, strain U[-1 %, 1 %], rotation U[-1, 1] deg).
-BATCH_SIZES = [(640, 480)] * 4 + [(1280, 720)] * 4 + [(1920, 1080)] * 4 + [(3840, 2160)] * 4
-BATCH_LEVELS = {640: 2, 1280: 3, 1920: 3, 3840: 4}
-
-
 def batched_heterogeneous(frames: int = 20) -> dict:
-    """Aggregate subsets/s of the 16-pair batch replayed as one HIP graph per frame, and the
-    p50/p99 latency of a frame (every pair of a frame completes with it)."""
-    from live_dic import ICGNWGPUFullPyramid, _native, seednet, synthetic
+    """Aggregate subsets/s of the 16-pair batch (synthetic.heterogeneous_batch) replayed as one HIP
+    graph per frame, and the p50/p99 latency of a frame (every pair of a frame completes with it)."""
+    from live_dic import ICGNWGPUFullPyramid, _native, synthetic
     from live_dic.batch import BatchRegistration
 
-    rng = np.random.default_rng(4242)
     engines, targets, fields, configs = [], [], [], []
-    for i, (w, h) in enumerate(BATCH_SIZES):
-        S = (15, 21, 31, 41)[i % 4]
-        D = min((4, 8, 16)[i % 3], S)
-        model = "quadratic" if i % 2 else "affine"
-        fld = seednet.random_affine(rng, w, h, translation=5.0, rotation_deg=1.0, strain=0.01)
-        tex = synthetic.SpeckleTexture(w, h, synthetic.blobs(w, h, int(round(synthetic.BLOB_DENSITY * w * h)), 77 + i))
-        ref = tex.render(None, noise_sigma=1.0, noise_seed=2 * i + 1)
-        tgt = _native.DeviceBuffer(w * h)
-        tex.render_device(tgt, fld, 1.0, 2 * i + 2)
-        eng = ICGNWGPUFullPyramid(pyramid_levels=BATCH_LEVELS[w], subset_size=S, subset_stride=D, motion_model=model)
-        eng.set_reference(ref)
+    for pair in synthetic.heterogeneous_batch():
+        tgt = _native.DeviceBuffer(pair.width * pair.height)
+        pair.target_device(tgt)
+        eng = ICGNWGPUFullPyramid(pyramid_levels=pair.levels, subset_size=pair.subset, subset_stride=pair.stride,
+                                  motion_model=pair.model)
+        eng.set_reference(pair.reference())
         engines.append(eng)
         targets.append(tgt)
-        fields.append(fld)
-        configs.append(f"{w}x{h} S{S} D{D} {model} L{BATCH_LEVELS[w]}")
+        fields.append(pair.field)
+        configs.append(pair.label())
     subsets = sum(e.grid_width * e.grid_height for e in engines)
     with BatchRegistration(engines, [t.address for t in targets]) as batch:
         table = batch.tune()

@@ -270,3 +270,47 @@ class QuadraticPairs:
             my -= (-j21 * rx + j11 * ry) / det
         u, v = self.field.displacement(mx, my)
         return (s1 - s0) * u, (s1 - s0) * v
+
+
+# The 16-pair heterogeneous batch: 4 each of 640x480, 1280x720, 1920x1080 and 3840x2160;
+# subset sizes 15/21/31/41 and steps 4/8/16 in rotation, affine and quadratic warps
+# alternating; random affine fields (translation U[-5, 5] px, rotation U[-1, 1] deg,
+# normal strains U[-1 %, 1 %]); each pair's pyramid depth follows its displacement range.
+BATCH_SIZES = [(640, 480)] * 4 + [(1280, 720)] * 4 + [(1920, 1080)] * 4 + [(3840, 2160)] * 4
+BATCH_LEVELS = {640: 2, 1280: 3, 1920: 3, 3840: 4}
+
+
+@dataclass(frozen=True)
+class BatchPair:
+    width: int
+    height: int
+    subset: int
+    stride: int
+    model: str
+    levels: int
+    texture: SpeckleTexture
+    field: Field
+    seed: int
+
+    def label(self) -> str:
+        return f"{self.width}x{self.height} S{self.subset} D{self.stride} {self.model} L{self.levels}"
+
+    def reference(self) -> np.ndarray:
+        return self.texture.render(None, noise_sigma=1.0, noise_seed=2 * self.seed + 1)
+
+    def target_device(self, out: _native.DeviceBuffer) -> None:
+        self.texture.render_device(out, self.field, 1.0, 2 * self.seed + 2)
+
+
+def heterogeneous_batch(seed: int = 4242) -> list[BatchPair]:
+    from .seednet import random_affine
+
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for i, (w, h) in enumerate(BATCH_SIZES):
+        S = (15, 21, 31, 41)[i % 4]
+        D = min((4, 8, 16)[i % 3], S)
+        fld = random_affine(rng, w, h, translation=5.0, rotation_deg=1.0, strain=0.01)
+        tex = SpeckleTexture(w, h, blobs(w, h, int(round(BLOB_DENSITY * w * h)), 77 + i))
+        pairs.append(BatchPair(w, h, S, D, "quadratic" if i % 2 else "affine", BATCH_LEVELS[w], tex, fld, i))
+    return pairs
