@@ -318,3 +318,50 @@ def test_device_field_summary_matches_host_statistics(small_pair):
         assert abs(res.mean_translation[0] - mx) <= 1e-12 * max(1.0, abs(mx))
         assert abs(res.mean_translation[1] - my) <= 1e-12 * max(1.0, abs(my))
         assert abs(res.mean_iterations - mi) <= 1e-12 * mi
+
+
+@pytest.mark.parametrize("size,stride,theta_deg", [(41, 16, 20.0), (31, 12, 25.0)])
+def test_seeded_large_rotation_uses_clamped_gathers_and_matches_oracle(size, stride, theta_deg):
+    """A rotation of 20-25 degrees, seeded near the truth, makes each subset's warped footprint
+    wider than the wave's LDS slice (41: (S + 9)^2, 31: 44-word rows), so the solve samples by
+    clamped L2 gathers from the first iteration: statuses and |du| against the oracle's seeded
+    solve on the same seeds.  (A 15-px subset's footprint stays inside its 32-wide tile at any
+    rotation.)"""
+    import math
+
+    from live_dic import ICGNWGPU, synthetic
+    from oracle import reference_prep as prep
+
+    w = h = 384
+    _say("seeded rotation", size, stride, theta_deg)
+    tex = synthetic.SpeckleTexture(w, h, synthetic.blobs(w, h, int(round(synthetic.BLOB_DENSITY * w * h)), 91))
+    fld = synthetic.affine_field(w, h, 0.6, -0.4, math.radians(theta_deg), 0.01, -0.005)
+    ref = tex.render(None, noise_sigma=1.0, noise_seed=5)
+    tgt = tex.render(fld, noise_sigma=1.0, noise_seed=6)
+    gw, gh = prep.grid_shape(w, h, size, stride)
+    cx, cy = synthetic.subset_centres(w, h, size, stride)
+    u, v = fld.displacement(cx, cy)
+    a = fld.poly
+    rng = np.random.default_rng(3)
+    seeds = np.zeros((gh, gw, 6), np.float32)
+    seeds[..., 0] = u + rng.uniform(-0.3, 0.3, u.shape)
+    seeds[..., 3] = v + rng.uniform(-0.3, 0.3, v.shape)
+    seeds[..., 1], seeds[..., 2], seeds[..., 4], seeds[..., 5] = a[1], a[2], a[4], a[5]
+    active = np.ones((gh, gw), bool)
+    with ICGNWGPU(subset_size=size, subset_stride=stride) as eng:
+        eng.set_reference(ref, iterations=20, tolerance=1e-4)
+        res = eng.register_seeded(tgt, seeds, active)
+        plane, meta, inv = eng.export_reference(0)
+    got = np.concatenate([res.parameters, res.status[..., None], res.iterations[..., None]], axis=-1)
+    cfg = oracle.SolveCfg(w, h, gw, gh, size, stride, 0, 0, 0, 0, 0.25 * size, 20, 1e-4, 0.8, 16, 0, 0, 4)
+    seed_state = np.zeros((gh * gw, 8), np.float32)
+    seed_state[:, :6] = seeds.reshape(-1, 6)
+    state = oracle.solve(cfg, plane.reshape(-1, 4), meta, inv, oracle.prefilter(tgt), tgt, seed_state,
+                         np.ones(gh * gw, np.uint32)).reshape(gh, gw, 8)
+    stats = compare_fields(got, state, 6)
+    ok = res.status == 1
+    err = np.hypot(res.parameters[..., 0] - u, res.parameters[..., 3] - v)[ok]
+    inside = (cx + u > size) & (cx + u < w - size) & (cy + v > size) & (cy + v < h - size)
+    print(stats, "accepted inside", ok[inside].mean(), "rmse", np.sqrt(np.mean(err**2)))
+    assert ok[inside].mean() > 0.9 and np.sqrt(np.mean(err**2)) < 0.02
+    assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
