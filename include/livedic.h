@@ -288,6 +288,21 @@ int ldic_export_level_state(struct ldic_engine* engine, int32_t slot, int32_t le
 int ldic_op_reduce(const uint8_t* source, int32_t width, int32_t height, uint8_t* destination);
 int ldic_op_prefilter(const uint8_t* source, int32_t width, int32_t height, float* coefficients);
 
+/* The outlier pass over a given packed state: the median_filter / UOD entry of
+ * icgn.wgsl:1325-1605 dispatched on a written state buffer, as the reference's
+ * own tests drive it (python/tests/test_gpu_regressions.py:860-905).  state and
+ * filtered hold grid_width*grid_height*(8|16) floats; metadata 16 (28 quadratic)
+ * floats per subset, of which the validity word (index 1) is read; roi_words is
+ * NULL or 4 floats per subset (the ROI tail of the metadata).  Codes follow
+ * ldic_options; seed_available marks a pass over a coarser pyramid level whose
+ * median also fills unusable subsets for the seed transfer. */
+typedef struct ldic_median_op {
+    int32_t grid_width, grid_height, subset_stride, motion_model;
+    int32_t outlier_policy, median_filter_size, median_filter_scope, seed_available;
+} ldic_median_op;
+int ldic_op_median(const ldic_median_op* config, const float* state, const float* metadata, const float* roi_words,
+                   float* filtered);
+
 /* ---- Workload generation and device memory (no counterpart in the reference,
  * whose benchmarks read PNG fixtures; used for the synthetic speckle pairs of
  * the measurements and for training data of the learned seed). ---- */

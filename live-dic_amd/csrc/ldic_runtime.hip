@@ -1640,6 +1640,57 @@ int ldic_op_prefilter(const uint8_t* src, int32_t w, int32_t h, float* coeff) {
     return LDIC_OK;
 }
 
+int ldic_op_median(const ldic_median_op* c, const float* state, const float* metadata, const float* roi_words,
+                   float* filtered) {
+    if (!c || !state || !metadata || !filtered) return set_error(LDIC_ERR_ARGUMENT, "config, state, metadata and "
+                                                                     "filtered are required");
+    if (c->grid_width < 1 || c->grid_height < 1 || c->subset_stride < 1)
+        return set_error(LDIC_ERR_ARGUMENT, "grid and stride must be positive");
+    if (c->motion_model < 0 || c->motion_model > 2 || c->outlier_policy < 0 || c->outlier_policy > 3)
+        return set_error(LDIC_ERR_ARGUMENT, "unknown motion model or outlier policy");
+    if (c->median_filter_size < 3 || c->median_filter_size > 9 || !(c->median_filter_size & 1) ||
+        (c->median_filter_scope != 0 && c->median_filter_scope != 1))
+        return set_error(LDIC_ERR_ARGUMENT, "median_filter_size must be 3, 5, 7 or 9 and the scope 0 or 1");
+    const size_t n = (size_t)c->grid_width * c->grid_height;
+    const size_t stride = (size_t)state_stride(c->motion_model), mstride = (size_t)metadata_stride(c->motion_model);
+    Arena a;
+    float *ds = nullptr, *dm = nullptr, *dw = nullptr, *df = nullptr;
+    int rc = a.alloc(&ds, n * stride);
+    if (rc == LDIC_OK) rc = a.alloc(&dm, n * mstride);
+    if (rc == LDIC_OK) rc = a.alloc(&df, n * stride);
+    if (rc == LDIC_OK && roi_words) rc = a.alloc(&dw, n * 4);
+    hipError_t he = hipSuccess;
+    if (rc == LDIC_OK) {
+        he = hipMemcpy(ds, state, sizeof(float) * n * stride, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(dm, metadata, sizeof(float) * n * mstride, hipMemcpyHostToDevice);
+        if (he == hipSuccess && dw) he = hipMemcpy(dw, roi_words, sizeof(float) * n * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemset(df, 0, sizeof(float) * n * stride);
+        if (he == hipSuccess) {
+            MedianArgs m{};
+            m.grid_w = c->grid_width;
+            m.grid_count = (int)n;
+            m.D = c->subset_stride;
+            m.model = c->motion_model;
+            m.policy = c->outlier_policy;
+            m.size = c->median_filter_size;
+            m.scope = c->median_filter_scope;
+            m.roi_enabled = dw ? 1 : 0;
+            m.seed_available = c->seed_available ? 1 : 0;
+            m.state = ds;
+            m.metadata = dm;
+            m.roi_words = dw;
+            m.filtered = df;
+            he = c->outlier_policy == kDisabled ? hipMemcpy(df, ds, sizeof(float) * n * stride, hipMemcpyDeviceToDevice)
+                                                 : launch_median(m, nullptr);
+        }
+        if (he == hipSuccess) he = hipMemcpy(filtered, df, sizeof(float) * n * stride, hipMemcpyDeviceToHost);
+    }
+    a.release();
+    if (rc != LDIC_OK) return rc;
+    if (he != hipSuccess) return set_error(LDIC_ERR_DEVICE, "median pass failed: %s", hipGetErrorString(he));
+    return LDIC_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

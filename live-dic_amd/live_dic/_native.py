@@ -54,6 +54,12 @@ class Field(ctypes.Structure):
     ]
 
 
+class MedianOp(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in (
+        "grid_width", "grid_height", "subset_stride", "motion_model", "outlier_policy", "median_filter_size",
+        "median_filter_scope", "seed_available")]
+
+
 class Speckle(ctypes.Structure):
     _fields_ = [
         ("device_blobs", ctypes.c_void_p),
@@ -171,6 +177,7 @@ SIGNATURES = {
     "ldic_export_level_state": (ctypes.c_int, [_P, _I32, _I32, _P]),
     "ldic_op_reduce": (ctypes.c_int, [_P, _I32, _I32, _P]),
     "ldic_op_prefilter": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_op_median": (ctypes.c_int, [ctypes.POINTER(MedianOp), _P, _P, _P, _P]),
     "ldic_render_speckle": (ctypes.c_int, [ctypes.POINTER(Speckle), ctypes.POINTER(Field), _I32, _I32,
                                            ctypes.c_float, ctypes.c_uint64, _P, _P, _P]),
     "ldic_render_speckle_batch": (ctypes.c_int, [ctypes.POINTER(Speckle), _P, _P, _I32, _I32, _I32,
