@@ -552,7 +552,11 @@ __device__ void uod_policy3(const MedianArgs& a, int i) {
     const float own_status = model == kQuadratic ? own[12] : own[6];
     if (own_status >= 0.5f) {
         Neighbours3 n;
-        neighbours3(a, i, a.roi_enabled && model != kQuadratic, n);
+        // ROI links are honoured for every model.  The reference's UOD reads them for
+        // translation and affine only (icgn.wgsl:1356-1400 has no ROI test) because it
+        // rejects quadratic warps with an ROI (roi.py:45-46); this build supports that
+        // combination, and a neighbour across a hole is not evidence about a subset.
+        neighbours3(a, i, a.roi_enabled, n);
         if (n.count >= 4) {
             const int model_count = model == kTranslation ? 2 : (model == kQuadratic ? 12 : 6);
             const int median_count = a.policy == kReplace ? model_count : 2;

@@ -910,7 +910,10 @@ void ora_median(const ora_median_cfg* c, const float* state, const float* metada
                 if (dx == 0 && dy == 0) continue;
                 int nx = px + dx, ny = py + dy;
                 if (nx < 0 || ny < 0 || nx >= c->grid_width || ny >= gh) continue;
-                if (c->roi_enabled && model != 2) {
+                /* ROI links for every model: icgn.wgsl's quadratic branch (:1356-1400) has
+                 * no ROI test because the reference rejects quadratic + ROI (roi.py:45-46);
+                 * the library supports it and honours the links there as for affine. */
+                if (c->roi_enabled) {
                     uint32_t perm = (uint32_t)roi_words[(size_t)i * 4];
                     uint32_t bit = (uint32_t)((dy + 1) * 3 + dx + 1);
                     if ((perm & (1u << bit)) == 0u) continue;

@@ -173,3 +173,18 @@ def test_registration_with_outlier_policies_matches_oracle(policy, size, scope):
     stats = compare_fields(got, want, 6)
     print(stats)
     assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
+
+
+@pytest.mark.parametrize("model", ["quadratic", "affine"])
+@pytest.mark.parametrize("policy", ["flag", "replace"])
+def test_uod_beside_a_hole_uses_only_linked_neighbours(model, policy):
+    """A subset beside a hole keeps its state when the ROI links exclude the neighbours across
+    the hole, for the quadratic model as for affine; without links it is outvoted."""
+    from test_oracle_median import hole_edge_field
+
+    st, words, centre, P = hole_edge_field(model)
+    meta = _valid_metadata(len(st), model)
+    linked = _gpu_median(st, meta, 5, 5, 16, model, policy, 3, words=words)
+    np.testing.assert_array_equal(linked[centre], st[centre])
+    unlinked = _gpu_median(st, meta, 5, 5, 16, model, policy, 3)
+    assert unlinked[centre, P] == 2.0

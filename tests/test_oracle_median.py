@@ -157,3 +157,35 @@ def test_uod_preserves_a_localized_affine_band():
     assert np.median(raw[:, band, 1][usable]) > 0.003
     np.testing.assert_allclose(filt[:, band, 1][usable], raw[:, band, 1][usable], rtol=0, atol=1e-7)
     assert not np.any(filt[:, band, 6][usable] == 2.0)
+
+
+def hole_edge_field(model="quadratic"):
+    """A 5x5 grid with a hole to the right of and below the centre subset: the five neighbours
+    across it (column > 2 or row > 2) moved 3 px more.  The centre agrees with its own side, and
+    its link word allows only the neighbours up and to the left."""
+    P = 12 if model == "quadratic" else 6
+    side = 5
+    st = np.zeros((side * side, 16 if model == "quadratic" else 8), np.float32)
+    gy, gx = np.divmod(np.arange(side * side), side)
+    st[:, 0] = 0.4 + np.where((gx > 2) | (gy > 2), 3.0, 0.0)
+    st[:, 3] = -0.3
+    st[:, P] = 1.0
+    st[:, P + 1] = 5.0
+    words = np.zeros((side * side, 4), np.float32)
+    words[:, 0] = sum(1 << ((dy + 1) * 3 + dx + 1) for dy in (-1, 0) for dx in (-1, 0) if dx or dy)
+    return st, words, side * 2 + 2, P
+
+
+@pytest.mark.parametrize("model", ["quadratic", "affine"])
+@pytest.mark.parametrize("policy", ["flag", "replace"])
+def test_uod_beside_a_hole_uses_only_linked_neighbours(model, policy):
+    """The UOD of a subset beside a hole: with the ROI links its neighbours across the hole are
+    ignored (three linked neighbours are too few to judge it) and it stays accepted and unchanged;
+    without them the five across-hole neighbours outvote it.  Quadratic follows the affine rule
+    (the reference defines no quadratic ROI)."""
+    st, words, centre, P = hole_edge_field(model)
+    meta = _valid_metadata(len(st), model)
+    linked = oracle.median(_median_cfg(5, 16, model, policy, 3, roi=True), st, meta, words)
+    np.testing.assert_array_equal(linked[centre], st[centre])
+    unlinked = oracle.median(_median_cfg(5, 16, model, policy, 3), st, meta)
+    assert unlinked[centre, P] == 2.0
