@@ -125,6 +125,9 @@ int ldic_register(struct ldic_engine* engine, const uint8_t* target, int64_t row
 int ldic_register_pair(struct ldic_engine* engine, const uint8_t* reference, int64_t reference_stride,
                        const uint8_t* target, int64_t target_stride, float* out_state,
                        uint8_t* out_reference_valid, ldic_timing* timing);
+/* The reference buffers are engine-wide: both pair calls, and
+ * ldic_prepare_reference_resident, refuse (LDIC_ERR_STATE) while another slot
+ * has queued work; references in flight on several slots need one engine each. */
 int ldic_register_pair_resident(struct ldic_engine* engine, int32_t slot, const uint8_t* device_reference,
                                 const uint8_t* device_target, int64_t row_stride);
 
@@ -248,8 +251,10 @@ int ldic_op_zncc_search(const uint8_t* reference, const uint8_t* target, int32_t
  * a solver variant run in one persistent launch whose grid size is picked by
  * timing candidates on the device.  The targets are device planes (row pitch =
  * width) read at every launch; results stay in each engine's slot 0 and are
- * read with ldic_slot_output after ldic_batch_wait.  The engines must not be
- * used otherwise while the batch exists. ---- */
+ * read with ldic_slot_output after ldic_batch_wait.  While the batch exists its
+ * engines refuse (LDIC_ERR_STATE) every call that would re-prepare, reconfigure or
+ * queue work on them, and ldic_destroy leaves them alive with that error set:
+ * destroy the batch first. ---- */
 typedef struct ldic_batch ldic_batch;
 int ldic_batch_create(struct ldic_engine* const* engines, int32_t count, const uint8_t* const* device_targets,
                       ldic_batch** out_batch);

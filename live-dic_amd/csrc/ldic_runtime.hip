@@ -160,6 +160,9 @@ struct Slot {
 
 struct ldic_engine {
     ldic_options opt{};
+    // Batches (ldic_batch_create) that hold this engine's device addresses: while any
+    // does, calls that would free, re-prepare, reconfigure or use slot 0 are refused.
+    int batch_refs = 0;
     int device = 0;
     hipDeviceProp_t prop{};
     int levels = 0;
@@ -805,6 +808,26 @@ int check_slot(ldic_engine* e, int slot) {
     return LDIC_OK;
 }
 
+// A batch captured this engine's buffers and solver settings.
+int check_unbatched(const ldic_engine* e) {
+    if (e && e->batch_refs > 0)
+        return set_error(LDIC_ERR_STATE, "the engine belongs to a batch; destroy the batch (ldic_batch_destroy) first");
+    return LDIC_OK;
+}
+
+// The reference buffers are engine-wide: a slot may replace them only when no other
+// slot has queued work that reads them.  (Separate references in flight need one
+// engine per reference.)
+int check_others_idle(const ldic_engine* e, int slot) {
+    for (int j = 0; j < (int)e->slots.size(); ++j)
+        if (j != slot && e->slots[j]->pending)
+            return set_error(LDIC_ERR_STATE,
+                             "slot %d has queued work that reads the current reference; wait for it "
+                             "(ldic_retire or ldic_slot_wait) before replacing the reference on slot %d",
+                             j, slot);
+    return LDIC_OK;
+}
+
 // The adaptive pyramid's feedback from a retired frame (full_pyramid_backend.py:
 // 1781-1791): the finest level alone next time if the merged finest subsets reach
 // the acceptance fraction of the reference-valid ones and none expired.
@@ -1143,6 +1166,9 @@ int ldic_create(const ldic_options* options, ldic_engine** out_engine) {
 
 void ldic_destroy(ldic_engine* e) {
     if (!e) return;
+    // A batch still holds the engine's device addresses: destroying it would leave the
+    // batch launching on freed memory.  The engine stays; the error says why.
+    if (check_unbatched(e) != LDIC_OK) return;
     (void)hipSetDevice(e->device);
     free_slots(e);
     e->ref_arena.release();
@@ -1162,6 +1188,7 @@ int ldic_device_name(ldic_engine* e, char* buffer, int32_t capacity) {
 int ldic_set_reference(ldic_engine* e, const uint8_t* image, int32_t width, int32_t height, int64_t row_stride,
                        const ldic_level_config* configs, const ldic_level_roi* rois, uint8_t* out_reference_valid) {
     if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
+    CHECK(check_unbatched(e));
     return set_reference_impl(e, image, width, height, row_stride, configs, rois, out_reference_valid);
 }
 
@@ -1177,6 +1204,7 @@ int ldic_level_shape(ldic_engine* e, int32_t level, int32_t* width, int32_t* hei
 }
 
 int ldic_set_level_config(ldic_engine* e, int32_t level, const ldic_level_config* c) {
+    CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     if (level < 0 || level >= e->levels || !c) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
     CHECK(validate_config(c));
@@ -1185,12 +1213,14 @@ int ldic_set_level_config(ldic_engine* e, int32_t level, const ldic_level_config
 }
 
 int ldic_set_slots(ldic_engine* e, int32_t count) {
+    CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     if (count < 1 || count > 64) return set_error(LDIC_ERR_ARGUMENT, "slot count must be 1..64");
     return ensure_slots(e, count);
 }
 
 int ldic_register(ldic_engine* e, const uint8_t* target, int64_t row_stride, float* out_state, ldic_timing* t) {
+    CHECK(check_unbatched(e));
     double wall = now_ms();
     CHECK(check_ready(e));
     if (!target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "target and out_state are required");
@@ -1203,6 +1233,7 @@ int ldic_register(ldic_engine* e, const uint8_t* target, int64_t row_stride, flo
 
 int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t reference_stride, const uint8_t* target,
                        int64_t target_stride, float* out_state, uint8_t* out_reference_valid, ldic_timing* t) {
+    CHECK(check_unbatched(e));
     double wall = now_ms();
     CHECK(check_ready(e));
     if (!reference || !target || !out_state) return set_error(LDIC_ERR_ARGUMENT, "reference, target and out_state are required");
@@ -1216,6 +1247,7 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
     HIP_CHECK(hipSetDevice(e->device));
     Slot* s = e->slots[0];
     if (s->pending) return set_error(LDIC_ERR_STATE, "slot 0 holds an unretired frame");
+    CHECK(check_others_idle(e, 0));
     // A new reference starts a new temporal history.
     if (e->init.kind == LDIC_INIT_TEMPORAL) HIP_CHECK(reset_histories(e, s->stream));
     for (int y = 0; y < L0.height; ++y)
@@ -1234,6 +1266,7 @@ int ldic_register_pair(ldic_engine* e, const uint8_t* reference, int64_t referen
 
 int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* device_reference,
                                 const uint8_t* device_target, int64_t row_stride) {
+    CHECK(check_unbatched(e));
     CHECK(check_slot(e, slot));
     if (!device_reference || !device_target) return set_error(LDIC_ERR_ARGUMENT, "device pointers are required");
     if (has_partial(e))
@@ -1242,6 +1275,7 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
         return set_error(LDIC_ERR_STATE, "consecutive-pair mode does not support the adaptive pyramid");
     Slot* s = e->slots[slot];
     if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    CHECK(check_others_idle(e, slot));
     HIP_CHECK(hipSetDevice(e->device));
     const Level& L0 = e->lv[0];
     if (!s->pending) HIP_CHECK(hipEventRecord(s->ev_begin, s->stream));
@@ -1261,6 +1295,7 @@ int ldic_register_pair_resident(ldic_engine* e, int32_t slot, const uint8_t* dev
 }
 
 int ldic_submit(ldic_engine* e, int32_t slot, const uint8_t* target, int64_t row_stride) {
+    CHECK(check_unbatched(e));
     CHECK(check_slot(e, slot));
     if (!target) return set_error(LDIC_ERR_ARGUMENT, "target is NULL");
     if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
@@ -1279,6 +1314,7 @@ int ldic_retire(ldic_engine* e, int32_t slot, float* out_state, ldic_timing* t) 
 int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stride, const float* seed_state,
                          const uint8_t* active, int32_t include_zncc, float* out_state, float* out_zncc,
                          ldic_timing* t) {
+    CHECK(check_unbatched(e));
     double wall = now_ms();
     CHECK(check_ready(e));
     if (!target || !seed_state || !active || !out_state)
@@ -1314,6 +1350,7 @@ int ldic_register_seeded(ldic_engine* e, const uint8_t* target, int64_t row_stri
 }
 
 int ldic_register_resident(ldic_engine* e, int32_t slot, const uint8_t* device_target, int64_t row_stride) {
+    CHECK(check_unbatched(e));
     CHECK(check_slot(e, slot));
     if (!device_target) return set_error(LDIC_ERR_ARGUMENT, "device_target is NULL");
     Slot* s = e->slots[slot];
@@ -1334,6 +1371,7 @@ int ldic_register_resident(ldic_engine* e, int32_t slot, const uint8_t* device_t
 
 int ldic_register_seeded_resident(ldic_engine* e, int32_t slot, const uint8_t* device_target, int64_t row_stride,
                                   const float* device_seed_state, const uint32_t* device_active) {
+    CHECK(check_unbatched(e));
     CHECK(check_slot(e, slot));
     if (!device_target || !device_seed_state || !device_active)
         return set_error(LDIC_ERR_ARGUMENT, "device target, seed_state and active are required");
@@ -1360,12 +1398,14 @@ int ldic_register_seeded_resident(ldic_engine* e, int32_t slot, const uint8_t* d
 
 int ldic_prepare_reference_resident(ldic_engine* e, int32_t slot, const uint8_t* device_reference,
                                     int64_t row_stride) {
+    CHECK(check_unbatched(e));
     CHECK(check_slot(e, slot));
     if (!device_reference) return set_error(LDIC_ERR_ARGUMENT, "device_reference is NULL");
     if (has_partial(e))
         return set_error(LDIC_ERR_STATE, "a new reference would discard the partial-support statistics");
     Slot* s = e->slots[slot];
     if (s->pending && !s->resident) return set_error(LDIC_ERR_STATE, "slot %d holds an unretired frame", slot);
+    CHECK(check_others_idle(e, slot));
     if (row_stride < e->lv[0].width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
     HIP_CHECK(hipSetDevice(e->device));
     const Level& L0 = e->lv[0];
@@ -1381,6 +1421,7 @@ int ldic_prepare_reference_resident(ldic_engine* e, int32_t slot, const uint8_t*
 }
 
 int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
+    CHECK(check_unbatched(e));
     if (!e) return set_error(LDIC_ERR_ARGUMENT, "engine is NULL");
     CHECK(validate_initialization(e, init));
     for (Slot* s : e->slots)
@@ -1394,6 +1435,7 @@ int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
 int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_t* indices,
                      const uint32_t* support_words, const float* boxes, const float* metadata_rows,
                      const float* inverse_rows) {
+    CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
     if (count < 0 || (count > 0 && (!indices || !support_words || !boxes || !metadata_rows || !inverse_rows)))
@@ -1436,6 +1478,7 @@ int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_
 }
 
 int ldic_reset_temporal(ldic_engine* e) {
+    CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     for (Slot* s : e->slots)
         if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
@@ -1446,6 +1489,7 @@ int ldic_reset_temporal(ldic_engine* e) {
 }
 
 int ldic_set_temporal_links(ldic_engine* e, const uint32_t* coarse_links) {
+    CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     HIP_CHECK(hipSetDevice(e->device));
     if (!coarse_links) {
@@ -1724,6 +1768,7 @@ struct ldic_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool launched = false;
+    bool holds = false;  // the engines' batch_refs count this batch
 };
 
 namespace {
@@ -1966,6 +2011,8 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
         if (e->slots[0]->pending) return fail(set_error(LDIC_ERR_STATE, "an engine's slot 0 holds a pending frame"));
     int rc = batch_build(b);
     if (rc != LDIC_OK) return fail(rc);
+    for (ldic_engine* e : b->engines) ++e->batch_refs;
+    b->holds = true;
     *out = b;
     return LDIC_OK;
 }
@@ -2051,6 +2098,8 @@ void ldic_batch_destroy(ldic_batch* b) {
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->holds)
+        for (ldic_engine* e : b->engines) --e->batch_refs;
     delete b;
 }
 

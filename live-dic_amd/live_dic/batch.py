@@ -20,16 +20,37 @@ from .engine import ICGNError
 
 
 class BatchRegistration:
+    """Holds each member engine's registration lock and the library's batch reference from
+    creation to close(): meanwhile the engines refuse registrations (``ICGNError``), and the
+    library refuses to re-prepare, reconfigure or destroy them."""
+
     def __init__(self, engines, device_targets):
         if len(engines) != len(device_targets) or not engines:
             raise ValueError("one device target per engine is required")
+        if len({id(e) for e in engines}) != len(engines):
+            raise ValueError("an engine can contribute only one pair to a batch")
         self.engines = list(engines)
         self._lib = _native.library()
-        handles = (ctypes.c_void_p * len(engines))(*[e._handle.value for e in engines])
-        targets = (ctypes.c_void_p * len(engines))(*[int(t) for t in device_targets])
         self._handle = ctypes.c_void_p()
-        _native.check(self._lib.ldic_batch_create(handles, len(engines), targets, ctypes.byref(self._handle)))
+        self._held = []
+        try:
+            for e in self.engines:
+                if not e._lock.acquire(blocking=False):
+                    raise ICGNError("Another registration or registration stream is active.")
+                self._held.append(e)
+            handles = (ctypes.c_void_p * len(engines))(*[e._handle.value for e in engines])
+            targets = (ctypes.c_void_p * len(engines))(*[int(t) for t in device_targets])
+            _native.check(self._lib.ldic_batch_create(handles, len(engines), targets, ctypes.byref(self._handle)))
+        except BaseException:
+            self._release_engines()
+            raise
+        for e in self.engines:
+            e._batch_holders += 1
         self.tuned = False
+
+    def _release_engines(self) -> None:
+        while self._held:
+            self._held.pop()._lock.release()
 
     def tune(self) -> list[dict]:
         """Run the autotuner and capture the frame graph; returns the launch table."""
@@ -67,6 +88,9 @@ class BatchRegistration:
         if self._handle:
             self._lib.ldic_batch_destroy(self._handle)
             self._handle = ctypes.c_void_p()
+            for e in self.engines:
+                e._batch_holders -= 1
+        self._release_engines()
 
     def __enter__(self):
         return self

@@ -130,6 +130,7 @@ class _NativeEngine:
         if self.device_type == "cpu":
             raise ICGNError("device_type='cpu' has no HIP device; the engine runs on the GPU only")
         self._lock = threading.Lock()
+        self._batch_holders = 0
         self._closed = False
         self._reference_ready = False
         self._handle = ctypes.c_void_p()
@@ -158,6 +159,8 @@ class _NativeEngine:
     def _prepare_reference(self, image: ImageInput, configs: list[dict], roi_mask, roi_policy, roi_truncation):
         if self._closed:
             raise ICGNError("This engine is closed; create a new engine before set_reference().")
+        if self._batch_holders:
+            raise ICGNError("The engine belongs to an open BatchRegistration; close the batch first.")
         frame = _as_uint8_gray(image)
         height, width = frame.shape
         mask = validate_roi(roi_mask, roi_policy, frame.shape, self.motion_model, roi_truncation)
@@ -469,6 +472,8 @@ class _NativeEngine:
         """Release device resources permanently; repeated calls are harmless."""
         if self._closed:
             return
+        if self._batch_holders:
+            raise ICGNError("The engine belongs to an open BatchRegistration; close the batch first.")
         self._reference_ready = False
         self._closed = True
         if self._handle:
