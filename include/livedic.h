@@ -391,6 +391,17 @@ int ldic_nn_conv_forward(const ldic_nn_conv* conv, const uint16_t* x, const uint
 int ldic_nn_conv_forward_split(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
                                int32_t leaky, uint16_t* y, float* y_f32, float* workspace, int32_t splits,
                                void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the split-K
+ * forward above with its partial sums in a library-owned static scratch of
+ * 1,048,576 floats (4 MiB) per device instead of a workspace argument; it calls
+ * no allocator.  batch * out_height * out_width * out_channels must not exceed
+ * that (LDIC_ERR_ARGUMENT otherwise).  All calls on one device share its
+ * scratch, zeroed by a kernel at the start of each call: calls that overlap on
+ * one device race. */
+int ldic_nn_conv_forward_split_static(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w,
+                                      const float* bias, int32_t leaky, uint16_t* y, float* y_f32, int32_t splits,
+                                      void* stream);
 /* Gradients from dy (float32, layout of the output, taken after the
  * activation's backward): dx (nullable; float32 with its own channel stride
  * and offset, overwritten or accumulated) needs the transposed weights; dw

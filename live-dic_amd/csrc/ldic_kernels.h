@@ -15,6 +15,9 @@ constexpr int kWavesPerBlock = 4;   // most subsets (waves) per persistent batch
 // slowest wave of its block has finished.
 constexpr int kSolveBlockWaves = 1;
 
+// Sets the calling thread's ldic_last_error() message and returns `code`.
+int report_error(int code, const char* message);
+
 inline int param_count(int model) { return model == kQuadratic ? 12 : 6; }
 inline int state_stride(int model) { return model == kQuadratic ? 16 : 8; }
 inline int metadata_stride(int model) { return model == kQuadratic ? 28 : 16; }

@@ -38,6 +38,15 @@ int set_error(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int ldic::report_error(int code, const char* message) {
+    g_error = message;
+    return code;
+}
+
+namespace {
+
 #define HIP_CHECK(expr)                                                                                \
     do {                                                                                               \
         hipError_t e_ = (expr);                                                                        \

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "ldic_device.h"
 #include "livedic.h"
@@ -668,6 +669,44 @@ void conv_fwd_launch(int tile, bool vec, dim3 grid, hipStream_t st, const ldic_n
 }  // namespace
 }  // namespace ldic
 
+namespace ldic {
+int report_error(int code, const char* message);  // ldic_runtime.hip
+}
+
+namespace {
+
+// Scratch of the allocation-free split-K forward (ldic_nn_conv_forward_split_static):
+// the float32 partial sums of one call, at most kStaticSplitFloats of them.  Each
+// GPU the library runs on holds its own copy until the process ends.  Host code
+// never passes `scratch` itself to a kernel: on the host it names a stand-in, not
+// a device address; static_scratch() looks up each device's address once.
+constexpr size_t kStaticSplitFloats = size_t(1) << 20;  // 4 MiB
+__device__ __align__(256) float scratch[kStaticSplitFloats];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, from hipGetSymbolAddress on the first
+// call on that device (made with the device current).
+int static_scratch(float** out) {
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxScratchDevices)
+        return ldic::report_error(LDIC_ERR_DEVICE, "no usable current HIP device for the static scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return ldic::report_error(LDIC_ERR_DEVICE, "hipGetSymbolAddress of the static scratch failed");
+        g_scratch[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch[device];
+    return LDIC_OK;
+}
+
+}  // namespace
+
 using namespace ldic;
 
 extern "C" {
@@ -708,6 +747,23 @@ int ldic_nn_conv_forward_split(const ldic_nn_conv* g, const uint16_t* x, const u
     conv_fwd_launch<true>(tile, vec, grid, st, *g, x, w, nullptr, 0, nullptr, workspace, kps);
     k_conv_epilogue<<<blocks(count), 256, 0, st>>>(*g, workspace, bias, leaky, y, y_f32);
     return status(hipGetLastError());
+}
+
+int ldic_nn_conv_forward_split_static(const ldic_nn_conv* g, const uint16_t* x, const uint16_t* w, const float* bias,
+                                      int32_t leaky, uint16_t* y, float* y_f32, int32_t splits, void* stream) {
+    if (check_conv(g) != LDIC_OK || !x || !w || (!y && !y_f32) || splits < 1)
+        return report_error(LDIC_ERR_ARGUMENT, "invalid convolution arguments");
+    const size_t count = (size_t)g->batch * g->out_height * g->out_width * g->out_channels;
+    if (splits > 1 && count > kStaticSplitFloats)
+        return report_error(LDIC_ERR_ARGUMENT,
+                            "the split-K partial sums (batch * out_height * out_width * out_channels floats) exceed "
+                            "the static scratch of 1048576 floats; use ldic_nn_conv_forward_split with a workspace");
+    float* workspace = nullptr;
+    if (splits > 1) {
+        const int rc = static_scratch(&workspace);
+        if (rc != LDIC_OK) return rc;
+    }
+    return ldic_nn_conv_forward_split(g, x, w, bias, leaky, y, y_f32, workspace, splits, stream);
 }
 
 int ldic_nn_conv_backward(const ldic_nn_conv* g, const uint16_t* x, const uint16_t* w_transposed, const float* dy,

@@ -186,6 +186,8 @@ SIGNATURES = {
     "ldic_slot_stream": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_P)]),
     "ldic_nn_conv_forward": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _I32, _P, _P, _P]),
     "ldic_nn_conv_forward_split": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _I32, _P, _P, _P, _I32, _P]),
+    "ldic_nn_conv_forward_split_static": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _I32, _P, _P, _I32,
+                                                         _P]),
     "ldic_nn_conv_backward": (ctypes.c_int, [ctypes.POINTER(NNConv), _P, _P, _P, _P, _I32, _I32, _I32, _P, _P,
                                              _P]),
     "ldic_nn_transpose_weights": (ctypes.c_int, [_I32, _I32, _P, _P, _P]),
