@@ -411,9 +411,11 @@ def numpy_boundary(eng, host_frames, steps: int, warmup: int, depth: int = 2) ->
         values = [t[key] for t in timings if key in t]
         return float(np.median(values)) if values else None
 
+    solve = [t["solveGpuMs"] for t in timings if t.get("solveGpuMs")]
     return {
         "frames_per_s": steps / elapsed,
         "ms_per_frame": elapsed / steps * 1e3,
+        "fine_solve_ms_mean": float(np.mean(solve)) if solve else None,
         "pipeline_depth": depth,
         "split_ms_median": {key: median(key) for key in (
             "inputPreparationWallMs", "uploadWallMs", "gpuMs", "solveGpuMs", "completionWaitWallMs",
@@ -554,7 +556,10 @@ def main() -> int:
     boundary["accepted_fraction_last_frame"] = float(ok2.mean())
     eng.close()
 
-    avg_solve_ms = solve_total / solve_count if solve_count else float("nan")
+    resident_solve_ms = solve_total / solve_count if solve_count else float("nan")
+    # The dominant kernel's duration over the headline's timed region: the fine solve's HIP
+    # events on its slot stream, one per NumPy-boundary frame.
+    avg_solve_ms = boundary["fine_solve_ms_mean"] or resident_solve_ms
     traffic_bytes = n_fine * subset_bytes(SUBSET)
     achieved = traffic_bytes / (avg_solve_ms * 1e-3) / 1e9
     alone_ms = alone_total / alone_count if alone_count else None
@@ -597,6 +602,7 @@ def main() -> int:
             "l2_hit_rate": MEASURED_FINE_SOLVE_L2_HIT_RATE,
             "kernel": "k_solve<affine, bspline> (finest level)",
             "kernel_avg_ms": avg_solve_ms,
+            "kernel_avg_ms_resident_run": resident_solve_ms,
             "algorithmic_bytes_per_launch": traffic_bytes,
             # The timed region runs two frames concurrently, so each launch shares the GPU;
             # the same kernel on one stream, measured after the timed region:
