@@ -192,15 +192,17 @@ typedef struct ldic_initialization {
 
 /* Partial-support subsets of one level (roi_policy="partial"; roi.py:266-355,
  * partial.py:26-130): their list, support bits (ceil(S*S/32) little-endian
- * words each, row-major pixels), local support boxes (xmin, ymin, xmax, ymax)
- * and the float64-derived metadata rows (validity -4 when admitted) and
- * inverse Hessians computed by the host planner from the prepared reference
- * (ldic_export_reference).  Call after ldic_set_reference; each frame then
- * solves these subsets in a partial pass after the full-window pass
- * (solve_partial*, icgn.wgsl:1040-1085).  Translation and affine only. */
-int ldic_set_partial(struct ldic_engine* engine, int32_t level, int32_t count, const uint32_t* indices,
-                     const uint32_t* support_words, const float* boxes, const float* metadata_rows,
-                     const float* inverse_rows);
+ * words each, row-major pixels) and local support boxes (xmin, ymin, xmax,
+ * ymax), from ldic_roi_planner_partial.  The device computes each support's
+ * float64 statistics from the prepared reference of that level and overwrites
+ * its metadata row and inverse Hessian (validity -4 when admitted, 0 and a zero
+ * inverse otherwise); out_diagnostics (nullable, 4 floats per subset) receives
+ * admitted (1/0), Hessian rank, normalised condition and weighted RMS contrast.
+ * Call after ldic_set_reference; each frame then solves these subsets in a
+ * partial pass after the full-window pass (solve_partial*, icgn.wgsl:1040-1085).
+ * Translation and affine only. */
+int ldic_prepare_partial(struct ldic_engine* engine, int32_t level, int32_t count, const uint32_t* indices,
+                         const uint32_t* support_words, const float* boxes, float* out_diagnostics);
 
 /* Forget the temporal history (reset_temporal_state, full_pyramid_backend.py:
  * 660-677).  ldic_set_reference and a change of initialisation also reset it.
@@ -307,6 +309,43 @@ typedef struct ldic_median_op {
 } ldic_median_op;
 int ldic_op_median(const ldic_median_op* config, const float* state, const float* metadata, const float* roi_words,
                    float* filtered);
+
+/* ---- ROI planning on the device: the static per-level tables of the reference's
+ * host planner (roi.py:63-97 selection and reduced masks, :178-263 neighbour and
+ * median link words, :266-355 partial supports, :358-437 seed links), computed
+ * from the boolean mask by HIP kernels.  A planner holds the mask of every
+ * pyramid level (reduced on the device, roi.py:89-97) and their summed-area
+ * tables; plan() fills one level's tables; seed links read two planned levels. ---- */
+typedef struct ldic_roi_planner ldic_roi_planner;
+typedef struct ldic_roi_plan_options {
+    int32_t subset_size, subset_stride;
+    int32_t median_filter_size;  /* 3: link bits only; 5-9: also the 81-bit median words */
+    int32_t partial;             /* roi_policy="partial": supports of incomplete windows (S <= 64) */
+    int32_t visibility;          /* roi_truncation="visibility" */
+} ldic_roi_plan_options;
+/* mask: width*height bytes, non-zero = allowed. */
+int ldic_roi_planner_create(const uint8_t* mask, int32_t width, int32_t height, int32_t levels,
+                            int32_t device_ordinal, ldic_roi_planner** out_planner);
+/* Size, contents (nullable) and "every pixel allowed" flag of one level's mask. */
+int ldic_roi_planner_mask(ldic_roi_planner* planner, int32_t level, int32_t* width, int32_t* height,
+                          uint8_t* out_mask, int32_t* all_allowed);
+/* One level's tables over its gh*gw grid: selection, 3x3 link bits, median words
+ * (gh*gw*4, nullable; filled for median_filter_size > 3), support coverage (nullable;
+ * 1 for full windows, the support fraction of partial ones) and the number of
+ * partial subsets, fetched by ldic_roi_planner_partial. */
+int ldic_roi_planner_plan(ldic_roi_planner* planner, int32_t level, const ldic_roi_plan_options* options,
+                          uint8_t* out_selected, uint32_t* out_neighbour_bits, float* out_median_words,
+                          float* out_coverage, int32_t* out_partial_count);
+/* Partial subsets of the last plan of `level`, in subset order: flat indices, support
+ * bits (ceil(S*S/32) words each, row-major, pixel 0 in bit 0) and boxes (local
+ * xmin, ymin, xmax, ymax). */
+int ldic_roi_planner_partial(ldic_roi_planner* planner, int32_t level, uint32_t* out_indices,
+                             uint32_t* out_support_words, float* out_support_boxes);
+/* Transfer links into destination_level from source_level (both planned): the
+ * pyramid transfer (source coarser) or the temporal projection (source finer). */
+int ldic_roi_planner_seed_links(ldic_roi_planner* planner, int32_t source_level, int32_t destination_level,
+                                uint32_t* out_links);
+void ldic_roi_planner_destroy(ldic_roi_planner* planner);
 
 /* ---- Workload generation and device memory (no counterpart in the reference,
  * whose benchmarks read PNG fixtures; used for the synthetic speckle pairs of

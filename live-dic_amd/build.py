@@ -25,6 +25,7 @@ SOURCES = (
     "ldic_render.hip",
     "ldic_seed.hip",
     "ldic_seednet.hip",
+    "ldic_roi.hip",
     "ldic_runtime.hip",
 )
 ARCH = os.environ.get("LDIC_OFFLOAD_ARCH", "gfx950")

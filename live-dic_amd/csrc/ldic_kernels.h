@@ -118,6 +118,20 @@ struct PrepArgs {
     uint8_t* valid;             // reference validity before the ROI override
 };
 
+// Float64 statistics of partial-support subsets written into a level's metadata and
+// inverse rows (translation or affine; 16-float rows, 36-float inverses).
+struct PartialStatsArgs {
+    int width, grid_w, S, D, model, count;
+    const float4* reference;   // prepared plane (f, gx, gy, 0)
+    const double* weights;     // S*S integration weights, or null (uniform)
+    const uint32_t* indices;   // flat subset index of each partial subset
+    const uint32_t* support;   // ceil(S*S/32) words per subset
+    float* metadata;
+    float* inverse;
+    float* diagnostics;        // 4 per subset: admitted, rank, normalised condition, RMS contrast
+};
+hipError_t launch_partial_stats(const PartialStatsArgs& a, hipStream_t s);
+
 // Pick the register-cached solver variant for a subset width.
 int solver_pixels_per_lane(int S, int model);
 // Compile-time LDS row pitch of the solver variant (0: the staged width).

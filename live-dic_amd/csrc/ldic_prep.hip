@@ -543,6 +543,204 @@ __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows
     }
 }
 
+// ---------------------------------------------------------------------------
+// Partial-support statistics (roi_policy="partial"; the reference computes them on
+// the host, live_dic/partial.py:26-130).  One wave per partial subset, float64, from
+// the prepared float32 plane (f, gx, gy) over the support pixels only:
+//   pass 1: weight sum and weighted mean of f;
+//   pass 2: with fc = f - mean, the energy sum w fc^2, J^T fc, sum w J and the
+//           Hessian sum w J J^T (J = [gx, gx x, gx y, gy, gy x, gy y]);
+// then, on one lane, the admission gates: full rank of the Hessian scaled by
+// (1, 1/S, 1/S, 1, 1/S, 1/S) (eigenvalues by cyclic Jacobi rotations; rank counts
+// those above eps * P * the largest), its condition <= 1e8, weighted RMS contrast
+// >= 1 and more support pixels than parameters; an admitted subset gets its inverse
+// (closed form for translation, partially pivoted Gauss-Jordan for affine), which
+// must be finite and representable in float32, and validity -4.
+// ---------------------------------------------------------------------------
+constexpr double kPartialMaxCondition = 1e8;
+constexpr double kPartialMinContrast = 1.0;
+
+__device__ void jacobi_eigenvalues(double (&a)[6][6], int n, double* eig) {
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += a[i][i] * a[i][i];
+            for (int j = i + 1; j < n; ++j) off += a[i][j] * a[i][j];
+        }
+        if (off <= 1e-40 * diag || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - sn * akq;
+                    a[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - sn * aqk;
+                    a[q][k] = sn * apk + c * aqk;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) eig[i] = a[i][i];
+    for (int i = 1; i < n; ++i)  // ascending
+        for (int j = i; j > 0 && eig[j] < eig[j - 1]; --j) {
+            const double t = eig[j];
+            eig[j] = eig[j - 1];
+            eig[j - 1] = t;
+        }
+}
+
+// In-place inverse of an n x n matrix by Gauss-Jordan with partial pivoting; false if singular.
+__device__ bool gauss_jordan_inverse(double (&a)[6][6], int n, double (&inv)[6][6]) {
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) inv[i][j] = i == j ? 1.0 : 0.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < n; ++k) {
+                double t = a[col][k]; a[col][k] = a[piv][k]; a[piv][k] = t;
+                t = inv[col][k]; inv[col][k] = inv[piv][k]; inv[piv][k] = t;
+            }
+        const double d = a[col][col];
+        for (int k = 0; k < n; ++k) {
+            a[col][k] /= d;
+            inv[col][k] /= d;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                a[r][k] -= f * a[col][k];
+                inv[r][k] -= f * inv[col][k];
+            }
+        }
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(64) k_partial_stats(PartialStatsArgs a) {
+    const int j = blockIdx.x, lane = threadIdx.x;
+    if (j >= a.count) return;
+    const int S = a.S, P = a.model == kTranslation ? 2 : 6;
+    const uint32_t index = a.indices[j];
+    const int x0 = (int)(index % a.grid_w) * a.D, y0 = (int)(index / a.grid_w) * a.D;
+    const uint32_t* words = a.support + (size_t)j * ((S * S + 31) / 32);
+    const double half = 0.5 * (S - 1);
+    double w_sum = 0.0, wf_sum = 0.0, npix = 0.0;
+    for (int p = lane; p < S * S; p += 64) {
+        if (!((words[p >> 5] >> (p & 31)) & 1u)) continue;
+        const double w = a.weights ? a.weights[p] : 1.0;
+        const float4 r = a.reference[(size_t)(y0 + p / S) * a.width + x0 + p % S];
+        w_sum += w;
+        wf_sum += w * (double)r.x;
+        npix += 1.0;
+    }
+    w_sum = wave_sum(w_sum);
+    wf_sum = wave_sum(wf_sum);
+    npix = wave_sum(npix);
+    const double mean = wf_sum / w_sum;
+    double energy = 0.0, jtf[6] = {0, 0, 0, 0, 0, 0}, sj[6] = {0, 0, 0, 0, 0, 0}, h[21];
+    for (int k = 0; k < 21; ++k) h[k] = 0.0;
+    for (int p = lane; p < S * S; p += 64) {
+        if (!((words[p >> 5] >> (p & 31)) & 1u)) continue;
+        const double w = a.weights ? a.weights[p] : 1.0;
+        const float4 r = a.reference[(size_t)(y0 + p / S) * a.width + x0 + p % S];
+        const double x = (p % S) - half, y = (p / S) - half;
+        const double fc = (double)r.x - mean, gx = r.y, gy = r.z;
+        const double jac[6] = {gx, gx * x, gx * y, gy, gy * x, gy * y};
+        energy += w * fc * fc;
+        int t = 0;
+        for (int u = 0; u < 6; ++u) {
+            jtf[u] += w * fc * jac[u];
+            sj[u] += w * jac[u];
+            for (int v = u; v < 6; ++v) h[t++] += w * jac[u] * jac[v];
+        }
+    }
+    energy = wave_sum(energy);
+    for (int u = 0; u < 6; ++u) {
+        jtf[u] = wave_sum(jtf[u]);
+        sj[u] = wave_sum(sj[u]);
+    }
+    for (int k = 0; k < 21; ++k) h[k] = wave_sum(h[k]);
+    if (lane != 0) return;
+    const int order_t[2] = {0, 3}, order_a[6] = {0, 1, 2, 3, 4, 5};
+    const int* keep = P == 2 ? order_t : order_a;
+    auto hess = [&](int u, int v) {
+        if (u > v) { const int t = u; u = v; v = t; }
+        return h[u * 6 - u * (u - 1) / 2 + (v - u)];
+    };
+    const double scale[6] = {1.0, 1.0 / S, 1.0 / S, 1.0, 1.0 / S, 1.0 / S};
+    double m[6][6], eig[6];
+    for (int r = 0; r < P; ++r)
+        for (int c = 0; c < P; ++c) m[r][c] = hess(keep[r], keep[c]) * scale[keep[r]] * scale[keep[c]];
+    jacobi_eigenvalues(m, P, eig);
+    const double top = eig[P - 1];
+    int rank = 0;
+    for (int i = 0; i < P; ++i) rank += eig[i] > DBL_EPSILON * P * fmax(top, 0.0) ? 1 : 0;
+    double condition = eig[0] > 0.0 ? top / eig[0] : 0.0;
+    if (!isfinite(condition)) condition = 0.0;
+    const double contrast = sqrt(energy / w_sum);
+    bool ok = rank == P && condition <= kPartialMaxCondition && contrast >= kPartialMinContrast && npix > P;
+    double inv[6][6];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) inv[r][c] = 0.0;
+    if (ok) {
+        if (P == 2) {
+            const double h00 = hess(0, 0), h01 = hess(0, 3), h11 = hess(3, 3);
+            const double det = h00 * h11 - h01 * h01;
+            inv[0][0] = h11 / det;
+            inv[0][3] = -h01 / det;
+            inv[3][0] = -h01 / det;
+            inv[3][3] = h00 / det;
+        } else {
+            double hm[6][6], hi[6][6];
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) hm[r][c] = hess(r, c);
+            ok = gauss_jordan_inverse(hm, 6, hi);
+            for (int r = 0; r < 6 && ok; ++r)
+                for (int c = 0; c < 6; ++c) inv[r][c] = hi[r][c];
+        }
+        for (int r = 0; r < 6 && ok; ++r)
+            for (int c = 0; c < 6; ++c)
+                if (!isfinite(inv[r][c]) || fabs(inv[r][c]) > (double)FLT_MAX) ok = false;
+        if (!ok)
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) inv[r][c] = 0.0;
+    }
+    float* row = a.metadata + (size_t)index * 16;
+    row[0] = (float)sqrt(energy);
+    row[1] = ok ? -4.0f : 0.0f;
+    for (int u = 0; u < 6; ++u) {
+        row[2 + u] = (float)jtf[u];
+        row[8 + u] = (float)sj[u];
+    }
+    row[14] = (float)mean;
+    row[15] = (float)w_sum;
+    float* out = a.inverse + (size_t)index * 36;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) out[r * 6 + c] = (float)inv[r][c];
+    float* diag = a.diagnostics + (size_t)j * 4;
+    diag[0] = ok ? 1.0f : 0.0f;
+    diag[1] = (float)rank;
+    diag[2] = (float)condition;
+    diag[3] = (float)contrast;
+}
+
+hipError_t launch_partial_stats(const PartialStatsArgs& a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    k_partial_stats<<<a.count, 64, 0, s>>>(a);
+    return hipGetLastError();
+}
+
 hipError_t launch_subset_prep(const PrepArgs& a, hipStream_t s) {
     int N = a.grid_w * a.grid_h;
     if (N <= 0) return hipSuccess;

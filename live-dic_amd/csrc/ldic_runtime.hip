@@ -1441,13 +1441,12 @@ int ldic_set_initialization(ldic_engine* e, const ldic_initialization* init) {
     return LDIC_OK;
 }
 
-int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_t* indices,
-                     const uint32_t* support_words, const float* boxes, const float* metadata_rows,
-                     const float* inverse_rows) {
+int ldic_prepare_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_t* indices,
+                         const uint32_t* support_words, const float* boxes, float* out_diagnostics) {
     CHECK(check_unbatched(e));
     CHECK(check_ready(e));
     if (level < 0 || level >= e->levels) return set_error(LDIC_ERR_ARGUMENT, "level out of range");
-    if (count < 0 || (count > 0 && (!indices || !support_words || !boxes || !metadata_rows || !inverse_rows)))
+    if (count < 0 || (count > 0 && (!indices || !support_words || !boxes)))
         return set_error(LDIC_ERR_ARGUMENT, "partial tables are required");
     if (count > 0 && e->opt.motion_model == kQuadratic)
         return set_error(LDIC_ERR_ARGUMENT, "partial supports exist for translation and affine only");
@@ -1455,31 +1454,43 @@ int ldic_set_partial(ldic_engine* e, int32_t level, int32_t count, const uint32_
         if (s->pending) return set_error(LDIC_ERR_STATE, "a slot holds an unretired frame");
     Level& L = e->lv[level];
     const int S = e->opt.subset_size, W = (S * S + 31) / 32;
-    const int M = metadata_stride(e->opt.motion_model), P = param_count(e->opt.motion_model);
     for (int i = 0; i < count; ++i)
         if (indices[i] >= (uint32_t)L.count()) return set_error(LDIC_ERR_ARGUMENT, "partial index %u out of range", indices[i]);
     HIP_CHECK(hipSetDevice(e->device));
     L.partial_count = 0;
     if (count == 0) return LDIC_OK;
-    float *rows = nullptr, *inv = nullptr;
+    float* diag = nullptr;
     CHECK(e->ref_arena.alloc(&L.partial_list, (size_t)count));
     CHECK(e->ref_arena.alloc(&L.support, (size_t)count * W));
     CHECK(e->ref_arena.alloc(&L.support_boxes, (size_t)count));
-    CHECK(e->ref_arena.alloc(&rows, (size_t)count * M));
-    CHECK(e->ref_arena.alloc(&inv, (size_t)count * P * P));
+    CHECK(e->ref_arena.alloc(&diag, (size_t)count * 4));
     HIP_CHECK(hipMemcpy(L.partial_list, indices, sizeof(uint32_t) * count, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(L.support, support_words, sizeof(uint32_t) * (size_t)count * W, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(L.support_boxes, boxes, sizeof(float4) * count, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(rows, metadata_rows, sizeof(float) * (size_t)count * M, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(inv, inverse_rows, sizeof(float) * (size_t)count * P * P, hipMemcpyHostToDevice));
-    HIP_CHECK(launch_scatter_rows(count, M, L.partial_list, rows, L.metadata, e->prep_stream));
-    HIP_CHECK(launch_scatter_rows(count, P * P, L.partial_list, inv, L.inverse, e->prep_stream));
+    PartialStatsArgs pa{};
+    pa.width = L.width;
+    pa.grid_w = L.gw;
+    pa.S = S;
+    pa.D = e->opt.subset_stride;
+    pa.model = e->opt.motion_model;
+    pa.count = count;
+    pa.reference = L.reference;
+    pa.weights = e->opt.spatial_weighting == 1 ? e->weights : nullptr;
+    pa.indices = L.partial_list;
+    pa.support = L.support;
+    pa.metadata = L.metadata;
+    pa.inverse = L.inverse;
+    pa.diagnostics = diag;
+    HIP_CHECK(launch_partial_stats(pa, e->prep_stream));
+    std::vector<float> d((size_t)count * 4);
+    HIP_CHECK(hipMemcpyAsync(d.data(), diag, d.size() * sizeof(float), hipMemcpyDeviceToHost, e->prep_stream));
     HIP_CHECK(hipStreamSynchronize(e->prep_stream));
+    if (out_diagnostics) std::memcpy(out_diagnostics, d.data(), d.size() * sizeof(float));
     L.partial_count = count;
     if (level == 0 && e->valid_host.size() == (size_t)L.count())
         for (int i = 0; i < count; ++i) {
             // A partial subset is reference-valid when its support is admitted.
-            const uint8_t admitted = metadata_rows[(size_t)i * M + 1] == -4.0f ? 1 : 0;
+            const uint8_t admitted = d[(size_t)i * 4] != 0.0f ? 1 : 0;
             e->adaptive_population += (int64_t)admitted - (int64_t)(e->valid_host[indices[i]] != 0);
             e->valid_host[indices[i]] = admitted;
         }

@@ -54,6 +54,11 @@ class Field(ctypes.Structure):
     ]
 
 
+class RoiPlanOptions(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in (
+        "subset_size", "subset_stride", "median_filter_size", "partial", "visibility")]
+
+
 class MedianOp(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in (
         "grid_width", "grid_height", "subset_stride", "motion_model", "outlier_policy", "median_filter_size",
@@ -160,7 +165,15 @@ SIGNATURES = {
     "ldic_register_seeded_resident": (ctypes.c_int, [_P, _I32, _P, _I64, _P, _P]),
     "ldic_export_search": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "ldic_reset_temporal": (ctypes.c_int, [_P]),
-    "ldic_set_partial": (ctypes.c_int, [_P, _I32, _I32, _P, _P, _P, _P, _P]),
+    "ldic_prepare_partial": (ctypes.c_int, [_P, _I32, _I32, _P, _P, _P, _P]),
+    "ldic_roi_planner_create": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(ctypes.c_void_p)]),
+    "ldic_roi_planner_mask": (ctypes.c_int, [_P, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P,
+                                             ctypes.POINTER(_I32)]),
+    "ldic_roi_planner_plan": (ctypes.c_int, [_P, _I32, ctypes.POINTER(RoiPlanOptions), _P, _P, _P, _P,
+                                             ctypes.POINTER(_I32)]),
+    "ldic_roi_planner_partial": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "ldic_roi_planner_seed_links": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ldic_roi_planner_destroy": (None, [_P]),
     "ldic_set_temporal_links": (ctypes.c_int, [_P, _P]),
     "ldic_prepare_reference_resident": (ctypes.c_int, [_P, _I32, _P, _I64]),
     "ldic_batch_create": (ctypes.c_int, [_P, _I32, _P, ctypes.POINTER(_P)]),

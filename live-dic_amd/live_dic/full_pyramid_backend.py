@@ -236,14 +236,11 @@ class ICGNWGPUFullPyramid(_NativeEngine):
             **self._option_config()})
 
     def _set_temporal_links(self) -> None:
-        """Projection links finest -> coarsest under an effective ROI (full_pyramid_backend.py:608-620)."""
-        from .roi import seed_permissions
-
+        """Projection links finest -> coarsest under an effective ROI (full_pyramid_backend.py:608-620),
+        planned on the device with the reference's tables."""
         links = None
-        if self.roi is not None and self.roi.effective:
-            links = np.ascontiguousarray(seed_permissions(
-                self.roi, self._roi_plans[-1], source_scale=1, destination_scale=1 << (self.pyramid_levels - 1),
-                size=self.subset_size, stride=self.subset_stride, reference_mask=self._roi_mask), dtype=np.uint32)
+        if self.roi is not None and self.roi.effective and self._temporal_links is not None:
+            links = np.ascontiguousarray(self._temporal_links, dtype=np.uint32)
         _native.check(self._lib.ldic_set_temporal_links(self._handle, None if links is None else links.ctypes.data))
 
     def reset_temporal_state(self) -> None:

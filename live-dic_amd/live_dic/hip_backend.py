@@ -10,6 +10,7 @@ work, and every numerical step runs in the gfx950 kernels.
 from __future__ import annotations
 
 import ctypes
+from dataclasses import replace
 import operator
 import threading
 import time
@@ -31,8 +32,7 @@ from .outlier import (
     normalize_median_filter_size,
     normalize_outlier_policy,
 )
-from .partial import partial_rows
-from .roi import integral, prepare_roi, reduce_mask, seed_permissions, validate_roi
+from .roi import immutable, plan_pyramid, validate_roi
 from .subset import DEFAULT_SUBSET_SIZE, normalize_subset_size, normalize_subset_stride, subset_grid_shape
 from .weighting import SPATIAL_WEIGHTING_CODES, normalize_gaussian_sigma, normalize_spatial_weighting
 
@@ -131,6 +131,8 @@ class _NativeEngine:
             raise ICGNError("device_type='cpu' has no HIP device; the engine runs on the GPU only")
         self._lock = threading.Lock()
         self._batch_holders = 0
+        self._temporal_links = None
+        self.roi_planning_ms = 0.0
         self._closed = False
         self._reference_ready = False
         self._handle = ctypes.c_void_p()
@@ -166,23 +168,22 @@ class _NativeEngine:
         mask = validate_roi(roi_mask, roi_policy, frame.shape, self.motion_model, roi_truncation)
         shapes = []
         w, h = width, height
-        masks = [mask]
         for k in range(self._levels_count):
             if k:
                 w, h = w // 2, h // 2
-                masks.append(None if masks[-1] is None else reduce_mask(masks[-1]))
             gw, gh = subset_grid_shape(w, h, subset_size=self.subset_size, subset_stride=self.subset_stride)
             if gw < 1 or gh < 1:
                 raise ValueError(f"Reference dimensions must each be at least {self.subset_size} pixels"
                                  + (f" at pyramid level {k}." if k else "."))
             shapes.append((w, h, gw, gh))
-        plans = [prepare_roi(m, self.subset_size, self.subset_stride, policy=roi_policy, truncation=roi_truncation,
-                             median_filter_size=self.median_filter_size if self.outlier_policy == "median" else 3)
-                 for m in masks]
+        pyramid = plan_pyramid(mask, self._levels_count, size=self.subset_size, stride=self.subset_stride,
+                               policy=roi_policy, truncation=roi_truncation,
+                               median_filter_size=self.median_filter_size if self.outlier_policy == "median" else 3,
+                               device=0, temporal=getattr(self, "initialization", "") == "temporal")
+        plans = pyramid.plans
         started = time.perf_counter()
         keep = []
         rois = (_native.LevelRoi * self._levels_count)()
-        prefix = integral(mask) if mask is not None and plans[0] is not None and plans[0].effective else None
         for k, plan in enumerate(plans):
             if plan is not None and plan.effective:
                 sel = np.ascontiguousarray(plan.selected, dtype=np.uint8)
@@ -190,11 +191,8 @@ class _NativeEngine:
                 keep += [sel, words]
                 rois[k].selected = sel.ctypes.data
                 rois[k].neighbour_words = words.ctypes.data
-            if k + 1 < self._levels_count and prefix is not None:
-                links = np.ascontiguousarray(seed_permissions(
-                    plans[k + 1], plan, source_scale=1 << (k + 1), destination_scale=1 << k,
-                    size=self.subset_size, stride=self.subset_stride, reference_mask=mask,
-                    reference_prefix=prefix), dtype=np.uint32)
+            if pyramid.seed_links[k] is not None:
+                links = np.ascontiguousarray(pyramid.seed_links[k], dtype=np.uint32)
                 keep.append(links)
                 rois[k].seed_links = links.ctypes.data
         cfgs = (_native.LevelConfig * self._levels_count)(
@@ -207,18 +205,20 @@ class _NativeEngine:
         for k, plan in enumerate(plans):
             if plan is None or plan.partial_indices is None or plan.partial_indices.size == 0:
                 continue
-            plans[k], admitted = self._upload_partial(k, plan)
+            plans[k], admitted = self._prepare_partial(k, plan)
             if k == 0:
                 # reference_valid of a partial subset is its admission (wgpu_backend.py:2428-2433).
                 part = np.asarray(plan.partial_indices, np.int64)
                 valid.ravel()[part] = 0
                 valid.ravel()[admitted] = 1
-        build_ms = (time.perf_counter() - started) * 1e3
+        build_ms = (time.perf_counter() - started) * 1e3 + pyramid.planning_ms
         self.width, self.height = width, height
         self.grid_width, self.grid_height = shapes[0][2], shapes[0][3]
         self.roi = plans[0]
         self._roi_plans = plans
         self._roi_mask = mask
+        self._temporal_links = pyramid.temporal_links
+        self.roi_planning_ms = pyramid.planning_ms
         self.reference_valid = valid.astype(bool)
         self.reference_valid.setflags(write=False)
         self._roi_level_counts = [int(p.selected.sum()) if p is not None else s[2] * s[3]
@@ -227,28 +227,27 @@ class _NativeEngine:
         self._reference_ready = True
         return build_ms
 
-    def _upload_partial(self, level: int, plan):
-        """Float64 statistics of the level's partial-support subsets from the prepared reference
-        plane, uploaded with their support tables (wgpu_backend.py:2412-2450).  Returns the plan
-        with its support diagnostics and the flat indices of the admitted subsets."""
-        w, h, gw, gh = self._level_shapes[level]
-        reference = np.empty((h, w, 4), np.float32)
-        meta = np.empty((gw * gh, 16), np.float32)
-        inv = np.empty((gw * gh, 36), np.float32)
-        _native.check(self._lib.ldic_export_reference(self._handle, level, reference.ctypes.data, meta.ctypes.data,
-                                                      inv.ctypes.data))
-        rows, inverses, plan = partial_rows(reference, plan, width=w, size=self.subset_size,
-                                            stride=self.subset_stride, motion_model=self.motion_model,
-                                            weighting=self.spatial_weighting, sigma=self.gaussian_sigma)
+    def _prepare_partial(self, level: int, plan):
+        """Float64 statistics of the level's partial-support subsets, computed on the device from
+        the prepared reference with their support tables (wgpu_backend.py:2412-2450).  Returns the
+        plan with its support diagnostics and the flat indices of the admitted subsets."""
         indices = np.ascontiguousarray(plan.partial_indices, dtype=np.uint32)
         words = np.ascontiguousarray(plan.support_words, dtype=np.uint32)
         boxes = np.ascontiguousarray(plan.support_boxes, dtype=np.float32)
-        inverse = np.ascontiguousarray(inverses, dtype=np.float32)
-        rows = np.ascontiguousarray(rows, dtype=np.float32)
-        _native.check(self._lib.ldic_set_partial(self._handle, level, int(indices.size), indices.ctypes.data,
-                                                 words.ctypes.data, boxes.ctypes.data, rows.ctypes.data,
-                                                 inverse.ctypes.data))
-        return plan, indices[rows[:, 1] == -4.0].astype(np.int64)
+        diag = np.empty((indices.size, 4), np.float32)
+        _native.check(self._lib.ldic_prepare_partial(self._handle, level, int(indices.size), indices.ctypes.data,
+                                                     words.ctypes.data, boxes.ctypes.data, diag.ctypes.data))
+        grid = plan.selected.shape
+        rank = np.zeros(grid, np.uint32)
+        condition = np.zeros(grid, np.float64)
+        contrast = np.zeros(grid, np.float64)
+        flat = indices.astype(np.int64)
+        rank.ravel()[flat] = diag[:, 1].astype(np.uint32)
+        condition.ravel()[flat] = diag[:, 2]
+        contrast.ravel()[flat] = diag[:, 3]
+        plan = replace(plan, support_condition=immutable(condition), support_rank=immutable(rank),
+                       support_contrast=immutable(contrast))
+        return plan, flat[diag[:, 0] != 0]
 
     def _reference_info(self, started: float, build_ms: float, config: dict) -> ReferenceInfo:
         total = (time.perf_counter() - started) * 1e3

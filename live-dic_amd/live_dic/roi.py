@@ -1,46 +1,50 @@
-"""Static reference-coordinate ROI planning (reference: live_dic/roi.py).
+"""Static reference-coordinate ROI planning, once per reference.
 
-Runs once per reference on the host; nothing here runs per target.  A subset
-is selected when its whole S x S window lies inside the mask (``full`` policy).
-Under the ``partial`` policy (roi.py:266-355) a boundary subset is also
-selected when the 4-connected component of allowed pixels anchored on its
-centre is not empty; that component (optionally cut to the pixels visible from
-the centre, ``roi_truncation="visibility"``) is its support.  Neighbour and
-seed links require the closed pixel box around the segment joining two subset
-centres to be free of masked pixels.
+The tables the device reads (window selection, 3x3 neighbour links, the 81-bit
+link words of wider medians, partial supports, transfer links between pyramid
+levels and the reduced level masks) are computed on the device by the
+library's planner (``ldic_roi_planner_*``, csrc/ldic_roi.hip).  This module
+validates the mask with the reference's rules and messages (roi.py:29-60),
+drives the planner and keeps its output as immutable :class:`RoiPlan` tables,
+whose fields, summary keys and archive keys follow the reference's RoiPlan.
 
-The reference rejects ROIs for the quadratic model (roi.py:45-46); this build
-supports them under the full policy with the same full-window rule, since the
-solver only needs the selection flags and link words.  Partial supports exist
-for translation and affine, as in the reference.
+A subset is selected when its whole S x S window is allowed (``full``).  Under
+``partial`` (translation and affine, as in the reference) an incomplete window
+is also selected when the 4-connected component of allowed pixels grown from
+its centre is not empty; that component, optionally cut to the pixels visible
+from the centre (``roi_truncation="visibility"``), is its support.  The
+reference rejects ROIs for the quadratic model (roi.py:45-46); this build
+supports them under the full policy.
 """
 
 from __future__ import annotations
 
+import ctypes
 import hashlib
 import struct
-from dataclasses import dataclass, replace
-from functools import lru_cache
+from dataclasses import dataclass
 
 import numpy as np
 
 from .outlier import normalize_median_filter_size
-from .subset import subset_grid_shape
 
-
-# Partial-support admission limits (roi.py:20-21): normalised Hessian condition
-# and weighted RMS contrast of the support.
+# Partial-support admission limits (roi.py:20-21): normalised Hessian condition and
+# weighted RMS contrast of the support.
 PARTIAL_MAX_NORMALIZED_CONDITION = 1e8
 PARTIAL_MIN_WEIGHTED_CONTRAST = 1.0
+# The device planner grows partial supports one 64-bit row per lane.
+MAX_PARTIAL_SUBSET = 64
 
 
 def immutable(array: np.ndarray) -> np.ndarray:
     """Read-only copy owning its bytes."""
-    return np.frombuffer(array.tobytes(order="C"), dtype=array.dtype).reshape(array.shape)
+    out = np.array(array, copy=True, order="C")
+    out.setflags(write=False)
+    return out
 
 
 def validate_roi(mask, policy: str, shape: tuple[int, int], motion_model: str, truncation: str = "none"):
-    """Checks of roi.py:29-60 (policy, truncation, dtype, shape) before any device work."""
+    """The reference's checks (policy, truncation, dtype, shape) before any device work."""
     if not isinstance(policy, str) or policy not in ("full", "partial"):
         raise ValueError("roi_policy must be 'full' or 'partial'")
     if not isinstance(truncation, str) or truncation not in ("none", "visibility"):
@@ -55,49 +59,13 @@ def validate_roi(mask, policy: str, shape: tuple[int, int], motion_model: str, t
         raise TypeError("roi_mask must be a two-dimensional boolean NumPy array")
     if mask.ndim != 2 or mask.shape != tuple(shape):
         raise ValueError(f"roi_mask must have reference shape {tuple(shape)}")
-    return immutable(np.logical_and(mask, True))
+    return immutable(mask)
 
 
-def integral(mask: np.ndarray) -> np.ndarray:
-    """Exclusive 2-D prefix count of forbidden pixels, shape (h+1, w+1), int64."""
-    forbidden = (~mask).astype(np.int64)
-    out = np.zeros((mask.shape[0] + 1, mask.shape[1] + 1), dtype=np.int64)
-    out[1:, 1:] = forbidden.cumsum(0).cumsum(1)
-    return out
-
-
-def boxes_clear(prefix, x0, y0, x1, y1) -> np.ndarray:
-    """True where the half-open box [x0, x1) x [y0, y1) holds no forbidden pixel."""
-    return (prefix[y1, x1] - prefix[y0, x1] - prefix[y1, x0] + prefix[y0, x0]) == 0
-
-
-def select_windows(mask: np.ndarray, size: int, stride: int, *, summed_area=None) -> np.ndarray:
-    h, w = mask.shape
-    gw, gh = subset_grid_shape(w, h, subset_size=size, subset_stride=stride)
-    ys, xs = np.mgrid[:gh, :gw] * stride
-    prefix = integral(mask) if summed_area is None else summed_area
-    return boxes_clear(prefix, xs, ys, xs + size, ys + size)
-
-
-def reduce_mask(mask: np.ndarray) -> np.ndarray:
-    """Coarse pixel allowed iff all 25 reflected source pixels at (2x, 2y) are (roi.py:89-97)."""
-    h, w = mask.shape
-    padded = np.pad(mask, 2, mode="reflect")
-    out = np.ones((h // 2, w // 2), dtype=bool)
-    for dy in range(5):
-        for dx in range(5):
-            out &= padded[dy : dy + 2 * (h // 2) : 2, dx : dx + 2 * (w // 2) : 2]
-    return immutable(out)
-
-
-def _segment_clear(prefix, ax, ay, bx, by) -> np.ndarray:
-    """Closed pixel box enclosing the segment (ax, ay)-(bx, by) is mask-free."""
-    h, w = prefix.shape[0] - 1, prefix.shape[1] - 1
-    x0 = np.maximum(0, np.ceil(np.minimum(ax, bx) - 0.5).astype(np.int64))
-    y0 = np.maximum(0, np.ceil(np.minimum(ay, by) - 0.5).astype(np.int64))
-    x1 = np.minimum(w, np.floor(np.maximum(ax, bx) + 0.5).astype(np.int64) + 1)
-    y1 = np.minimum(h, np.floor(np.maximum(ay, by) + 0.5).astype(np.int64) + 1)
-    return boxes_clear(prefix, x0, y0, x1, y1)
+def mask_digest(mask: np.ndarray) -> str:
+    """SHA-256 of "live-dic-roi-v1\\0", the uint64le height and width, and the C-order bytes."""
+    head = b"live-dic-roi-v1\0" + struct.pack("<QQ", *mask.shape)
+    return hashlib.sha256(head + np.ascontiguousarray(mask, dtype=np.bool_).tobytes()).hexdigest()
 
 
 @dataclass(frozen=True, slots=True)
@@ -114,7 +82,7 @@ class RoiPlan:
     truncation: str = "none"
     partial_indices: np.ndarray | None = None
     median_neighbour_words: np.ndarray | None = None
-    support_words: np.ndarray | None = None     # (partial, ceil(S*S/32)) uint32, row-major, bit 0 first
+    support_words: np.ndarray | None = None     # (partial, ceil(S*S/32)) uint32, row-major, pixel 0 in bit 0
     support_boxes: np.ndarray | None = None     # (partial, 4) float32 xmin, ymin, xmax, ymax (local pixels)
     coverage: np.ndarray | None = None          # (gh, gw) support fraction of each selected subset
     support_condition: np.ndarray | None = None
@@ -126,21 +94,20 @@ class RoiPlan:
         return self.selected_count == 0
 
     def summary(self) -> dict:
-        return {
+        out = {
             "policy": self.policy,
             "truncation": self.truncation,
             "partial": 0 if self.partial_indices is None else int(self.partial_indices.size),
-            **({"supportConditionLimit": PARTIAL_MAX_NORMALIZED_CONDITION,
-                "supportContrastRmsMinimum": PARTIAL_MIN_WEIGHTED_CONTRAST,
-                "supportConnectivity": 4,
-                "evenCenter": "all four central pixels required"} if self.policy == "partial" else {}),
-            "maskShape": list(self.mask.shape),
-            "maskSha256": self.mask_hash,
-            "gridTotal": int(self.selected.size),
-            "selected": self.selected_count,
-            "excluded": int(self.selected.size) - self.selected_count,
-            "hashFormat": "live-dic-roi-v1\\0 + uint64le(height,width) + C-order uint8 bool",
         }
+        if self.policy == "partial":
+            out.update(supportConditionLimit=PARTIAL_MAX_NORMALIZED_CONDITION,
+                       supportContrastRmsMinimum=PARTIAL_MIN_WEIGHTED_CONTRAST, supportConnectivity=4,
+                       evenCenter="all four central pixels required")
+        total = int(self.selected.size)
+        out.update(maskShape=list(self.mask.shape), maskSha256=self.mask_hash, gridTotal=total,
+                   selected=self.selected_count, excluded=total - self.selected_count,
+                   hashFormat="live-dic-roi-v1\\0 + uint64le(height,width) + C-order uint8 bool")
+        return out
 
     def archive_fields(self) -> dict:
         fields = {"roi_mask": self.mask, "roi_selected": self.selected}
@@ -153,7 +120,7 @@ class RoiPlan:
         return fields
 
     def neighbour_words(self) -> np.ndarray:
-        """(grid, 4) float32 link words as appended to the solver metadata."""
+        """(grid, 4) float32 link words as the device reads them after the metadata."""
         if self.median_neighbour_words is not None:
             return np.ascontiguousarray(self.median_neighbour_words, dtype=np.float32)
         words = np.zeros((self.selected.size, 4), dtype=np.float32)
@@ -161,179 +128,90 @@ class RoiPlan:
         return words
 
 
-def pack_support(support: np.ndarray) -> np.ndarray:
-    """Row-major pixels as little-endian uint32 words, pixel 0 in bit 0 (roi.py:266-270)."""
-    bits = np.packbits(np.asarray(support, bool).ravel(), bitorder="little")
-    bits = np.pad(bits, (0, (-bits.size) % 4))
-    return bits.view("<u4").astype(np.uint32)
+@dataclass(frozen=True)
+class PyramidPlan:
+    """Plans of every pyramid level (finest first; None without a mask), the transfer links into
+    each finer level (``seed_links[k]``: from level k + 1 into k) and the projection links from the
+    finest level onto the coarsest (temporal initialisation), when the finest plan is effective."""
+
+    plans: list
+    seed_links: list
+    temporal_links: np.ndarray | None
+    planning_ms: float
 
 
-def unpack_support(words: np.ndarray, size: int) -> np.ndarray:
-    raw = np.asarray(words, dtype="<u4").view(np.uint8)
-    return np.unpackbits(raw, bitorder="little")[: size * size].reshape(size, size).astype(bool)
+def plan_pyramid(mask, levels: int, *, size: int, stride: int, policy: str = "full", truncation: str = "none",
+                 median_filter_size: int = 3, device: int = 0, temporal: bool = False) -> PyramidPlan:
+    """Plan every level of a validated mask on the device (one upload of the mask)."""
+    import time
 
+    from . import _native
 
-def central_component(support: np.ndarray) -> np.ndarray:
-    """4-connected component of allowed pixels grown from the central pixel; for an even width
-    all four central pixels must be allowed, otherwise the component is empty (roi.py:280-299)."""
-    size = support.shape[0]
-    lo, hi = (size - 1) // 2, size // 2
-    out = np.zeros_like(support, dtype=bool)
-    if not support[lo:hi + 1, lo:hi + 1].all():
-        return out
-    out[lo, lo] = True
-    stack = [(lo, lo)]
-    while stack:
-        y, x = stack.pop()
-        for ny, nx in ((y - 1, x), (y + 1, x), (y, x - 1), (y, x + 1)):
-            if 0 <= ny < size and 0 <= nx < size and support[ny, nx] and not out[ny, nx]:
-                out[ny, nx] = True
-                stack.append((ny, nx))
-    return out
-
-
-@lru_cache(maxsize=7)
-def visibility_paths(size: int) -> np.ndarray:
-    """(S*S, words) packed sets: the closed pixel cells that the segment from the window centre
-    to each pixel centre touches (slab test per axis; touching a cell edge or corner counts),
-    the visibility rule of roi.py:302-330."""
-    n = size * size
-    centre = (size - 1) / 2
-    yy, xx = np.divmod(np.arange(n, dtype=np.float64), size)
-    t_lo = np.zeros((n, n))
-    t_hi = np.ones((n, n))
-    for coord in (xx, yy):                       # rows: target pixel, columns: cell
-        d = coord - centre
-        for i in range(n):
-            if d[i] == 0.0:
-                outside = np.abs(coord - centre) > 0.5
-                t_hi[i, outside] = -1.0
-                continue
-            a = (coord - 0.5 - centre) / d[i]
-            b = (coord + 0.5 - centre) / d[i]
-            t_lo[i] = np.maximum(t_lo[i], np.minimum(a, b))
-            t_hi[i] = np.minimum(t_hi[i], np.maximum(a, b))
-    paths = t_lo <= t_hi
-    out = np.stack([pack_support(row) for row in paths])
-    out.setflags(write=False)
-    return out
-
-
-def partial_windows(mask: np.ndarray, complete: np.ndarray, size: int, stride: int, *, truncation: str):
-    """Selection, indices, support words, support boxes and coverage of the partial policy
-    (roi.py:333-355): every incomplete window whose central component is not empty."""
-    if truncation not in ("none", "visibility"):
-        raise ValueError("roi_truncation must be 'none' or 'visibility'")
-    selected = complete.copy()
-    coverage = complete.astype(np.float32)
-    indices, words, boxes = [], [], []
-    paths = visibility_paths(size) if truncation == "visibility" else None
-    for y, x in np.argwhere(~complete):
-        local = mask[y * stride:y * stride + size, x * stride:x * stride + size]
-        support = central_component(local)
-        if support.any() and paths is not None:
-            packed = pack_support(support)
-            visible = ~np.any(paths & ~packed, axis=1)
-            support = central_component(support & visible.reshape(size, size))
-        if not support.any():
-            continue
-        selected[y, x] = True
-        indices.append(y * complete.shape[1] + x)
-        words.append(pack_support(support))
-        sy, sx = np.nonzero(support)
-        boxes.append((sx.min(), sy.min(), sx.max(), sy.max()))
-        coverage[y, x] = support.sum() / (size * size)
-    nwords = (size * size + 31) // 32
-    return (selected, immutable(np.asarray(indices, np.uint32)),
-            immutable(np.asarray(words, np.uint32).reshape(-1, nwords)),
-            immutable(np.asarray(boxes, np.float32).reshape(-1, 4)), immutable(coverage))
-
-
-def prepare_roi(mask, size: int, stride: int, *, policy: str = "full", truncation: str = "none",
-                median_filter_size: int = 3) -> RoiPlan | None:
-    """Selection, 3x3 neighbour bits and optional 9x9 median words (roi.py:178-263)."""
     median_filter_size = normalize_median_filter_size(median_filter_size)
     if mask is None:
-        return None
-    if policy not in ("full", "partial") or (policy == "full" and truncation != "none"):
-        raise ValueError("Unknown ROI policy/truncation")
+        return PyramidPlan([None] * levels, [None] * levels, None, 0.0)
+    if policy == "partial" and size > MAX_PARTIAL_SUBSET:
+        raise ValueError(f"roi_policy='partial' supports subset sizes up to {MAX_PARTIAL_SUBSET} pixels")
+    lib = _native.library()
+    started = time.perf_counter()
     h, w = mask.shape
-    every = bool(mask.all())
-    prefix = None if every else integral(mask)
-    gw, gh = subset_grid_shape(w, h, subset_size=size, subset_stride=stride)
-    selected = np.ones((gh, gw), dtype=bool) if every else select_windows(mask, size, stride, summed_area=prefix)
+    raw = np.ascontiguousarray(mask, dtype=np.uint8)
+    handle = ctypes.c_void_p()
+    _native.check(lib.ldic_roi_planner_create(raw.ctypes.data, w, h, levels, device, ctypes.byref(handle)))
+    try:
+        opts = _native.RoiPlanOptions(size, stride, median_filter_size, 1 if policy == "partial" else 0,
+                                      1 if truncation == "visibility" else 0)
+        plans = [_plan_level(lib, handle, k, opts, policy, truncation) for k in range(levels)]
+        links = [None] * levels
+        temporal_links = None
+        if plans[0].effective:
+            for k in range(levels - 1):
+                links[k] = _links(lib, handle, k + 1, k, plans[k].selected.shape)
+            if temporal:
+                temporal_links = _links(lib, handle, 0, levels - 1, plans[-1].selected.shape)
+    finally:
+        lib.ldic_roi_planner_destroy(handle)
+    return PyramidPlan(plans, links, temporal_links, (time.perf_counter() - started) * 1e3)
+
+
+def _links(lib, handle, source: int, destination: int, shape) -> np.ndarray:
+    from . import _native
+
+    out = np.empty(shape, np.uint32)
+    _native.check(lib.ldic_roi_planner_seed_links(handle, source, destination, out.ctypes.data))
+    return immutable(out)
+
+
+def _plan_level(lib, handle, level: int, opts, policy: str, truncation: str) -> RoiPlan:
+    from . import _native
+
+    w, h, every = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _native.check(lib.ldic_roi_planner_mask(handle, level, ctypes.byref(w), ctypes.byref(h), None, ctypes.byref(every)))
+    mask = np.empty((h.value, w.value), np.uint8)
+    _native.check(lib.ldic_roi_planner_mask(handle, level, None, None, mask.ctypes.data, None))
+    mask = mask.astype(bool)
+    S, D = opts.subset_size, opts.subset_stride
+    gw, gh = 1 + (w.value - S) // D, 1 + (h.value - S) // D
+    selected = np.empty((gh, gw), np.uint8)
+    bits = np.empty((gh, gw), np.uint32)
+    wide = opts.median_filter_size > 3
+    words = np.empty((gh * gw, 4), np.float32) if wide else None
+    coverage = np.empty((gh, gw), np.float32)
+    count = ctypes.c_int32()
+    _native.check(lib.ldic_roi_planner_plan(handle, level, ctypes.byref(opts), selected.ctypes.data, bits.ctypes.data,
+                                            None if words is None else words.ctypes.data, coverage.ctypes.data,
+                                            ctypes.byref(count)))
     partial = (None,) * 4
     if policy == "partial":
-        selected, *partial = partial_windows(mask, selected, size, stride, truncation=truncation)
-    gy, gx = np.indices(selected.shape)
-    cx = gx * stride + (size - 1) / 2
-    cy = gy * stride + (size - 1) / 2
-
-    def linked(dx: int, dy: int) -> np.ndarray:
-        nx, ny = gx + dx, gy + dy
-        inside = (nx >= 0) & (ny >= 0) & (nx < gw) & (ny < gh)
-        ok = inside & selected & selected[ny.clip(0, gh - 1), nx.clip(0, gw - 1)]
-        if prefix is not None:
-            ok &= _segment_clear(prefix, cx, cy, cx + dx * stride, cy + dy * stride)
-        return ok
-
-    bits = np.zeros(selected.shape, dtype=np.uint32)
-    for dy in (-1, 0, 1):
-        for dx in (-1, 0, 1):
-            bits |= linked(dx, dy).astype(np.uint32) << np.uint32((dy + 1) * 3 + dx + 1)
-    words = None
-    if median_filter_size > 3:
-        r = median_filter_size // 2
-        words = np.zeros((gh, gw, 4), dtype=np.float32)
-        for dy in range(-r, r + 1):
-            for dx in range(-r, r + 1):
-                bit = (dy + 4) * 9 + dx + 4
-                words[..., bit // 24] += linked(dx, dy).astype(np.float32) * float(1 << (bit % 24))
-        words = immutable(words.reshape(-1, 4))
-    digest = hashlib.sha256(b"live-dic-roi-v1\0" + struct.pack("<QQ", *mask.shape) + mask.tobytes(order="C")).hexdigest()
-    indices, support_words, boxes, coverage = partial
-    return RoiPlan(mask, immutable(selected), immutable(bits), digest, not every, int(selected.sum()),
-                   policy, truncation, indices, words, support_words, boxes, coverage)
-
-
-def seed_permissions(source: RoiPlan, destination: RoiPlan, *, source_scale: int, destination_scale: int,
-                     size: int, stride: int, reference_mask: np.ndarray, reference_prefix=None) -> np.ndarray:
-    """Transfer link words (roi.py:358-437): bits 0-3 corners, bit 4 selection,
-    bits 5.. one-based index of a fallback coarse subset."""
-    sel_s, sel_d = source.selected, destination.selected
-    sh, sw = sel_s.shape
-    y, x = np.indices(sel_d.shape)
-    c = size / 2
-    sx = (destination_scale * (c + x * stride) + (source_scale - destination_scale) / 2) / source_scale
-    sy = (destination_scale * (c + y * stride) + (source_scale - destination_scale) / 2) / source_scale
-    lx = np.clip(np.floor((sx - c) / stride), 0, max(0, sw - 2)).astype(int)
-    ly = np.clip(np.floor((sy - c) / stride), 0, max(0, sh - 2)).astype(int)
-    prefix = reference_prefix if reference_prefix is not None else integral(reference_mask)
-    half = (size - 1) / 2
-    bits = sel_d.astype(np.uint32) << np.uint32(4)
-    for corner in range(4):
-        nx, ny = lx + (corner & 1), ly + (corner >> 1)
-        inside = (nx < sw) & (ny < sh)
-        nx, ny = nx.clip(0, sw - 1), ny.clip(0, sh - 1)
-        ok = inside & sel_s[ny, nx] & sel_d
-        ok &= _segment_clear(prefix, destination_scale * (half + x * stride), destination_scale * (half + y * stride),
-                             source_scale * (half + nx * stride), source_scale * (half + ny * stride))
-        bits |= ok.astype(np.uint32) << np.uint32(corner)
-    if source_scale > destination_scale and sel_s.size < (1 << 27) - 1:
-        my, mx = np.nonzero(sel_d & ((bits & 15) == 0))
-        best = np.full(mx.shape, np.inf)
-        chosen = np.zeros(mx.shape, dtype=np.uint32)
-        for dy in range(-2, 4):
-            for dx in range(-2, 4):
-                nx, ny = lx[my, mx] + dx, ly[my, mx] + dy
-                inside = (nx >= 0) & (ny >= 0) & (nx < sw) & (ny < sh)
-                cxn, cyn = nx.clip(0, sw - 1), ny.clip(0, sh - 1)
-                dist = (sx[my, mx] - (c + cxn * stride)) ** 2 + (sy[my, mx] - (c + cyn * stride)) ** 2
-                ok = inside & sel_s[cyn, cxn] & (dist < best)
-                ok &= _segment_clear(prefix, destination_scale * (half + mx * stride),
-                                     destination_scale * (half + my * stride), source_scale * (half + cxn * stride),
-                                     source_scale * (half + cyn * stride))
-                best[ok] = dist[ok]
-                chosen[ok] = (cyn[ok] * sw + cxn[ok] + 1).astype(np.uint32)
-        bits[my, mx] |= chosen << np.uint32(5)
-    return immutable(bits)
+        n, nwords = count.value, (S * S + 31) // 32
+        indices = np.empty(n, np.uint32)
+        support = np.empty((n, nwords), np.uint32)
+        boxes = np.empty((n, 4), np.float32)
+        if n:
+            _native.check(lib.ldic_roi_planner_partial(handle, level, indices.ctypes.data, support.ctypes.data,
+                                                       boxes.ctypes.data))
+        partial = (immutable(indices), immutable(support), immutable(boxes), immutable(coverage))
+    chosen = selected.astype(bool)
+    return RoiPlan(immutable(mask), immutable(chosen), immutable(bits), mask_digest(mask), not every.value,
+                   int(chosen.sum()), policy, truncation, partial[0], None if words is None else immutable(words),
+                   partial[1], partial[2], partial[3])

@@ -39,9 +39,9 @@ def compare_fields(candidate: np.ndarray, expected: np.ndarray, params: int) -> 
 
 def oracle_rois(size: int, stride: int, mask: np.ndarray, levels: int, *, policy: str = "full",
                 truncation: str = "none", median_filter_size: int = 3) -> list[dict]:
-    """Per-level ROI tables for ``oracle.Pipeline`` from the library's host planner
+    """Per-level ROI tables for ``oracle.Pipeline`` from the oracle's planner
     (selection, link bits, seed links and partial supports)."""
-    from live_dic.roi import prepare_roi, reduce_mask, seed_permissions
+    from oracle.roi_plan import prepare_roi, reduce_mask, seed_permissions
 
     masks = [mask]
     for _ in range(1, levels):

@@ -214,7 +214,7 @@ def test_latency_translation_goldens(case, frames, roi):
     from iteration 4): against the capture and the oracle's pipeline."""
     from helpers import GOLDEN, load_gray, oracle_rois
     from live_dic import ICGNWGPUFullPyramid
-    from live_dic.roi import seed_permissions
+    from oracle.roi_plan import seed_permissions
 
     _say("latency golden", case)
     ref = load_gray("uhd_reference.png")

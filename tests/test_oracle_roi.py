@@ -1,11 +1,13 @@
-"""ROI planning (host): selection, reduced masks, neighbour and seed links (roi.py semantics)."""
+"""The oracle's ROI planner against brute force: selection, reduced masks, neighbour and seed
+links (roi.py semantics).  The library's device planner is compared with it on the GPU."""
 
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
-from live_dic.roi import integral, prepare_roi, reduce_mask, seed_permissions, select_windows, validate_roi
+from live_dic.roi import validate_roi
+from oracle.roi_plan import integral, prepare_roi, reduce_mask, seed_permissions, select_windows
 
 
 def test_full_window_selection_matches_brute_force():

@@ -175,7 +175,7 @@ def test_latency_translation_goldens(case, frames, roi):
     """WGSL latency-profile goldens (qualify_binding_budget.py:54-60: velocity prediction, fine
     correction, divergence patience 3 from iteration 4, median), frames np.roll(reference, (-i, 2i))."""
     from helpers import GOLDEN, compare_fields, load_gray, oracle_rois
-    from live_dic.roi import seed_permissions
+    from oracle.roi_plan import seed_permissions
 
     ref = load_gray("uhd_reference.png")
     goldens = np.load(GOLDEN / "uhd_goldens.npz", allow_pickle=False)
