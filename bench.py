@@ -452,18 +452,18 @@ def main() -> int:
                         help="only copy 256 MiB with one plain device copy kernel (PMC byte-counter calibration)")
     args = parser.parse_args()
     if args.calibrate_copy:
-        import torch
+        from live_dic import _native
 
-        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copied_bytes": src.numel(), "read_bytes": src.numel(), "written_bytes": dst.numel()}))
+        lib = _native.library()
+        src = _native.DeviceArray.zeros(256 << 20, np.uint8)
+        dst = _native.DeviceArray(256 << 20, np.uint8)
+        _native.check(lib.ldic_device_copy_async(dst.ptr, src.ptr, src.nbytes, None))
+        _native.check(lib.ldic_device_synchronize())
+        print(json.dumps({"copied_bytes": src.nbytes, "read_bytes": src.nbytes, "written_bytes": dst.nbytes}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for the "
-              "sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (frames would shard by "
+              "bands of the subset grid); run with --gpus 1", file=sys.stderr)
         return 2
     if args.only:
         legs = {"learned": learned_vs_zncc, "batch": batched_heterogeneous, "pairs": consecutive_quadratic}

@@ -513,6 +513,16 @@ int ldic_copy_to_device(void* device_dst, const void* host_src, size_t bytes);
 int ldic_copy_to_host(void* host_dst, const void* device_src, size_t bytes);
 int ldic_device_synchronize(void);
 
+/* Device-memory helpers of the learned seeder's host layer (no counterpart in the
+ * reference; they replace the framework tensors it would otherwise need):
+ * ldic_device_zero clears `bytes` bytes on `stream` with a fill kernel (so a
+ * captured stream records a kernel node, not a memset node);
+ * ldic_device_copy_async copies device to device on `stream`;
+ * ldic_stream_synchronize waits for `stream` (NULL: the null stream). */
+int ldic_device_zero(void* device_ptr, size_t bytes, void* stream);
+int ldic_device_copy_async(void* device_dst, const void* device_src, size_t bytes, void* stream);
+int ldic_stream_synchronize(void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,15 @@ __global__ __launch_bounds__(256) void k_render(RenderArgs a) {
     a.out[(size_t)b * a.width * a.height + idx] = (uint8_t)fmin(fmax(value, 0.0), 255.0);
 }
 
+// Zero fill of `bytes` bytes: whole 32-bit words, then the trailing bytes.  A
+// kernel rather than hipMemsetAsync, so the fill is an ordinary kernel node when
+// a caller captures the stream into a graph.
+__global__ __launch_bounds__(256) void k_zero(uint32_t* words, size_t count, uint8_t* tail, int tail_bytes) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) words[i] = 0u;
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail_bytes) tail[threadIdx.x] = 0;
+}
+
 }  // namespace ldic
 
 extern "C" int ldic_render_speckle(const ldic_speckle* sp, const ldic_field* field, int32_t width, int32_t height,
@@ -185,4 +194,34 @@ extern "C" int ldic_copy_to_host(void* dst, const void* src, size_t bytes) {
 
 extern "C" int ldic_device_synchronize(void) {
     return hipDeviceSynchronize() == hipSuccess ? LDIC_OK : LDIC_ERR_DEVICE;
+}
+
+extern "C" int ldic_device_zero(void* p, size_t bytes, void* stream) {
+    if (!p && bytes) return LDIC_ERR_ARGUMENT;
+    if (!bytes) return LDIC_OK;
+    uint8_t* b = static_cast<uint8_t*>(p);
+    // Leading bytes up to the first 32-bit boundary go through the byte path.
+    size_t head = (4u - ((uintptr_t)b & 3u)) & 3u;
+    if (head > bytes) head = bytes;
+    if (head) ldic::k_zero<<<1, 256, 0, (hipStream_t)stream>>>(nullptr, 0, b, (int)head);
+    const size_t rest = bytes - head, words = rest / 4;
+    if (rest) {
+        size_t blocks = (words + 255) / 256;
+        blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+        ldic::k_zero<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(reinterpret_cast<uint32_t*>(b + head), words,
+                                                                        b + head + words * 4, (int)(rest % 4));
+    }
+    return hipGetLastError() == hipSuccess ? LDIC_OK : LDIC_ERR_DEVICE;
+}
+
+extern "C" int ldic_device_copy_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if ((!dst || !src) && bytes) return LDIC_ERR_ARGUMENT;
+    if (!bytes) return LDIC_OK;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream) == hipSuccess
+               ? LDIC_OK
+               : LDIC_ERR_DEVICE;
+}
+
+extern "C" int ldic_stream_synchronize(void* stream) {
+    return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? LDIC_OK : LDIC_ERR_DEVICE;
 }

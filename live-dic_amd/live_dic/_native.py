@@ -10,6 +10,8 @@ import ctypes
 import os
 from pathlib import Path
 
+import numpy as np
+
 from .engine import ICGNError
 
 LIBRARY_PATH = Path(os.environ.get("LIVEDIC_LIBRARY", Path(__file__).resolve().parent / "liblivedic.so"))
@@ -224,28 +226,18 @@ SIGNATURES = {
     "ldic_copy_to_device": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "ldic_copy_to_host": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "ldic_device_synchronize": (ctypes.c_int, []),
+    "ldic_device_zero": (ctypes.c_int, [_P, ctypes.c_size_t, _P]),
+    "ldic_device_copy_async": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P]),
+    "ldic_stream_synchronize": (ctypes.c_int, [_P]),
 }
 
 _lib = None
-
-
-def _load_hip_runtime_of_torch() -> None:
-    """PyTorch-ROCm bundles its own HIP runtime under the soname of the system one
-    (libamdhip64.so.7).  Whichever is loaded first serves the whole process, and torch
-    only initialises its device on its own build, so when torch is installed it is
-    imported before liblivedic.so is loaded: the library then runs on torch's runtime
-    and the learned seed network can share the process."""
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
 
 
 def library() -> ctypes.CDLL:
     """Load liblivedic.so once and declare its signatures."""
     global _lib
     if _lib is None:
-        _load_hip_runtime_of_torch()
         if not LIBRARY_PATH.exists():
             raise ICGNError(
                 f"{LIBRARY_PATH} is missing; build it with `python live-dic_amd/build.py` (hipcc, gfx950)"
@@ -294,6 +286,80 @@ class DeviceBuffer:
             self.close()
         except Exception:
             pass
+
+
+BF16 = "bfloat16"
+
+
+class DeviceArray(DeviceBuffer):
+    """A C-ordered device array of a fixed shape and element type (``BF16`` is stored as 16-bit
+    patterns and read back as float32).  Transfers run on a HIP stream handle (0 or None: the
+    null stream); host reads wait for the device first."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(n) for n in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = dtype if dtype == BF16 else np.dtype(dtype)
+        self.itemsize = 2 if dtype == BF16 else self.dtype.itemsize
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        super().__init__(self.size * self.itemsize)
+
+    @classmethod
+    def zeros(cls, shape, dtype, stream: int | None = 0) -> "DeviceArray":
+        """A cleared array; the clear has completed on return, so any stream may use it."""
+        out = cls(shape, dtype)
+        out.zero(stream)
+        check(library().ldic_stream_synchronize(stream or None))
+        return out
+
+    @classmethod
+    def from_numpy(cls, array, dtype=None) -> "DeviceArray":
+        host = np.ascontiguousarray(array)
+        out = cls(host.shape, dtype or host.dtype)
+        out.upload(host)
+        return out
+
+    def numel(self) -> int:
+        return self.size
+
+    def data_ptr(self) -> int:
+        return self.address
+
+    def zero(self, stream: int | None = 0) -> None:
+        check(library().ldic_device_zero(self.ptr, self.nbytes, stream or None))
+
+    def copy_from(self, source, stream: int | None = 0) -> None:
+        """Copy a same-sized device array (or any object with ``data_ptr()``) or a host array."""
+        if isinstance(source, np.ndarray):
+            self.upload(source)
+            return
+        check(library().ldic_device_copy_async(self.ptr, source.data_ptr(), self.nbytes, stream or None))
+
+    def upload(self, array) -> None:
+        host = np.ascontiguousarray(array)
+        if self.dtype == BF16:
+            if host.dtype != np.uint16:
+                # float32 -> bfloat16 bit patterns, rounded to nearest even.
+                u = np.ascontiguousarray(host, np.float32).view(np.uint32).astype(np.uint64)
+                host = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        elif host.dtype != self.dtype:
+            host = host.astype(self.dtype)
+        if host.nbytes != self.nbytes:
+            raise ValueError(f"host array of {host.nbytes} bytes for a device array of {self.nbytes}")
+        super().upload(host)
+
+    def numpy(self) -> np.ndarray:
+        """Wait for the device, then copy to a host array (float32 for ``BF16``)."""
+        check(library().ldic_device_synchronize())
+        if self.dtype == BF16:
+            bits = np.empty(self.shape, np.uint16)
+            self.download(bits)
+            return (bits.astype(np.uint32) << 16).view(np.float32)
+        out = np.empty(self.shape, self.dtype)
+        self.download(out)
+        return out
+
+    def item(self) -> float:
+        return float(self.numpy().reshape(-1)[0])
 
 
 def check(code: int) -> None:

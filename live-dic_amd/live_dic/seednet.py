@@ -24,8 +24,9 @@ Architecture (``Architecture``):
 Every stage is supervised (Huber loss against the analytic field); gradients
 do not flow through the upsampled flow (stage-wise training).
 
-PyTorch provides device memory and streams only; every arithmetic kernel is
-in liblivedic.so.
+Device memory comes from the library (``_native.DeviceArray``) and work is
+queued on HIP streams given as handles (0: the null stream); every arithmetic
+kernel is in liblivedic.so, and no other framework is loaded.
 """
 
 from __future__ import annotations
@@ -38,17 +39,9 @@ from pathlib import Path
 import numpy as np
 
 from . import _native
-from .engine import ICGNError
+from ._native import BF16, DeviceArray
 
 WEIGHTS_PATH = Path(__file__).resolve().parent / "weights" / "seednet.npz"
-
-
-def _torch():
-    import torch
-
-    if not torch.cuda.is_available():
-        raise ICGNError("the learned seed network needs a HIP device")
-    return torch
 
 
 @dataclass(frozen=True)
@@ -117,19 +110,23 @@ def _kp(k: int) -> int:
 
 
 def _ptr(t) -> int:
+    """Device address of a DeviceArray (or of any array object exposing data_ptr())."""
     return t.data_ptr()
+
+
+def _zeros(shape, dtype, stream: int = 0) -> DeviceArray:
+    return DeviceArray.zeros(shape, dtype, stream)
 
 
 class SeedNet:
     """Parameters, activations and training state of the network on one device."""
 
-    def __init__(self, arch: Architecture | None = None, *, seed: int = 0, device: str = "cuda"):
-        torch = _torch()
-        self.torch = torch
+    def __init__(self, arch: Architecture | None = None, *, seed: int = 0, device: str | int = "cuda"):
+        if device not in ("cuda", "cuda:0", 0):
+            raise ValueError("the seed network runs on the library's current device (device 0)")
         self.arch = arch or Architecture()
-        self.device = torch.device(device)
         self.lib = _native.library()
-        self._tstream = torch.cuda.current_stream(self.device)
+        self._stream = 0
         rng = np.random.default_rng(seed)
         self.params = {}
         for name, real, stored, cout, stride, act in self.arch.convs():
@@ -145,21 +142,15 @@ class SeedNet:
 
     # -- parameters ------------------------------------------------------------------------------
     def _new_param(self, w, b, real, stored, cout, stride, act):
-        torch = self.torch
-        dev = self.device
-        p = {
+        f32 = np.float32
+        return {
             "real": real, "cin": stored, "cout": cout, "stride": stride, "act": act,
-            "w": torch.from_numpy(w).to(dev), "b": torch.from_numpy(b).to(dev),
-            "w16": torch.zeros(w.shape, dtype=torch.bfloat16, device=dev),
-            "wt16": torch.zeros((stored, _kp(9 * cout)), dtype=torch.bfloat16, device=dev),
-            "dw": torch.zeros(w.shape, dtype=torch.float32, device=dev),
-            "db": torch.zeros(b.shape, dtype=torch.float32, device=dev),
-            "mw": torch.zeros(w.shape, dtype=torch.float32, device=dev),
-            "vw": torch.zeros(w.shape, dtype=torch.float32, device=dev),
-            "mb": torch.zeros(b.shape, dtype=torch.float32, device=dev),
-            "vb": torch.zeros(b.shape, dtype=torch.float32, device=dev),
+            "w": DeviceArray.from_numpy(w), "b": DeviceArray.from_numpy(b),
+            "w16": _zeros(w.shape, BF16), "wt16": _zeros((stored, _kp(9 * cout)), BF16),
+            "dw": _zeros(w.shape, f32), "db": _zeros(b.shape, f32),
+            "mw": _zeros(w.shape, f32), "vw": _zeros(w.shape, f32),
+            "mb": _zeros(b.shape, f32), "vb": _zeros(b.shape, f32),
         }
-        return p
 
     def _refresh_weights(self) -> None:
         s = self.stream_handle
@@ -171,27 +162,27 @@ class SeedNet:
         """float64 (w (Cout, 3, 3, Cin), b) per convolution, in the oracle's layout (from the bf16 copies)."""
         out = {}
         for name, p in self.params.items():
-            w = p["w16"].float().cpu().numpy().astype(np.float64)
+            w = p["w16"].numpy().astype(np.float64)
             cout, cin, real = p["cout"], p["cin"], p["real"]
             w = w[:, :9 * cin].reshape(cout, 9, cin)[:, :, :real].reshape(cout, 3, 3, real)
-            out[name] = (w, p["b"].cpu().numpy().astype(np.float64))
+            out[name] = (w, p["b"].numpy().astype(np.float64))
         return out
 
     def gradients(self) -> dict:
         """float64 (dw, db) per convolution in the oracle's layout."""
         out = {}
         for name, p in self.params.items():
-            dw = p["dw"].cpu().numpy().astype(np.float64)
+            dw = p["dw"].numpy().astype(np.float64)
             cout, cin, real = p["cout"], p["cin"], p["real"]
             dw = dw[:, :9 * cin].reshape(cout, 9, cin)[:, :, :real].reshape(cout, 3, 3, real)
-            out[name] = (dw, p["db"].cpu().numpy().astype(np.float64))
+            out[name] = (dw, p["db"].numpy().astype(np.float64))
         return out
 
     def save(self, path: str | Path) -> None:
         arrays = {}
         for name, p in self.params.items():
-            arrays[f"{name}.w"] = p["w"].cpu().numpy()
-            arrays[f"{name}.b"] = p["b"].cpu().numpy()
+            arrays[f"{name}.w"] = p["w"].numpy()
+            arrays[f"{name}.b"] = p["b"].numpy()
         arrays["step"] = np.array(self.step_count)
         Path(path).parent.mkdir(parents=True, exist_ok=True)
         np.savez_compressed(path, **arrays)
@@ -200,69 +191,75 @@ class SeedNet:
         with np.load(path, allow_pickle=False) as data:
             for name, p in self.params.items():
                 w, b = data[f"{name}.w"], data[f"{name}.b"]
-                if w.shape != tuple(p["w"].shape) or b.shape != tuple(p["b"].shape):
+                if w.shape != p["w"].shape or b.shape != p["b"].shape:
                     raise ValueError(f"weight file does not match the architecture at {name}")
-                p["w"].copy_(self.torch.from_numpy(w))
-                p["b"].copy_(self.torch.from_numpy(b))
+                _native.check(self.lib.ldic_device_synchronize())
+                p["w"].upload(w)
+                p["b"].upload(b)
             self.step_count = int(data["step"]) if "step" in data else 0
         self._refresh_weights()
+        self.synchronize()
 
     # -- streams ---------------------------------------------------------------------------------
     @property
     def stream_handle(self) -> int:
-        return self._tstream.cuda_stream
+        return self._stream
 
     def use_stream(self, handle: int | None) -> None:
-        """Queue all further work on an external HIP stream (None: torch's current stream)."""
-        torch = self.torch
-        self._tstream = torch.cuda.current_stream(self.device) if handle is None else torch.cuda.ExternalStream(
-            handle, device=self.device)
+        """Queue all further work on a HIP stream given by its handle (None or 0: the null stream).
+        Work already queued on the previous stream completes first, so the new stream (which may be
+        a non-blocking one) never reads weights that are still being written."""
+        self.synchronize()
+        self._stream = int(handle or 0)
+
+    def synchronize(self) -> None:
+        """Wait for the work queued on the network's stream."""
+        _native.check(self.lib.ldic_stream_synchronize(self._stream or None))
 
     # -- buffers ---------------------------------------------------------------------------------
     def _ensure(self, n: int, h: int, w: int, training: bool) -> None:
         key = (n, h, w, training)
         if self._shape == key:
             return
-        torch = self.torch
-        dev = self.device
-        bf, f32 = torch.bfloat16, torch.float32
+        bf, f32 = BF16, np.float32
         a = self.arch
         self.n, self.h, self.w = n, h, w
-        self.img16 = torch.zeros((2 * n, h, w, 1), dtype=bf, device=dev)
+        _native.check(self.lib.ldic_device_synchronize())   # the old buffers may still be in use
+        self.img16 = _zeros((2 * n, h, w, 1), bf)
         self.enc, self.denc, self.enc_shapes = [], [], []
         hh, ww = h, w
         for cout, stride in a.encoder:
             hh, ww = (hh - 1) // stride + 1, (ww - 1) // stride + 1
             self.enc_shapes.append((hh, ww, cout))
-            self.enc.append(torch.zeros((2 * n, hh, ww, cout), dtype=bf, device=dev))
-            self.denc.append(torch.zeros((2 * n, hh, ww, cout), dtype=f32, device=dev) if training else None)
+            self.enc.append(_zeros((2 * n, hh, ww, cout), bf))
+            self.denc.append(_zeros((2 * n, hh, ww, cout), f32) if training else None)
         self.st = []
         for si, stage in enumerate(a.stages):
             hh, ww, c = self.enc_shapes[stage.level]
             _, stored = a.stage_channels(si)
-            b = {"x": torch.zeros((n, hh, ww, stored), dtype=bf, device=dev)}
-            b["acts"] = [torch.zeros((n, hh, ww, co), dtype=bf, device=dev) for co, _ in stage.head[:-1]]
-            b["res"] = torch.zeros((n, hh, ww, 2), dtype=f32, device=dev)
-            b["flow"] = torch.zeros((n, hh, ww, 2), dtype=f32, device=dev)
+            b = {"x": _zeros((n, hh, ww, stored), bf)}
+            b["acts"] = [_zeros((n, hh, ww, co), bf) for co, _ in stage.head[:-1]]
+            b["res"] = _zeros((n, hh, ww, 2), f32)
+            b["flow"] = _zeros((n, hh, ww, 2), f32)
             if si > 0:
-                b["up"] = torch.zeros((n, hh, ww, 2), dtype=f32, device=dev)
-                b["warped"] = torch.zeros((n, hh, ww, c), dtype=bf, device=dev)
+                b["up"] = _zeros((n, hh, ww, 2), f32)
+                b["warped"] = _zeros((n, hh, ww, c), bf)
             if training:
-                b["dx"] = torch.zeros((n, hh, ww, stored), dtype=f32, device=dev)
-                b["dacts"] = [torch.zeros((n, hh, ww, co), dtype=f32, device=dev) for co, _ in stage.head[:-1]]
-                b["dflow"] = torch.zeros((n, hh, ww, 2), dtype=f32, device=dev)
-                b["truth"] = torch.zeros((n, hh, ww, 2), dtype=f32, device=dev)
+                b["dx"] = _zeros((n, hh, ww, stored), f32)
+                b["dacts"] = [_zeros((n, hh, ww, co), f32) for co, _ in stage.head[:-1]]
+                b["dflow"] = _zeros((n, hh, ww, 2), f32)
+                b["truth"] = _zeros((n, hh, ww, 2), f32)
                 if si > 0:
-                    b["dwarped"] = torch.zeros((n, hh, ww, c), dtype=f32, device=dev)
+                    b["dwarped"] = _zeros((n, hh, ww, c), f32)
             self.st.append(b)
-        self.loss = torch.zeros(1, dtype=f32, device=dev)
+        self.loss = _zeros(1, f32)
         # Split-K workspace: the largest output of any convolution at this shape.
         largest = max(2 * n * hh_ * ww_ * c for hh_, ww_, c in self.enc_shapes)
         for si, stage in enumerate(a.stages):
             hh, ww, _ = self.enc_shapes[stage.level]
             largest = max(largest, n * hh * ww * max(co for co, _ in stage.head))
-        self.workspace = torch.zeros(largest, dtype=f32, device=dev)
-        torch.cuda.synchronize(dev)
+        self.workspace = _zeros(largest, f32)
+        _native.check(self.lib.ldic_device_synchronize())
         self._shape = key
 
     # -- layers ----------------------------------------------------------------------------------
@@ -290,113 +287,114 @@ class SeedNet:
         self._ensure(n, h, w, training)
         lib, s = self.lib, self.stream_handle
         a = self.arch
-        with self.torch.cuda.stream(self._tstream):
-            if images_u8:
-                half = n * h * w
-                _native.check(lib.ldic_nn_images_to_bf16(half, ref16_src, _ptr(self.img16), s))
-                _native.check(lib.ldic_nn_images_to_bf16(half, tgt16_src, _ptr(self.img16) + 2 * half, s))
+        if images_u8:
+            half = n * h * w
+            _native.check(lib.ldic_nn_images_to_bf16(half, ref16_src, _ptr(self.img16), s))
+            _native.check(lib.ldic_nn_images_to_bf16(half, tgt16_src, _ptr(self.img16) + 2 * half, s))
+        else:
+            self.img16.copy_from(ref16_src, s)
+        self._gconv = {}
+        x, hh, ww, cin = self.img16, h, w, 1
+        for i, (cout, stride) in enumerate(a.encoder):
+            p = self.params[f"enc{i}"]
+            self._gconv[f"enc{i}"] = self._conv(p, 2 * n, hh, ww, x, cin, 0, self.enc[i], cout)
+            x, (hh, ww, cin) = self.enc[i], self.enc_shapes[i]
+        prev = None
+        for si, stage in enumerate(a.stages):
+            b = self.st[si]
+            hh, ww, c = self.enc_shapes[stage.level]
+            feats = self.enc[stage.level]
+            f1, f2 = _ptr(feats), _ptr(feats) + 2 * n * hh * ww * c   # bf16: 2 bytes per element
+            d2 = (2 * stage.radius + 1) ** 2
+            _, stored = a.stage_channels(si)
+            if prev is not None:
+                ph, pw = prev.shape[1], prev.shape[2]
+                _native.check(lib.ldic_nn_upsample_flow(n, ph, pw, hh, ww, _ptr(prev), _ptr(b["up"]),
+                                                        _ptr(b["x"]), stored, d2 + c, s))
+                _native.check(lib.ldic_nn_warp_forward(n, hh, ww, c, f2, _ptr(b["up"]), _ptr(b["warped"]), s))
+                f2 = _ptr(b["warped"])
+                f2_stride = c
             else:
-                self.img16.copy_(ref16_src)
-            self._gconv = {}
-            x, hh, ww, cin = self.img16, h, w, 1
-            for i, (cout, stride) in enumerate(a.encoder):
-                p = self.params[f"enc{i}"]
-                self._gconv[f"enc{i}"] = self._conv(p, 2 * n, hh, ww, x, cin, 0, self.enc[i], cout)
-                x, (hh, ww, cin) = self.enc[i], self.enc_shapes[i]
-            prev = None
-            for si, stage in enumerate(a.stages):
-                b = self.st[si]
-                hh, ww, c = self.enc_shapes[stage.level]
-                feats = self.enc[stage.level]
-                f1, f2 = _ptr(feats), _ptr(feats) + 2 * n * hh * ww * c   # bf16: 2 bytes per element
-                d2 = (2 * stage.radius + 1) ** 2
-                _, stored = a.stage_channels(si)
-                if prev is not None:
-                    ph, pw = prev.shape[1], prev.shape[2]
-                    _native.check(lib.ldic_nn_upsample_flow(n, ph, pw, hh, ww, _ptr(prev), _ptr(b["up"]),
-                                                            _ptr(b["x"]), stored, d2 + c, s))
-                    _native.check(lib.ldic_nn_warp_forward(n, hh, ww, c, f2, _ptr(b["up"]), _ptr(b["warped"]), s))
-                    f2 = _ptr(b["warped"])
-                    f2_stride = c
+                f2_stride = c
+            corr = _native.NNCorr(n, hh, ww, c, stage.radius, c, 0, f2_stride, 0, stored, 0, 1.0 / c)
+            b["corr"] = corr
+            _native.check(lib.ldic_nn_corr_forward(ctypes.byref(corr), f1, f2, _ptr(b["x"]), s))
+            _native.check(lib.ldic_nn_copy_slice(n * hh * ww, c, f1, c, 0, _ptr(b["x"]), stored, d2, s))
+            x, xs = b["x"], stored
+            b["g"] = []
+            for hi, (cout, act) in enumerate(stage.head):
+                p = self.params[f"s{si}h{hi}"]
+                last = hi == len(stage.head) - 1
+                if last:
+                    g = self._conv(p, n, hh, ww, x, xs, 0, None, 2, y_f32=b["res"])
                 else:
-                    f2_stride = c
-                corr = _native.NNCorr(n, hh, ww, c, stage.radius, c, 0, f2_stride, 0, stored, 0, 1.0 / c)
-                b["corr"] = corr
-                _native.check(lib.ldic_nn_corr_forward(ctypes.byref(corr), f1, f2, _ptr(b["x"]), s))
-                _native.check(lib.ldic_nn_copy_slice(n * hh * ww, c, f1, c, 0, _ptr(b["x"]), stored, d2, s))
-                x, xs = b["x"], stored
-                b["g"] = []
-                for hi, (cout, act) in enumerate(stage.head):
-                    p = self.params[f"s{si}h{hi}"]
-                    last = hi == len(stage.head) - 1
-                    if last:
-                        g = self._conv(p, n, hh, ww, x, xs, 0, None, 2, y_f32=b["res"])
-                    else:
-                        g = self._conv(p, n, hh, ww, x, xs, 0, b["acts"][hi], cout)
-                        x, xs = b["acts"][hi], cout
-                    b["g"].append(g)
-                b["flow"].copy_(b["res"])
-                if prev is not None:
-                    _native.check(lib.ldic_nn_add_slice(n * hh * ww, 2, _ptr(b["up"]), 2, 0, _ptr(b["flow"]), 2, 0,
-                                                        s))
-                prev = b["flow"]
+                    g = self._conv(p, n, hh, ww, x, xs, 0, b["acts"][hi], cout)
+                    x, xs = b["acts"][hi], cout
+                b["g"].append(g)
+            b["flow"].copy_from(b["res"], s)
+            if prev is not None:
+                _native.check(lib.ldic_nn_add_slice(n * hh * ww, 2, _ptr(b["up"]), 2, 0, _ptr(b["flow"]), 2, 0,
+                                                    s))
+            prev = b["flow"]
         return [b["flow"] for b in self.st]
 
     def backward(self, truths, weights) -> None:
         """Accumulate parameter gradients of the supervised loss (call after forward(training=True));
         ``truths`` are (n, h_s, w_s, 2) float32 tensors per stage."""
-        torch = self.torch
         lib, s = self.lib, self.stream_handle
         a = self.arch
         n = self.n
-        with torch.cuda.stream(self._tstream):
-            self.loss.zero_()
-            for p in self.params.values():
-                p["dw"].zero_()
-                p["db"].zero_()
-            for d in self.denc:
-                d.zero_()
-            for si in reversed(range(len(a.stages))):
-                stage, b = a.stages[si], self.st[si]
-                hh, ww, c = self.enc_shapes[stage.level]
-                d2 = (2 * stage.radius + 1) ** 2
-                _, stored = a.stage_channels(si)
-                b["truth"].copy_(truths[si])
-                _native.check(lib.ldic_nn_flow_loss(n * hh * ww, _ptr(b["flow"]), _ptr(b["truth"]),
-                                                    float(weights[si]), _ptr(b["dflow"]), _ptr(self.loss), s))
-                dy = b["dflow"]
-                for hi in reversed(range(len(stage.head))):
-                    p = self.params[f"s{si}h{hi}"]
-                    x = b["x"] if hi == 0 else b["acts"][hi - 1]
-                    if p["act"]:
-                        _native.check(lib.ldic_nn_leaky_backward(dy.numel(), _ptr(b["acts"][hi]), _ptr(dy), s))
-                    if hi == 0:
-                        dx, dxs = b["dx"], stored
-                    else:
-                        dx, dxs = b["dacts"][hi - 1], stage.head[hi - 1][0]
-                    self._conv_back(p, b["g"][hi], x, dy, dx, dxs, False)
-                    dy = dx
-                feats, dfeats = self.enc[stage.level], self.denc[stage.level]
-                f1, f2 = _ptr(feats), _ptr(feats) + 2 * n * hh * ww * c
-                df1, df2 = _ptr(dfeats), _ptr(dfeats) + 4 * n * hh * ww * c   # float32: 4 bytes
-                if si > 0:
-                    b["dwarped"].zero_()
-                    _native.check(lib.ldic_nn_corr_backward(ctypes.byref(b["corr"]), f1, _ptr(b["warped"]),
-                                                            _ptr(b["dx"]), stored, 0, df1, c, 0,
-                                                            _ptr(b["dwarped"]), c, 0, s))
-                    _native.check(lib.ldic_nn_warp_backward(n, hh, ww, c, _ptr(b["up"]), _ptr(b["dwarped"]), df2, s))
+        if any(isinstance(t, np.ndarray) for t in truths):
+            # Host targets are copied synchronously into buffers that the previous step
+            # may still be reading on the stream.
+            self.synchronize()
+        self.loss.zero(s)
+        for p in self.params.values():
+            p["dw"].zero(s)
+            p["db"].zero(s)
+        for d in self.denc:
+            d.zero(s)
+        for si in reversed(range(len(a.stages))):
+            stage, b = a.stages[si], self.st[si]
+            hh, ww, c = self.enc_shapes[stage.level]
+            d2 = (2 * stage.radius + 1) ** 2
+            _, stored = a.stage_channels(si)
+            b["truth"].copy_from(truths[si], s)
+            _native.check(lib.ldic_nn_flow_loss(n * hh * ww, _ptr(b["flow"]), _ptr(b["truth"]),
+                                                float(weights[si]), _ptr(b["dflow"]), _ptr(self.loss), s))
+            dy = b["dflow"]
+            for hi in reversed(range(len(stage.head))):
+                p = self.params[f"s{si}h{hi}"]
+                x = b["x"] if hi == 0 else b["acts"][hi - 1]
+                if p["act"]:
+                    _native.check(lib.ldic_nn_leaky_backward(dy.numel(), _ptr(b["acts"][hi]), _ptr(dy), s))
+                if hi == 0:
+                    dx, dxs = b["dx"], stored
                 else:
-                    _native.check(lib.ldic_nn_corr_backward(ctypes.byref(b["corr"]), f1, f2, _ptr(b["dx"]), stored, 0,
-                                                            df1, c, 0, df2, c, 0, s))
-                _native.check(lib.ldic_nn_add_slice(n * hh * ww, c, _ptr(b["dx"]), stored, d2, df1, c, 0, s))
-            for i in reversed(range(len(a.encoder))):
-                p = self.params[f"enc{i}"]
-                _native.check(lib.ldic_nn_leaky_backward(self.denc[i].numel(), _ptr(self.enc[i]),
-                                                         _ptr(self.denc[i]), s))
-                x = self.img16 if i == 0 else self.enc[i - 1]
-                dx = None if i == 0 else self.denc[i - 1]
-                self._conv_back(p, self._gconv[f"enc{i}"], x, self.denc[i], dx,
-                                0 if i == 0 else a.encoder[i - 1][0], True)
+                    dx, dxs = b["dacts"][hi - 1], stage.head[hi - 1][0]
+                self._conv_back(p, b["g"][hi], x, dy, dx, dxs, False)
+                dy = dx
+            feats, dfeats = self.enc[stage.level], self.denc[stage.level]
+            f1, f2 = _ptr(feats), _ptr(feats) + 2 * n * hh * ww * c
+            df1, df2 = _ptr(dfeats), _ptr(dfeats) + 4 * n * hh * ww * c   # float32: 4 bytes
+            if si > 0:
+                b["dwarped"].zero(s)
+                _native.check(lib.ldic_nn_corr_backward(ctypes.byref(b["corr"]), f1, _ptr(b["warped"]),
+                                                        _ptr(b["dx"]), stored, 0, df1, c, 0,
+                                                        _ptr(b["dwarped"]), c, 0, s))
+                _native.check(lib.ldic_nn_warp_backward(n, hh, ww, c, _ptr(b["up"]), _ptr(b["dwarped"]), df2, s))
+            else:
+                _native.check(lib.ldic_nn_corr_backward(ctypes.byref(b["corr"]), f1, f2, _ptr(b["dx"]), stored, 0,
+                                                        df1, c, 0, df2, c, 0, s))
+            _native.check(lib.ldic_nn_add_slice(n * hh * ww, c, _ptr(b["dx"]), stored, d2, df1, c, 0, s))
+        for i in reversed(range(len(a.encoder))):
+            p = self.params[f"enc{i}"]
+            _native.check(lib.ldic_nn_leaky_backward(self.denc[i].numel(), _ptr(self.enc[i]),
+                                                     _ptr(self.denc[i]), s))
+            x = self.img16 if i == 0 else self.enc[i - 1]
+            dx = None if i == 0 else self.denc[i - 1]
+            self._conv_back(p, self._gconv[f"enc{i}"], x, self.denc[i], dx,
+                            0 if i == 0 else a.encoder[i - 1][0], True)
 
     def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         self.step_count += 1
@@ -412,20 +410,22 @@ class SeedNet:
         """Pick each forward convolution's output-channel tile (32, 64, 128 per workgroup) and its
         split of the GEMM columns (1, 2, 4, 8) by timing whole forward passes on device-resident
         inputs of this shape, one layer at a time."""
-        torch = self.torch
-        imgs = torch.randint(0, 256, (2 * n, h, w), dtype=torch.uint8, device=self.device)
+        import time
+
+        rng = np.random.default_rng(0)
+        imgs = DeviceArray.from_numpy(rng.integers(0, 256, (2 * n, h, w), dtype=np.uint8))
         ref, tgt = _ptr(imgs), _ptr(imgs) + n * h * w
 
         def timed():
-            with torch.cuda.stream(self._tstream):
-                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            # Wall time of `reps` queued forwards between two waits on the stream: the
+            # kernels of one forward keep the device busy longer than their launches take.
+            self.forward(ref, tgt, n=n, h=h, w=w)
+            self.synchronize()
+            start = time.perf_counter()
+            for _ in range(reps):
                 self.forward(ref, tgt, n=n, h=h, w=w)
-                start.record(self._tstream)
-                for _ in range(reps):
-                    self.forward(ref, tgt, n=n, h=h, w=w)
-                end.record(self._tstream)
-                end.synchronize()
-                return start.elapsed_time(end) / reps
+            self.synchronize()
+            return (time.perf_counter() - start) * 1e3 / reps
 
         for p in self.params.values():
             p["tile"], p["splits"] = 0, 1
@@ -476,12 +476,12 @@ class TrainingPairs:
         self.rng = np.random.default_rng(seed)
         self.refresh = texture_refresh
         self.synthetic = synthetic
-        torch = net.torch
-        dev = net.device
-        self.full = torch.zeros((2 * batch, crop, crop), dtype=torch.uint8, device=dev)
+        self.full = _zeros((2 * batch, crop, crop), np.uint8)
         c = crop >> net.arch.input_level
-        self.half = torch.zeros((2 * batch, crop // 2, crop // 2), dtype=torch.uint8, device=dev)
-        self.small = torch.zeros((2 * batch, c, c), dtype=torch.uint8, device=dev)
+        self.half = _zeros((2 * batch, crop // 2, crop // 2), np.uint8)
+        self.small = _zeros((2 * batch, c, c), np.uint8)
+        self.fields = DeviceArray(2 * batch * ctypes.sizeof(_native.Field), np.uint8)
+        self.offsets = DeviceArray((2 * batch, 2), np.int32)
         self.size = c
         self.pairs = 0
         self._texture = None
@@ -495,7 +495,7 @@ class TrainingPairs:
 
     def next(self):
         """Render one batch; returns (device pointers of reference and target inputs, truths per stage)."""
-        net, torch, lib = self.net, self.net.torch, self.net.lib
+        net, lib = self.net, self.net.lib
         if self.pairs and self.pairs // self.batch % self.refresh == 0:
             self._new_texture()
         B, C = self.batch, self.crop
@@ -509,19 +509,20 @@ class TrainingPairs:
             ox = int(self.rng.integers(0, FRAME[0] - C + 1))
             oy = int(self.rng.integers(0, FRAME[1] - C + 1))
             offsets[b] = offsets[B + b] = (ox, oy)
-        dev = net.device
-        with torch.cuda.stream(net._tstream):
-            fbuf = torch.frombuffer(bytearray(bytes(fields)), dtype=torch.uint8).to(dev)
-            obuf = torch.from_numpy(offsets).to(dev)
-            t = self._texture
-            sp = _native.Speckle(t._blobs.address, t._starts.address, t.cells_x, t.cells_y, t.cell, float(t.origin),
-                                 t.sigma, t.background)
-            s = net.stream_handle
-            _native.check(lib.ldic_render_speckle_batch(ctypes.byref(sp), _ptr(fbuf), _ptr(obuf), 2 * B, C, C,
-                                                        self.noise, int(self.rng.integers(1 << 62)), _ptr(self.full),
-                                                        s))
-            _native.check(lib.ldic_reduce_device(_ptr(self.full), C, C, 2 * B, _ptr(self.half), s))
-            _native.check(lib.ldic_reduce_device(_ptr(self.half), C // 2, C // 2, 2 * B, _ptr(self.small), s))
+        # The previous batch's render may still read the field and offset tables.
+        net.synchronize()
+        self.fields.upload(np.frombuffer(bytes(fields), np.uint8))
+        self.offsets.upload(offsets)
+        fbuf, obuf = self.fields, self.offsets
+        t = self._texture
+        sp = _native.Speckle(t._blobs.address, t._starts.address, t.cells_x, t.cells_y, t.cell, float(t.origin),
+                             t.sigma, t.background)
+        s = net.stream_handle
+        _native.check(lib.ldic_render_speckle_batch(ctypes.byref(sp), _ptr(fbuf), _ptr(obuf), 2 * B, C, C,
+                                                    self.noise, int(self.rng.integers(1 << 62)), _ptr(self.full),
+                                                    s))
+        _native.check(lib.ldic_reduce_device(_ptr(self.full), C, C, 2 * B, _ptr(self.half), s))
+        _native.check(lib.ldic_reduce_device(_ptr(self.half), C // 2, C // 2, 2 * B, _ptr(self.small), s))
         truths = []
         a = net.arch
         for st in a.stages:
@@ -532,7 +533,7 @@ class TrainingPairs:
             for b, fld in enumerate(flds):
                 u, v = fld.displacement(xx * sc + offsets[b, 0], yy * sc + offsets[b, 1])
                 tr[b, ..., 0], tr[b, ..., 1] = u / sc, v / sc
-            truths.append(torch.from_numpy(tr).to(dev))
+            truths.append(tr)
         self.pairs += B
         half = B * self.size * self.size
         return _ptr(self.small), _ptr(self.small) + half, truths
@@ -550,7 +551,6 @@ def train(net: SeedNet, *, pairs: int = 50_000, batch: int = 8, crop: int = 1024
     data = TrainingPairs(net, batch=batch, crop=crop, seed=seed)
     steps = max(1, pairs // batch)
     history = []
-    torch = net.torch
     for k in range(steps):
         ref, tgt, truths = data.next()
         net.forward(ref, tgt, n=batch, h=data.size, w=data.size, training=True)
@@ -558,12 +558,11 @@ def train(net: SeedNet, *, pairs: int = 50_000, batch: int = 8, crop: int = 1024
         rate = lr * (0.5 * (1 + math.cos(math.pi * k / steps)) * 0.95 + 0.05)
         net.adam_step(rate)
         if k % 50 == 0 or k == steps - 1:
-            with torch.cuda.stream(net._tstream):
-                value = float(net.loss.item())
+            value = net.loss.item()
             history.append((k, value))
             if log:
                 log(f"step {k}/{steps} loss {value:.5f}")
-    torch.cuda.synchronize(net.device)
+    net.synchronize()
     return history
 
 
@@ -577,7 +576,6 @@ class LearnedSeeder:
 
     def __init__(self, engine, net: SeedNet, *, slot: int = 0, fallback_zncc: float = 0.9,
                  fallback_radius: int = 8, reseed_passes: int = 2, gradients: bool = True):
-        torch = net.torch
         self.engine, self.net, self.slot = engine, net, slot
         lib = engine._lib
         handle = ctypes.c_void_p()
@@ -586,14 +584,13 @@ class LearnedSeeder:
         net.use_stream(self.stream)
         engine._set_initialization(_native.INIT_ZERO, 0, fallback_zncc, fallback_radius, reseed_passes=reseed_passes)
         self.width, self.height, self.gw, self.gh = engine._level_shapes[0]
-        dev = net.device
         L = net.arch.input_level
         self.sizes = [(self.width >> k, self.height >> k) for k in range(L + 1)]
-        self.ref_levels = [torch.zeros((h, w), dtype=torch.uint8, device=dev) for w, h in self.sizes[1:]]
-        self.tgt_levels = [torch.zeros((h, w), dtype=torch.uint8, device=dev) for w, h in self.sizes[1:]]
+        self.ref_levels = [_zeros((h, w), np.uint8) for w, h in self.sizes[1:]]
+        self.tgt_levels = [_zeros((h, w), np.uint8) for w, h in self.sizes[1:]]
         n = self.gw * self.gh
-        self.seeds = torch.zeros((n, 8), dtype=torch.float32, device=dev)
-        self.active = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.seeds = _zeros((n, 8), np.float32)
+        self.active = _zeros(n, np.int32)
         self.gradients = gradients
         self.scale = net.output_scale()
         w, h = self.sizes[-1]

@@ -3,7 +3,10 @@
 from __future__ import annotations
 
 import ctypes
+import os
 import re
+import subprocess
+import sys
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -40,3 +43,13 @@ def test_abi_version_and_error_channel_without_device():
     assert lib.ldic_create(ctypes.byref(opts), ctypes.byref(handle)) == -1
     assert b"subset_size" in lib.ldic_last_error()
     assert not handle.value
+
+
+def test_engines_and_the_seed_network_load_without_torch():
+    """Neither the library nor the seed-network module imports a tensor framework: device
+    memory and streams come from the library itself."""
+    code = ("import sys; from live_dic import ICGNWGPUFullPyramid, ICGNWGPU, seednet, batch, _native; "
+            "_native.library(); assert 'torch' not in sys.modules, 'torch was imported'")
+    env = {**os.environ, "PYTHONPATH": str(ROOT / "live-dic_amd")}
+    proc = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stderr[-2000:]

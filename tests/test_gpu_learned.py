@@ -11,6 +11,10 @@
 from __future__ import annotations
 
 import math
+import os
+import subprocess
+import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -102,3 +106,57 @@ def _download(buf):
     out = np.empty((H, W), np.uint8)
     buf.download(out)
     return out
+
+
+_NO_TORCH_RUN = r"""
+import sys
+import numpy as np
+from live_dic import ICGNWGPU, _native, seednet, synthetic
+
+W, H, S, D = 1920, 1080, 31, 8
+rng = np.random.default_rng(7)
+fld = seednet.random_affine(rng, W, H, translation=12.0, rotation_deg=2.0, strain=0.01)
+tex = synthetic.SpeckleTexture(W, H, synthetic.blobs(W, H, int(round(synthetic.BLOB_DENSITY * W * H)), 7))
+ref, tgt = _native.DeviceBuffer(W * H), _native.DeviceBuffer(W * H)
+tex.render_device(ref, None, 2.0, 1)
+tex.render_device(tgt, fld, 2.0, 2)
+host = np.empty((H, W), np.uint8)
+ref.download(host)
+lib = _native.library()
+with ICGNWGPU(subset_size=S, subset_stride=D) as eng:
+    eng.set_reference(host, iterations=20, tolerance=1e-3)
+    net = seednet.SeedNet()
+    net.load(seednet.WEIGHTS_PATH)
+    seeder = seednet.LearnedSeeder(eng, net)
+    seeder.set_reference(ref.address)
+    seeder.submit(tgt.address)
+    _native.check(lib.ldic_slot_wait(eng._handle, 0, None))
+    state = np.empty((eng.grid_height, eng.grid_width, 8), np.float32)
+    _native.check(lib.ldic_slot_output(eng._handle, 0, state.ctypes.data))
+cx, cy = synthetic.subset_centres(W, H, S, D)
+u, v = fld.displacement(cx, cy)
+ok = state[..., 6] > 0.5
+err = np.hypot(state[..., 0] - u, state[..., 3] - v)[ok]
+print("accepted", ok.mean(), "median", float(np.median(err)), flush=True)
+assert ok.mean() > 0.9 and np.median(err) < 0.02
+assert "torch" not in sys.modules, "torch was imported"
+"""
+
+
+def test_learned_path_runs_without_torch():
+    """The seed network, its seeder and the seeded solve run in a process that never imports
+    a tensor framework (a 1080p pair of modest displacement)."""
+    if not _trained():
+        pytest.skip("no trained weights committed")
+    _say("learned seeder in a torch-free process")
+    root = Path(__file__).resolve().parents[1]
+    proc = subprocess.run([sys.executable, "-u", "-c", _NO_TORCH_RUN], capture_output=True, text=True, timeout=240,
+                          env={**os.environ, "PYTHONPATH": str(root / "live-dic_amd")})
+    print(proc.stdout[-1000:])
+    assert proc.returncode == 0, proc.stderr[-3000:]
+
+
+def _trained() -> bool:
+    from live_dic import seednet
+
+    return seednet.WEIGHTS_PATH.exists()

@@ -31,18 +31,31 @@ def bf16(a):
 
 @pytest.fixture(scope="module")
 def env():
-    import torch
-
     from live_dic import _native
 
-    return torch, _native, _native.library()
+    return _native, _native.library()
 
 
-def _dev(torch, a, dtype=None):
-    t = torch.from_numpy(np.ascontiguousarray(a))
-    if dtype is not None:
-        t = t.to(dtype)
-    return t.cuda()
+def _dev(a, dtype=None):
+    """Device copy of a host array (dtype BF16: rounded to bfloat16)."""
+    from live_dic import _native
+
+    return _native.DeviceArray.from_numpy(np.ascontiguousarray(a), dtype)
+
+
+def _zeros(shape, dtype):
+    from live_dic import _native
+
+    return _native.DeviceArray.zeros(shape, dtype)
+
+
+def _sync():
+    from live_dic import _native
+
+    _native.check(_native.library().ldic_device_synchronize())
+
+
+BF16 = "bfloat16"
 
 
 def _kp(k):
@@ -68,7 +81,7 @@ def _rel(a, b):
     (2, 8, 9, 24, 2, 1, False, 24, 0),        # flow head: float32 output, 2 channels
 ])
 def test_conv_forward_and_gradients(env, n, h, w, cin, cout, stride, leaky, stored, offset):
-    torch, native, lib = env
+    native, lib = env
     _say("conv", n, h, w, cin, cout, stride, leaky, stored, offset)
     rng = np.random.default_rng(cin * 7 + cout)
     x_full = np.zeros((n, h, w, stored), np.float32)
@@ -78,55 +91,55 @@ def test_conv_forward_and_gradients(env, n, h, w, cin, cout, stride, leaky, stor
     b = rng.normal(scale=0.1, size=cout).astype(np.float32)
     ho, wo = (h - 1) // stride + 1, (w - 1) // stride + 1
     g = native.NNConv(n, h, w, cin, stored, offset, ho, wo, cout, cout, 0, stride)
-    xd = _dev(torch, x_full, torch.bfloat16)
-    wd = _dev(torch, _pack_weights(wt), torch.bfloat16)
-    bd = _dev(torch, b)
+    xd = _dev(x_full, BF16)
+    wd = _dev(_pack_weights(wt), BF16)
+    bd = _dev(b)
     f32_out = cout == 2
-    y16 = torch.zeros((n, ho, wo, cout), dtype=torch.bfloat16, device="cuda")
-    y32 = torch.zeros((n, ho, wo, cout), dtype=torch.float32, device="cuda")
+    y16 = _zeros((n, ho, wo, cout), BF16)
+    y32 = _zeros((n, ho, wo, cout), np.float32)
     native.check(lib.ldic_nn_conv_forward(ctypes.byref(g), xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
                                           1 if leaky else 0, None if f32_out else y16.data_ptr(),
                                           y32.data_ptr() if f32_out else None, None))
-    torch.cuda.synchronize()
-    y = (y32 if f32_out else y16.float()).cpu().numpy()
+    _sync()
+    y = (y32 if f32_out else y16).numpy()
     y_ref = onet.conv_forward(x, wt.astype(np.float64), b.astype(np.float64), stride, leaky)
     assert _rel(y, y_ref) < (1e-6 if f32_out else 4e-3)
     # Split-K: the GEMM columns in three slices through a float32 workspace.
-    work = torch.zeros(n * ho * wo * cout, dtype=torch.float32, device="cuda")
-    y16.zero_()
-    y32.zero_()
+    work = _zeros(n * ho * wo * cout, np.float32)
+    y16.zero()
+    y32.zero()
     native.check(lib.ldic_nn_conv_forward_split(ctypes.byref(g), xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
                                                 1 if leaky else 0, None if f32_out else y16.data_ptr(),
                                                 y32.data_ptr() if f32_out else None, work.data_ptr(), 3, None))
-    torch.cuda.synchronize()
-    ys = (y32 if f32_out else y16.float()).cpu().numpy()
+    _sync()
+    ys = (y32 if f32_out else y16).numpy()
     assert _rel(ys, y_ref) < (1e-6 if f32_out else 4e-3)
 
     dy = rng.normal(size=(n, ho, wo, cout)).astype(np.float32)
     dy_b = bf16(dy).astype(np.float64)          # the kernels feed dy to the matrix cores as bfloat16
     dx_ref, dw_ref, db_ref = onet.conv_backward(x, wt.astype(np.float64), dy_b, stride)
-    wtt = torch.zeros((cin, _kp(9 * cout)), dtype=torch.bfloat16, device="cuda")
+    wtt = _zeros((cin, _kp(9 * cout)), BF16)
     native.check(lib.ldic_nn_transpose_weights(cin, cout, wd.data_ptr(), wtt.data_ptr(), None))
-    dyd = _dev(torch, dy)
-    dx = torch.full((n, h, w, stored), 7.0, dtype=torch.float32, device="cuda")
-    dwd = torch.zeros((cout, _kp(9 * cin)), dtype=torch.float32, device="cuda")
-    dbd = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    dyd = _dev(dy)
+    dx = _dev(np.full((n, h, w, stored), 7.0, np.float32))
+    dwd = _zeros((cout, _kp(9 * cin)), np.float32)
+    dbd = _zeros(cout, np.float32)
     native.check(lib.ldic_nn_conv_backward(ctypes.byref(g), xd.data_ptr(), wtt.data_ptr(), dyd.data_ptr(),
                                            dx.data_ptr(), stored, offset, 1, dwd.data_ptr(), dbd.data_ptr(), None))
-    torch.cuda.synchronize()
-    dxh = dx.cpu().numpy()
+    _sync()
+    dxh = dx.numpy()
     # Accumulated into the slice, channels outside it untouched.
     assert _rel(dxh[..., offset:offset + cin] - 7.0, dx_ref) < 1e-5
     if stored > cin:
         assert np.all(np.delete(dxh, np.s_[offset:offset + cin], axis=-1) == 7.0)
-    dwh = dwd.cpu().numpy()
+    dwh = dwd.numpy()
     assert np.all(dwh[:, 9 * cin:] == 0.0)
     assert _rel(dwh[:, :9 * cin].reshape(dw_ref.shape), dw_ref) < 1e-5
-    assert _rel(dbd.cpu().numpy(), dy.sum(axis=(0, 1, 2))) < 1e-5
+    assert _rel(dbd.numpy(), dy.sum(axis=(0, 1, 2))) < 1e-5
 
 
 def test_correlation_warp_upsample_loss_adam(env):
-    torch, native, lib = env
+    native, lib = env
     _say("correlation, warp, upsample, loss, adam")
     rng = np.random.default_rng(11)
     n, h, w, c, r = 2, 9, 10, 12, 3
@@ -135,71 +148,73 @@ def test_correlation_warp_upsample_loss_adam(env):
     D2 = (2 * r + 1) ** 2
     stored = D2 + 3
     spec = native.NNCorr(n, h, w, c, r, c, 0, c, 0, stored, 0, 1.0 / c)
-    d1, d2 = _dev(torch, f1, torch.bfloat16), _dev(torch, f2, torch.bfloat16)
-    out = torch.zeros((n, h, w, stored), dtype=torch.bfloat16, device="cuda")
+    d1, d2 = _dev(f1, BF16), _dev(f2, BF16)
+    out = _zeros((n, h, w, stored), BF16)
     native.check(lib.ldic_nn_corr_forward(ctypes.byref(spec), d1.data_ptr(), d2.data_ptr(), out.data_ptr(), None))
     ref = onet.corr_forward(f1.astype(np.float64), f2.astype(np.float64), r, 1.0 / c)
-    torch.cuda.synchronize()
-    assert _rel(out.float().cpu().numpy()[..., :D2], ref) < 4e-3
+    _sync()
+    assert _rel(out.numpy()[..., :D2], ref) < 4e-3
     dc = rng.normal(size=(n, h, w, stored)).astype(np.float32)
     g1_ref, g2_ref = onet.corr_backward(f1.astype(np.float64), f2.astype(np.float64), dc[..., :D2].astype(np.float64),
                                         r, 1.0 / c)
-    g1 = torch.zeros((n, h, w, c), dtype=torch.float32, device="cuda")
-    g2 = torch.zeros((n, h, w, c), dtype=torch.float32, device="cuda")
-    dcd = _dev(torch, dc)
+    g1 = _zeros((n, h, w, c), np.float32)
+    g2 = _zeros((n, h, w, c), np.float32)
+    dcd = _dev(dc)
     native.check(lib.ldic_nn_corr_backward(ctypes.byref(spec), d1.data_ptr(), d2.data_ptr(), dcd.data_ptr(), stored, 0,
                                            g1.data_ptr(), c, 0, g2.data_ptr(), c, 0, None))
-    torch.cuda.synchronize()
-    assert _rel(g1.cpu().numpy(), g1_ref) < 1e-5 and _rel(g2.cpu().numpy(), g2_ref) < 1e-5
+    _sync()
+    assert _rel(g1.numpy(), g1_ref) < 1e-5 and _rel(g2.numpy(), g2_ref) < 1e-5
 
     flow = rng.uniform(-2.5, 2.5, size=(n, h, w, 2)).astype(np.float32)
-    fd = _dev(torch, flow)
-    wout = torch.zeros((n, h, w, c), dtype=torch.bfloat16, device="cuda")
+    fd = _dev(flow)
+    wout = _zeros((n, h, w, c), BF16)
     native.check(lib.ldic_nn_warp_forward(n, h, w, c, d2.data_ptr(), fd.data_ptr(), wout.data_ptr(), None))
     gout = rng.normal(size=(n, h, w, c)).astype(np.float32)
-    gw = torch.zeros((n, h, w, c), dtype=torch.float32, device="cuda")
-    native.check(lib.ldic_nn_warp_backward(n, h, w, c, fd.data_ptr(), _dev(torch, gout).data_ptr(), gw.data_ptr(),
+    gw = _zeros((n, h, w, c), np.float32)
+    native.check(lib.ldic_nn_warp_backward(n, h, w, c, fd.data_ptr(), _dev(gout).data_ptr(), gw.data_ptr(),
                                            None))
-    torch.cuda.synchronize()
-    assert _rel(wout.float().cpu().numpy(), onet.warp_forward(f2.astype(np.float64), flow.astype(np.float64))) < 4e-3
-    assert _rel(gw.cpu().numpy(), onet.warp_backward(f2.shape, flow.astype(np.float64), gout.astype(np.float64))) < 1e-5
+    _sync()
+    assert _rel(wout.numpy(), onet.warp_forward(f2.astype(np.float64), flow.astype(np.float64))) < 4e-3
+    assert _rel(gw.numpy(), onet.warp_backward(f2.shape, flow.astype(np.float64), gout.astype(np.float64))) < 1e-5
 
     coarse = rng.normal(size=(n, 5, 5, 2)).astype(np.float32)
-    fine = torch.zeros((n, h, w, 2), dtype=torch.float32, device="cuda")
-    native.check(lib.ldic_nn_upsample_flow(n, 5, 5, h, w, _dev(torch, coarse).data_ptr(), fine.data_ptr(), None, 0, 0,
+    fine = _zeros((n, h, w, 2), np.float32)
+    native.check(lib.ldic_nn_upsample_flow(n, 5, 5, h, w, _dev(coarse).data_ptr(), fine.data_ptr(), None, 0, 0,
                                            None))
-    torch.cuda.synchronize()
-    assert _rel(fine.cpu().numpy(), onet.upsample_flow(coarse.astype(np.float64), h, w)) < 1e-6
+    _sync()
+    assert _rel(fine.numpy(), onet.upsample_flow(coarse.astype(np.float64), h, w)) < 1e-6
 
     truth = rng.normal(size=flow.shape).astype(np.float32)
-    grad = torch.zeros_like(fd)
-    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
-    native.check(lib.ldic_nn_flow_loss(n * h * w, fd.data_ptr(), _dev(torch, truth).data_ptr(), 0.5, grad.data_ptr(),
+    grad = _zeros(fd.shape, np.float32)
+    loss = _zeros(1, np.float32)
+    native.check(lib.ldic_nn_flow_loss(n * h * w, fd.data_ptr(), _dev(truth).data_ptr(), 0.5, grad.data_ptr(),
                                        loss.data_ptr(), None))
     l_ref, g_ref = onet.huber(flow.astype(np.float64), truth.astype(np.float64), 0.5)
-    torch.cuda.synchronize()
+    _sync()
     assert abs(float(loss.item()) - l_ref) < 1e-5 * max(1.0, l_ref)
-    assert _rel(grad.cpu().numpy(), g_ref) < 1e-6
+    assert _rel(grad.numpy(), g_ref) < 1e-6
 
-    # Adam against torch.optim.Adam (an independent implementation) over three steps.
+    # Adam against Algorithm 1 of Kingma & Ba (2015), in float64, over three steps.
     p0 = rng.normal(size=257).astype(np.float32)
     grads = [rng.normal(size=257).astype(np.float32) for _ in range(3)]
-    p = _dev(torch, p0)
-    m = torch.zeros_like(p)
-    v = torch.zeros_like(p)
-    tp = torch.nn.Parameter(torch.from_numpy(p0.copy()))
-    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p = _dev(p0)
+    m = _zeros(p.shape, np.float32)
+    v = _zeros(p.shape, np.float32)
+    want, mr, vr = p0.astype(np.float64), np.zeros(257), np.zeros(257)
     for k, gk in enumerate(grads):
-        native.check(lib.ldic_nn_adam(257, p.data_ptr(), _dev(torch, gk).data_ptr(), m.data_ptr(), v.data_ptr(), 1e-2,
+        native.check(lib.ldic_nn_adam(257, p.data_ptr(), _dev(gk).data_ptr(), m.data_ptr(), v.data_ptr(), 1e-2,
                                       0.9, 0.999, 1e-8, k + 1, None, None))
-        tp.grad = torch.from_numpy(gk.copy())
-        opt.step()
-    torch.cuda.synchronize()
-    np.testing.assert_allclose(p.cpu().numpy(), tp.detach().numpy(), rtol=1e-5, atol=1e-6)
+        g64 = gk.astype(np.float64)
+        mr = 0.9 * mr + 0.1 * g64
+        vr = 0.999 * vr + 0.001 * g64 * g64
+        mh, vh = mr / (1 - 0.9 ** (k + 1)), vr / (1 - 0.999 ** (k + 1))
+        want = want - 1e-2 * mh / (np.sqrt(vh) + 1e-8)
+    _sync()
+    np.testing.assert_allclose(p.numpy(), want, rtol=1e-5, atol=1e-6)
 
 
 def test_network_gradients_match_the_oracle(env):
-    torch, native, lib = env
+    native, lib = env
     from live_dic import seednet
 
     _say("network gradient check, small architecture, 2 pairs of 48x40")
@@ -211,20 +226,20 @@ def test_network_gradients_match_the_oracle(env):
     n, h, w = 2, 48, 40
     ref = rng.integers(0, 256, size=(n, h, w), dtype=np.uint8)
     tgt = np.roll(ref, (1, 2), (1, 2))
-    rd, td = _dev(torch, ref), _dev(torch, tgt)
+    rd, td = _dev(ref), _dev(tgt)
     flows = net.forward(rd.data_ptr(), td.data_ptr(), n=n, h=h, w=w, training=True)
-    truths = [torch.from_numpy(rng.normal(scale=0.5, size=tuple(f.shape)).astype(np.float32)).cuda() for f in flows]
+    truths = [rng.normal(scale=0.5, size=tuple(f.shape)).astype(np.float32) for f in flows]
     weights = (0.6, 0.3)
     net.backward(truths, weights)
-    torch.cuda.synchronize()
+    _sync()
     params = net.oracle_parameters()
     x = lambda a: ((a.astype(np.float64) - 128.0) / 64.0)[..., None]
-    loss, grads, flow = onet.forward_backward(params, arch, x(ref), x(tgt), [t.cpu().numpy().astype(np.float64)
+    loss, grads, flow = onet.forward_backward(params, arch, x(ref), x(tgt), [t.astype(np.float64)
                                                                             for t in truths], weights)
     got = net.gradients()
     _say("loss gpu/oracle", float(net.loss.item()), loss)
     assert abs(float(net.loss.item()) - loss) < 0.02 * loss
-    assert _rel(flows[-1].cpu().numpy(), flow) < 0.03
+    assert _rel(flows[-1].numpy(), flow) < 0.03
     for name, (dw, db) in grads.items():
         gw, gb = got[name]
         cos = float(np.sum(gw * dw) / (np.linalg.norm(gw) * np.linalg.norm(dw) + 1e-30))
