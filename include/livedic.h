@@ -425,8 +425,10 @@ typedef struct ldic_nn_conv {
 int ldic_nn_conv_forward(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
                          int32_t leaky, uint16_t* y, float* y_f32, void* stream);
 /* The same with the GEMM columns split over `splits` workgroup slices (for
- * layers with few output pixels): partial sums go to `workspace` (batch *
- * out_height * out_width * out_channels floats), then bias and activation. */
+ * layers with few output pixels): each slice stores its partial sums in its own
+ * part of `workspace` (splits * batch * out_height * out_width * out_channels
+ * floats), then one pass adds the parts in order, with bias and activation, so
+ * the result is the same on every run. */
 int ldic_nn_conv_forward_split(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w, const float* bias,
                                int32_t leaky, uint16_t* y, float* y_f32, float* workspace, int32_t splits,
                                void* stream);
@@ -434,9 +436,9 @@ int ldic_nn_conv_forward_split(const ldic_nn_conv* conv, const uint16_t* x, cons
  * memory (code under HIP graph capture, tight serving loops): the split-K
  * forward above with its partial sums in a library-owned static scratch of
  * 1,048,576 floats (4 MiB) per device instead of a workspace argument; it calls
- * no allocator.  batch * out_height * out_width * out_channels must not exceed
- * that (LDIC_ERR_ARGUMENT otherwise).  All calls on one device share its
- * scratch, zeroed by a kernel at the start of each call: calls that overlap on
+ * no allocator.  splits * batch * out_height * out_width * out_channels must not
+ * exceed that (LDIC_ERR_ARGUMENT otherwise).  All calls on one device share its
+ * scratch, which every call overwrites before reading: calls that overlap on
  * one device race. */
 int ldic_nn_conv_forward_split_static(const ldic_nn_conv* conv, const uint16_t* x, const uint16_t* w,
                                       const float* bias, int32_t leaky, uint16_t* y, float* y_f32, int32_t splits,

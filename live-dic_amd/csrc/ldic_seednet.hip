@@ -5,15 +5,15 @@
 // on these kernels.  Activations and weights are bfloat16 (NHWC, channel
 // stride and offset so that concatenations are channel slices of one buffer);
 // every product accumulates in float32 on the matrix cores
-// (v_mfma_f32_16x16x16_bf16); gradients of activations and weights are float32.
+// (v_mfma_f32_16x16x32_bf16); gradients of activations and weights are float32.
 //
 // 3x3 convolutions (padding 1, stride 1 or 2) are implicit GEMMs:
 //   forward   Y[m][co]  = sum_k  im2col(X)[m][k]  W[co][k]        k = tap*Cin + ci
 //   data grad dX[m][ci] = sum_k' col(dY)[m][k']   WT[ci][k']      k' = tap*Cout + co
 //   weight    dW[co][k] = sum_m  dY[m][co]        im2col(X)[m][k]
 // with the operands gathered straight from HBM/L2 into the MFMA registers.
-// MFMA 16x16x16 bf16 layout (wave64): lane l holds A[l%16][4(l/16)+i],
-// B[4(l/16)+i][l%16] and D[4(l/16)+i][l%16], i = 0..3.
+// MFMA 16x16x32 bf16 layout (wave64): lane l holds A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16], i = 0..7, and D[4(l/16)+j][l%16], j = 0..3.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -27,6 +27,8 @@ namespace ldic {
 namespace {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -38,8 +40,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+__device__ __forceinline__ s16x8 join8(s16x4 lo, s16x4 hi) {
+    return s16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
 constexpr float kLeaky = 0.1f;
@@ -85,15 +92,16 @@ __device__ __forceinline__ void pixel_of(int m, int Ho, int Wo, int& n, int& oy,
 // Forward: a workgroup owns 64 output pixels (16 per wave) and up to 128
 // output channels.  The weights are staged through LDS in chunks of kChunk
 // GEMM columns, shared by the four waves; each wave gathers its own im2col
-// rows straight into the MFMA A registers (all four k-steps of a chunk are
-// issued before the first MFMA that needs them).
+// rows straight into the MFMA A registers (both 32-column k-steps of a chunk
+// are issued before the first MFMA that needs them).
 constexpr int kChunk = 64;
 constexpr int kChunkPitch = kChunk + 4;   // bf16 elements per LDS row (breaks bank aliasing)
 
 // PARTIAL: blockIdx.z selects a range of kChunk-aligned GEMM columns (split-K
-// for layers with few output pixels); the partial products are added into the
-// float32 workspace Yf (M x out_channels, dense) and k_conv_epilogue applies
-// bias and activation.
+// for layers with few output pixels); range z stores its partial products in
+// slice z of the float32 workspace Yf (slices of M x out_channels, dense), and
+// k_conv_epilogue adds the slices in order, then bias and activation, so the
+// result does not depend on which range finishes first.
 template <bool VEC, int NT, bool PARTIAL>
 __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t* X, const uint16_t* Wt,
                                                   const float* bias, int leaky, uint16_t* Y, float* Yf,
@@ -127,20 +135,30 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
         }
         __syncthreads();
         if (m0 < M) {
-            s16x4 a[kChunk / 16];
+            // kc is a multiple of 16 and a lane's 8 columns start at a multiple of 8,
+            // so they lie either all inside the chunk or all outside it (zeros).
+            s16x8 a[kChunk / 32];
 #pragma unroll
-            for (int s = 0; s < kChunk / 16; ++s)
-                a[s] = s * 16 < kc ? im2col4<VEC>(g, X, n, oy, ox, k0 + 16 * s + 4 * quad, mvalid) : s16x4{0, 0, 0, 0};
+            for (int s = 0; s < kChunk / 32; ++s) {
+                const int kq = 32 * s + 8 * quad;
+                a[s] = kq < kc ? join8(im2col4<VEC>(g, X, n, oy, ox, k0 + kq, mvalid),
+                                       im2col4<VEC>(g, X, n, oy, ox, k0 + kq + 4, mvalid))
+                               : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
 #pragma unroll 1
-            for (int s = 0; s < kChunk / 16; ++s) {
-                if (s * 16 >= kc) break;
+            for (int s = 0; s < kChunk / 32; ++s) {
+                if (s * 32 >= kc) break;
+                const int kq = 32 * s + 8 * quad;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     if (t < ntiles) {
                         const int r = 16 * t + row;
-                        s16x4 b = {0, 0, 0, 0};
-                        if (r < rows) b = *reinterpret_cast<const s16x4*>(Bs + r * kChunkPitch + 16 * s + 4 * quad);
-                        acc[t] = mfma16(a[s], b, acc[t]);
+                        s16x8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+                        if (r < rows && kq < kc) {
+                            const uint16_t* src = Bs + r * kChunkPitch + kq;
+                            b = join8(*reinterpret_cast<const s16x4*>(src), *reinterpret_cast<const s16x4*>(src + 4));
+                        }
+                        acc[t] = mfma32(a[s], b, acc[t]);
                     }
                 }
             }
@@ -153,10 +171,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
         const int co = co_base + 16 * t + row;
         if (co >= g.out_channels) continue;
         if (PARTIAL) {
+            float* slice = Yf + (size_t)blockIdx.z * M * g.out_channels;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int mm = m0 + 4 * quad + i;
-                if (mm < M) atomicAdd(Yf + (size_t)mm * g.out_channels + co, acc[t][i]);
+                if (mm < M) slice[(size_t)mm * g.out_channels + co] = acc[t][i];
             }
             continue;
         }
@@ -174,19 +193,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ldic_nn_conv g, const uint16_t
     }
 }
 
-__global__ void k_fill_zero(size_t count, float* p) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = 0.0f;
-}
-
-__global__ void k_conv_epilogue(ldic_nn_conv g, const float* work, const float* bias, int leaky, uint16_t* Y,
-                                float* Yf) {
+// Sum of the split-K slices in slice order, then bias and activation.
+__global__ void k_conv_epilogue(ldic_nn_conv g, const float* work, int slices, const float* bias, int leaky,
+                                uint16_t* Y, float* Yf) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t M = (size_t)g.batch * g.out_height * g.out_width;
-    if (i >= M * g.out_channels) return;
+    const size_t count = M * g.out_channels;
+    if (i >= count) return;
     const size_t m = i / g.out_channels;
     const int co = (int)(i - m * g.out_channels);
-    float v = work[i] + (bias ? bias[co] : 0.0f);
+    float sum = work[i];
+    for (int z = 1; z < slices; ++z) sum += work[(size_t)z * count + i];
+    float v = sum + (bias ? bias[co] : 0.0f);
     if (leaky) v = v > 0.0f ? v : kLeaky * v;
     const size_t o = m * g.out_stride + g.out_offset + co;
     if (Yf) Yf[o] = v;
@@ -291,16 +309,23 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(ldic_nn_conv g, const float*
     f32x4 acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < KPT; k0 += 16) {
-        const int k = k0 + 4 * quad;
-        const s16x4 a = dcol4<VEC>(g, dY, n, iy, ix, k, mvalid);
+    // 32 columns per step; KPT is a multiple of 16, so a lane's 8 columns lie
+    // either all inside the row or all beyond it (zeros).
+    for (int k0 = 0; k0 < KPT; k0 += 32) {
+        const int k = k0 + 8 * quad;
+        const bool kvalid = k < KPT;
+        const s16x8 a = kvalid ? join8(dcol4<VEC>(g, dY, n, iy, ix, k, mvalid), dcol4<VEC>(g, dY, n, iy, ix, k + 4, mvalid))
+                               : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             if (t < ntiles) {
                 const int ci = ci_base + 16 * t + row;
-                s16x4 b = {0, 0, 0, 0};
-                if (ci < g.in_channels) b = *reinterpret_cast<const s16x4*>(WT + (size_t)ci * KPT + k);
-                acc[t] = mfma16(a, b, acc[t]);
+                s16x8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (ci < g.in_channels && kvalid) {
+                    const uint16_t* src = WT + (size_t)ci * KPT + k;
+                    b = join8(*reinterpret_cast<const s16x4*>(src), *reinterpret_cast<const s16x4*>(src + 4));
+                }
+                acc[t] = mfma32(a, b, acc[t]);
             }
         }
     }
@@ -336,11 +361,12 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(ldic_nn_conv g, const uint16
     const bool covalid = co < g.out_channels;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const int mend = min(M, mstart + kWgradPixels);
-    for (int mb = mstart; mb < mend; mb += 16) {
-        s16x4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+    // 32 pixels per step, 8 per lane.
+    for (int mb = mstart; mb < mend; mb += 32) {
+        s16x8 a = {0, 0, 0, 0, 0, 0, 0, 0}, b = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int mm = mb + 4 * quad + i;
+        for (int i = 0; i < 8; ++i) {
+            const int mm = mb + 8 * quad + i;
             if (mm < mend) {
                 if (covalid) a[i] = (short)f2bf(dY[(size_t)mm * g.out_stride + g.out_offset + co]);
                 int n, oy, ox;
@@ -348,7 +374,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(ldic_nn_conv g, const uint16
                 b[i] = (short)im2col_at(g, X, n, oy, ox, k0 + row);
             }
         }
-        acc = mfma16(a, b, acc);
+        acc = mfma32(a, b, acc);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -741,11 +767,11 @@ int ldic_nn_conv_forward_split(const ldic_nn_conv* g, const uint16_t* x, const u
         return status(hipGetLastError());
     }
     const size_t count = (size_t)M * g->out_channels;
-    k_fill_zero<<<blocks(count), 256, 0, st>>>(count, workspace);
     const int kps = (chunks + splits - 1) / splits * kChunk;
-    dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile, (KP + kps - 1) / kps);
+    const int slices = (KP + kps - 1) / kps;   // <= splits slices of the workspace are written
+    dim3 grid(blocks(M, 64), (g->out_channels + tile - 1) / tile, slices);
     conv_fwd_launch<true>(tile, vec, grid, st, *g, x, w, nullptr, 0, nullptr, workspace, kps);
-    k_conv_epilogue<<<blocks(count), 256, 0, st>>>(*g, workspace, bias, leaky, y, y_f32);
+    k_conv_epilogue<<<blocks(count), 256, 0, st>>>(*g, workspace, slices, bias, leaky, y, y_f32);
     return status(hipGetLastError());
 }
 
@@ -754,10 +780,11 @@ int ldic_nn_conv_forward_split_static(const ldic_nn_conv* g, const uint16_t* x, 
     if (check_conv(g) != LDIC_OK || !x || !w || (!y && !y_f32) || splits < 1)
         return report_error(LDIC_ERR_ARGUMENT, "invalid convolution arguments");
     const size_t count = (size_t)g->batch * g->out_height * g->out_width * g->out_channels;
-    if (splits > 1 && count > kStaticSplitFloats)
+    if (splits > 1 && count * (size_t)splits > kStaticSplitFloats)
         return report_error(LDIC_ERR_ARGUMENT,
-                            "the split-K partial sums (batch * out_height * out_width * out_channels floats) exceed "
-                            "the static scratch of 1048576 floats; use ldic_nn_conv_forward_split with a workspace");
+                            "the split-K partial sums (splits * batch * out_height * out_width * out_channels floats) "
+                            "exceed the static scratch of 1048576 floats; use ldic_nn_conv_forward_split with a "
+                            "workspace");
     float* workspace = nullptr;
     if (splits > 1) {
         const int rc = static_scratch(&workspace);

@@ -253,12 +253,13 @@ class SeedNet:
                     b["dwarped"] = _zeros((n, hh, ww, c), f32)
             self.st.append(b)
         self.loss = _zeros(1, f32)
-        # Split-K workspace: the largest output of any convolution at this shape.
+        # Split-K workspace: one slice per split (at most 8) of the largest output of any
+        # convolution at this shape.
         largest = max(2 * n * hh_ * ww_ * c for hh_, ww_, c in self.enc_shapes)
         for si, stage in enumerate(a.stages):
             hh, ww, _ = self.enc_shapes[stage.level]
             largest = max(largest, n * hh * ww * max(co for co, _ in stage.head))
-        self.workspace = _zeros(largest, f32)
+        self.workspace = _zeros(8 * largest, f32)
         _native.check(self.lib.ldic_device_synchronize())
         self._shape = key
 

@@ -105,7 +105,7 @@ def test_conv_forward_and_gradients(env, n, h, w, cin, cout, stride, leaky, stor
     y_ref = onet.conv_forward(x, wt.astype(np.float64), b.astype(np.float64), stride, leaky)
     assert _rel(y, y_ref) < (1e-6 if f32_out else 4e-3)
     # Split-K: the GEMM columns in three slices through a float32 workspace.
-    work = _zeros(n * ho * wo * cout, np.float32)
+    work = _zeros(3 * n * ho * wo * cout, np.float32)
     y16.zero()
     y32.zero()
     native.check(lib.ldic_nn_conv_forward_split(ctypes.byref(g), xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
@@ -114,6 +114,11 @@ def test_conv_forward_and_gradients(env, n, h, w, cin, cout, stride, leaky, stor
     _sync()
     ys = (y32 if f32_out else y16).numpy()
     assert _rel(ys, y_ref) < (1e-6 if f32_out else 4e-3)
+    # The slices are added in a fixed order: a second run gives the same bits.
+    native.check(lib.ldic_nn_conv_forward_split(ctypes.byref(g), xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
+                                                1 if leaky else 0, None if f32_out else y16.data_ptr(),
+                                                y32.data_ptr() if f32_out else None, work.data_ptr(), 3, None))
+    np.testing.assert_array_equal((y32 if f32_out else y16).numpy(), ys)
 
     dy = rng.normal(size=(n, ho, wo, cout)).astype(np.float32)
     dy_b = bf16(dy).astype(np.float64)          # the kernels feed dy to the matrix cores as bfloat16
