@@ -40,6 +40,13 @@ HBM_PEAK_GBPS = 8000.0
 # profiles/k_solve_affine_4k_s31_d8_pmc.md.
 MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_898 + 4_191) * 1024
 MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.317e7 / (4.317e7 + 1.661e6)
+# VALU wave-instructions of the same launch (rocprofv3 SQ_INSTS_VALU, same profile), and the
+# issue limit it is set against: one wave64 FP32 operation per 4 cycles per SIMD, which is the
+# published non-packed FP32 (= FP64) vector peak of 78.6 TFLOP/s over 256 CUs x 4 SIMDs x
+# 64 lanes x 2 FLOP per FMA, i.e. 6.0e8 wave-instructions/s per SIMD at 2.4 GHz.
+MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS = 4.3985e8
+SIMDS = 256 * 4
+VALU_ISSUE_PEAK_PER_SIMD = 78.6e12 / (SIMDS * 64 * 2)
 SUBSET, STRIDE = 31, 8
 WIDTH, HEIGHT = 3840, 2160
 FRAMES = 60
@@ -358,6 +365,22 @@ PUBLISHED_REFERENCE_CPU = {
 SMALL_SUBSET, SMALL_STRIDE = 21, 8
 
 
+def valu_limiter(avg_ms: float, alone_ms: float | None) -> dict:
+    """VALU issue rate of the fine solve per SIMD against the FP32 issue limit."""
+    rate = MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS / SIMDS / (avg_ms * 1e-3)
+    out = {
+        "bound": "valu",
+        "unit": "VALU wave-instructions/s per SIMD",
+        "achieved": rate,
+        "peak": VALU_ISSUE_PEAK_PER_SIMD,
+        "frac": rate / VALU_ISSUE_PEAK_PER_SIMD,
+        "valu_instructions_per_launch": MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS,
+    }
+    if alone_ms:
+        out["frac_alone"] = MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS / SIMDS / (alone_ms * 1e-3) / VALU_ISSUE_PEAK_PER_SIMD
+    return out
+
+
 def small_pair_line(frames: int = 400, warmup: int = 20) -> dict:
     """The 512x512 pair (subset 21, step 8, 20 iterations, tolerance 1e-4, one level): GPU frames/s
     at the NumPy boundary through register_stream(pipeline_depth=2), accuracy against the analytic
@@ -614,6 +637,10 @@ def main() -> int:
             # the same kernel on one stream, measured after the timed region:
             "kernel_avg_ms_alone": alone_ms,
             "frac_alone": traffic_bytes / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if alone_ms else None,
+            # What actually limits the kernel: VALU instruction issue.  HBM moves 0.12x the
+            # algorithmic bytes (the rest is served by L2 and LDS), while the SIMDs issue VALU
+            # instructions at the FP32 issue limit.
+            "limiter": valu_limiter(avg_solve_ms, alone_ms),
         },
         "cpu_baseline": None,
         "numpy_boundary": boundary,
