@@ -275,6 +275,7 @@ def batched_heterogeneous(frames: int = 20) -> dict:
         "latency_ms_p99": float(np.percentile(lat, 99)),
         "accepted_fraction_min": min(accepted),
         "accepted_fraction_inside_frame_min": min(accepted_inside),
+        "accepted_fraction_inside_frame_per_pair": [round(a, 4) for a in accepted_inside],
         "median_error_px_max": max(medians),
         "persistent_launches": table,
     }

@@ -1783,6 +1783,13 @@ struct ldic_batch {
     int max_levels = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Branches of a frame: pair p's own kernels run on aux[p % kBatchBranches], and the
+    // solver groups of a level on aux[g % kBatchBranches]; fork_ev / join_ev order them
+    // against the main stream (in a captured frame they become the graph's edges).
+    static constexpr int kBatchBranches = 4;
+    hipStream_t aux[kBatchBranches] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kBatchBranches] = {};
     std::vector<std::vector<Group>> steps;  // steps[j]: groups at finest-aligned level j
     Arena arena;
     hipGraph_t graph = nullptr;
@@ -1793,12 +1800,11 @@ struct ldic_batch {
 
 namespace {
 
-int batch_pre_solve(ldic_batch* b, int p, int k, const float** seed, const uint32_t** active) {
+int batch_pre_solve(ldic_batch* b, int p, int k, hipStream_t st, const float** seed, const uint32_t** active) {
     ldic_engine* e = b->engines[p];
     Slot* s = e->slots[0];
     SlotLevel& sl = s->lv[k];
     const Level& lev = e->lv[k];
-    hipStream_t st = b->stream;
     *seed = nullptr;
     *active = nullptr;
     if (k < e->levels - 1) {
@@ -1835,31 +1841,59 @@ int batch_pre_solve(ldic_batch* b, int p, int k, const float** seed, const uint3
 }
 
 // One frame of the batch, queued on its stream (also what the graph captures).
+// One frame, as three kinds of phases separated by barriers on the main stream:
+// per-pair work (copy and reductions, then per level the seed transfer or search
+// and the prefilter) and per-pair outlier passes, each pair on its own branch
+// stream; and the level's solver groups, each group on a branch stream.  Pairs
+// and groups are independent, so the branches overlap the small kernels of
+// different pairs and the tails of the groups' persistent launches.
 int batch_enqueue(ldic_batch* b) {
     hipStream_t st = b->stream;
+    constexpr int nb = ldic_batch::kBatchBranches;
+    auto fork = [&]() -> int {
+        HIP_CHECK(hipEventRecord(b->fork_ev, st));
+        for (int i = 0; i < nb; ++i) HIP_CHECK(hipStreamWaitEvent(b->aux[i], b->fork_ev, 0));
+        return LDIC_OK;
+    };
+    auto join = [&]() -> int {
+        for (int i = 0; i < nb; ++i) {
+            HIP_CHECK(hipEventRecord(b->join_ev[i], b->aux[i]));
+            HIP_CHECK(hipStreamWaitEvent(st, b->join_ev[i], 0));
+        }
+        return LDIC_OK;
+    };
+    auto branch = [&](size_t i) { return b->aux[i % nb]; };
+    CHECK(fork());
     for (size_t p = 0; p < b->engines.size(); ++p) {
         ldic_engine* e = b->engines[p];
         const Level& L0 = e->lv[0];
         Slot* s = e->slots[0];
+        hipStream_t ps = branch(p);
         HIP_CHECK(hipMemcpy2DAsync(s->lv[0].target, L0.width, b->targets[p], L0.width, L0.width, L0.height,
-                                   hipMemcpyDeviceToDevice, st));
+                                   hipMemcpyDeviceToDevice, ps));
         for (int k = 1; k < e->levels; ++k)
-            HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, st));
+            HIP_CHECK(launch_reduce(s->lv[k - 1].target, e->lv[k - 1].width, e->lv[k - 1].height, s->lv[k].target, ps));
     }
     for (int j = b->max_levels - 1; j >= 0; --j) {
+        // A pair's kernels stay on one branch, in order; it is already forked here.
         for (size_t p = 0; p < b->engines.size(); ++p) {
             if (b->engines[p]->levels <= j) continue;
             const float* seed;
             const uint32_t* active;
-            CHECK(batch_pre_solve(b, (int)p, j, &seed, &active));
+            CHECK(batch_pre_solve(b, (int)p, j, branch(p), &seed, &active));
         }
-        for (const ldic_batch::Group& g : b->steps[j]) HIP_CHECK(launch_solve_batched(g.launch, st));
+        CHECK(join());
+        CHECK(fork());
+        for (size_t g = 0; g < b->steps[j].size(); ++g) HIP_CHECK(launch_solve_batched(b->steps[j][g].launch, branch(g)));
+        CHECK(join());
+        CHECK(fork());
         for (size_t p = 0; p < b->engines.size(); ++p) {
             ldic_engine* e = b->engines[p];
             if (e->levels <= j) continue;
-            CHECK(launch_median_level(e, j, e->slots[0]->lv[j], e->opt.outlier_policy == kMedian && j > 0, st));
+            CHECK(launch_median_level(e, j, e->slots[0]->lv[j], e->opt.outlier_policy == kMedian && j > 0, branch(p)));
         }
     }
+    CHECK(join());
     return LDIC_OK;
 }
 
@@ -2026,6 +2060,11 @@ int ldic_batch_create(ldic_engine* const* engines, int32_t count, const uint8_t*
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (he == hipSuccess) he = hipEventCreate(&b->ev0);
     if (he == hipSuccess) he = hipEventCreate(&b->ev1);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+    for (int i = 0; i < ldic_batch::kBatchBranches && he == hipSuccess; ++i) {
+        he = hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
+    }
     if (he != hipSuccess) return fail(set_error(LDIC_ERR_DEVICE, "batch set-up failed: %s", hipGetErrorString(he)));
     for (ldic_engine* e : b->engines)
         if (e->slots[0]->pending) return fail(set_error(LDIC_ERR_STATE, "an engine's slot 0 holds a pending frame"));
@@ -2115,8 +2154,15 @@ void ldic_batch_destroy(ldic_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_release_graph(b);
     b->arena.release();
+    for (int i = 0; i < ldic_batch::kBatchBranches; ++i)
+        if (b->aux[i]) (void)hipStreamSynchronize(b->aux[i]);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    for (int i = 0; i < ldic_batch::kBatchBranches; ++i) {
+        if (b->join_ev[i]) (void)hipEventDestroy(b->join_ev[i]);
+        if (b->aux[i]) (void)hipStreamDestroy(b->aux[i]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->holds)
         for (ldic_engine* e : b->engines) --e->batch_refs;
