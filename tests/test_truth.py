@@ -149,13 +149,20 @@ def _field(x, y, p):
     return u, v, ux, uy, vx, vy
 
 
-@pytest.mark.parametrize("model,size,stride,params", [
-    ("translation", 21, 15, (1.37, 0, 0, -0.81, 0, 0, 0, 0, 0, 0, 0, 0)),
-    ("affine", 21, 15, (0.93, 0.012, -0.008, -1.21, 0.006, -0.015, 0, 0, 0, 0, 0, 0)),
-    ("affine", 31, 20, (-0.47, -0.02, 0.011, 0.66, 0.009, 0.018, 0, 0, 0, 0, 0, 0)),
-    ("quadratic", 25, 15, (0.71, 0.01, -0.006, -0.55, 0.004, 0.012, 2e-4, -1.5e-4, 1e-4, -1e-4, 1.8e-4, -2.2e-4)),
+_QUAD = (0.71, 0.01, -0.006, -0.55, 0.004, 0.012, 2e-4, -1.5e-4, 1e-4, -1e-4, 1.8e-4, -2.2e-4)
+
+
+@pytest.mark.parametrize("model,size,stride,params,weighting", [
+    ("translation", 21, 15, (1.37, 0, 0, -0.81, 0, 0, 0, 0, 0, 0, 0, 0), "uniform"),
+    ("affine", 21, 15, (0.93, 0.012, -0.008, -1.21, 0.006, -0.015, 0, 0, 0, 0, 0, 0), "uniform"),
+    ("affine", 31, 20, (-0.47, -0.02, 0.011, 0.66, 0.009, 0.018, 0, 0, 0, 0, 0, 0), "uniform"),
+    ("quadratic", 25, 15, _QUAD, "uniform"),
+    # Gaussian weights change the sums, not the solution of an exact model: this pins the
+    # weighted path (weights in the preparation and in every iteration's sums).
+    ("affine", 21, 15, (0.93, 0.012, -0.008, -1.21, 0.006, -0.015, 0, 0, 0, 0, 0, 0), "gaussian"),
+    ("quadratic", 25, 15, _QUAD, "gaussian"),
 ])
-def test_exact_spline_warp_is_recovered(model, size, stride, params):
+def test_exact_spline_warp_is_recovered(model, size, stride, params, weighting):
     """The target is the cubic B-spline surface T of random coefficients c (so the solver's target
     plane is c itself, with no prefilter error); the reference is R(X) = T(X + u(X)) with its
     chain-rule gradient, both evaluated in float64.  IC-GN from zero must return u's local expansion
@@ -169,10 +176,11 @@ def test_exact_spline_warp_is_recovered(model, size, stride, params):
     gx = tx * (1 + ux) + ty * vx
     gy = tx * uy + ty * (1 + vy)
     reference, inverse, metadata = prep.subset_arrays(r, gx, gy, motion_model=model, subset_size=size,
-                                                      subset_stride=stride)
+                                                      subset_stride=stride, spatial_weighting=weighting,
+                                                      gaussian_sigma=0.25 * size)
     gw, gh = prep.grid_shape(_W, _H, size, stride)
-    cfg = oracle.SolveCfg(_W, _H, gw, gh, size, stride, oracle.MODELS[model], 0, 0, 0, 0.25 * size, 30, 2e-5, 0.8,
-                          16, 0, 0, 4)
+    cfg = oracle.SolveCfg(_W, _H, gw, gh, size, stride, oracle.MODELS[model], 0, 1 if weighting == "gaussian" else 0, 0,
+                          0.25 * size, 30, 2e-5, 0.8, 16, 0, 0, 4)
     state = oracle.solve(cfg, reference.reshape(-1, 4), metadata, inverse, coeff, None).reshape(gh, gw, -1)
     P = 12 if model == "quadratic" else 6
     ok = state[..., P] == 1
