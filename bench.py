@@ -142,7 +142,7 @@ def _accuracy(states, targets) -> dict:
             "within_0.01px": float(np.mean(e <= 0.01)), "over_1px": int(np.sum(e > 1.0))}
 
 
-def learned_vs_zncc(reps: int = 3) -> dict:
+def learned_vs_zncc(reps: int = 10) -> dict:
     """Throughput and accuracy of the two seeding paths on the same fixed reference and targets."""
     from live_dic import ICGNWGPU, ICGNWGPUFullPyramid, _native, seednet
 
