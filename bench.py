@@ -634,6 +634,10 @@ def main() -> int:
             "kernel_avg_ms": avg_solve_ms,
             "kernel_avg_ms_resident_run": resident_solve_ms,
             "algorithmic_bytes_per_launch": traffic_bytes,
+            # Each array the launch touches, read or written once: the float4 reference plane and
+            # the coefficient plane (W x H x 20 B), metadata and inverses (4 (M + P^2) B per
+            # subset) and the seed read plus state write (64 B per subset).
+            "compulsory_bytes_per_launch": WIDTH * HEIGHT * 20 + n_fine * (4 * (16 + 36) + 64),
             # The timed region runs two frames concurrently, so each launch shares the GPU;
             # the same kernel on one stream, measured after the timed region:
             "kernel_avg_ms_alone": alone_ms,

@@ -224,16 +224,21 @@ __device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12], co
             }
         }
         if (ps < 1e-8f) return false;
+        // The pivot row is wave-uniform; the exchange (rarely needed: the lifted operator of
+        // an increment is close to the identity) is skipped by a scalar branch.
+        pr = __builtin_amdgcn_readfirstlane(pr);
+        if (pr != c) {
 #pragma unroll
-        for (int r = c + 1; r < 6; ++r) {
-            const bool sw = r == pr;
-            float x = col[c], y = col[r];
-            col[c] = sw ? y : x;
-            col[r] = sw ? x : y;
-            x = pc[c];
-            y = pc[r];
-            pc[c] = sw ? y : x;
-            pc[r] = sw ? x : y;
+            for (int r = c + 1; r < 6; ++r) {
+                const bool sw = r == pr;
+                float x = col[c], y = col[r];
+                col[c] = sw ? y : x;
+                col[r] = sw ? x : y;
+                x = pc[c];
+                y = pc[r];
+                pc[c] = sw ? y : x;
+                pc[r] = sw ? x : y;
+            }
         }
         col[c] /= pc[c];
 #pragma unroll
