@@ -101,7 +101,8 @@ int ldic_device_name(struct ldic_engine* engine, char* buffer, int32_t capacity)
  * the host, wgpu_backend.py:318-334, 589-823, and set_reference
  * full_pyramid_backend.py:1135-1368).  configs and rois hold one entry per
  * level, finest first.  out_reference_valid (fine grid, nullable) receives the
- * reference validity before ROI exclusion. */
+ * reference validity before ROI exclusion.  Images are at least 16x16 and have
+ * fewer than 2^28 pixels. */
 int ldic_set_reference(struct ldic_engine* engine, const uint8_t* image, int32_t width, int32_t height,
                        int64_t row_stride, const ldic_level_config* configs, const ldic_level_roi* rois,
                        uint8_t* out_reference_valid);

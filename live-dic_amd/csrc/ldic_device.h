@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ldic {
 
 constexpr int kWave = 64;
@@ -125,5 +127,27 @@ __device__ __forceinline__ int mirror_clamp(int c, int size) {
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Loads from device memory by a 32-bit byte offset from a wave-uniform base,
+// in the global address space: a pointer read from a kernel-argument array is
+// generic to the compiler, and a flat load also counts against the LDS wait
+// counter, so every wait for an LDS read would wait for it too.  The base
+// stays in scalar registers and the offset is the only per-lane address.
+typedef __attribute__((address_space(1))) const char global_byte;
+
+template <typename T>
+__device__ __forceinline__ T load_global(const void* base, uint32_t byte_offset) {
+    const global_byte* g = (const global_byte*)base + byte_offset;
+    if constexpr (std::is_same<T, float3>::value) {
+        // float3 is a class type: its bits are read through the builtin
+        // three-element vector, one dwordx3 load (the first three floats of a
+        // float4 element, whose fourth is padding).
+        typedef float v3f __attribute__((ext_vector_type(3)));
+        const v3f v = *(__attribute__((address_space(1))) const v3f*)g;
+        return make_float3(v.x, v.y, v.z);
+    } else {
+        return *(__attribute__((address_space(1))) const T*)g;
+    }
+}
 
 }  // namespace ldic

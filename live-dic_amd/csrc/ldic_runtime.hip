@@ -1015,6 +1015,10 @@ int set_reference_impl(ldic_engine* e, const uint8_t* image, int width, int heig
                        const ldic_level_config* configs, const ldic_level_roi* rois, uint8_t* out_valid) {
     if (!image || !configs) return set_error(LDIC_ERR_ARGUMENT, "image and configs are required");
     if (width < 16 || height < 16) return set_error(LDIC_ERR_ARGUMENT, "image dimensions must be at least 16");
+    // The solver addresses the 16-byte reference plane and the target
+    // coefficients by 32-bit byte offsets.
+    if ((int64_t)width * height >= (int64_t)1 << 28)
+        return set_error(LDIC_ERR_ARGUMENT, "images must have fewer than 2^28 pixels (%dx%d)", width, height);
     if (row_stride < width) return set_error(LDIC_ERR_ARGUMENT, "row_stride is smaller than width");
     const int S = e->opt.subset_size, D = e->opt.subset_stride;
     // Validate every level before touching the previous reference.

@@ -22,12 +22,15 @@ namespace ldic {
 // invariant to a common scale of the sampled values (the mean, norm, ratio and
 // ZNCC all cancel it), so the samplers leave out the 1/36 of the two axes and
 // the norm threshold is scaled instead (kSampleScale).
+// w[2](t) = w[1](1 - t) (the kernel is even), so w[2] reuses the powers of
+// s = 1 - t that w[0] needs: 9 operations per axis.
 __device__ __forceinline__ void bspline_weights6(float t, float w[4]) {
-    float t2 = t * t, t3 = t2 * t, inv = 1.0f - t;
-    w[0] = inv * inv * inv;
+    const float s = 1.0f - t;
+    const float t2 = t * t, t3 = t2 * t, s2 = s * s, s3 = s2 * s;
+    w[0] = s3;
     w[1] = fmaf(3.0f, t3, fmaf(-6.0f, t2, 4.0f));
+    w[2] = fmaf(3.0f, s3, fmaf(-6.0f, s2, 4.0f));
     w[3] = t3;
-    w[2] = fmaf(3.0f, t + t2 - t3, 1.0f);
 }
 
 template <int INTERP>
@@ -54,6 +57,19 @@ __device__ __forceinline__ void floor_split(float x, int& i, float& f) {
     f = __builtin_amdgcn_fractf(x);
 }
 
+// origin + iy * TP + ix for a tile with compile-time row pitch TP and a
+// wave-uniform origin (the callers fold the staged box's corner and the
+// stencil's -1 offsets into it), so (ix, iy) are image coordinates: one
+// v_mad_u32_u24 and one shift-add (written out because the compiler folds the
+// origin into the product and falls back to a full 32-bit multiply).  Image
+// coordinates are non-negative and below 2^24.
+template <int TP>
+__device__ __forceinline__ const float* tile_tap(const float* origin, int ix, int iy) {
+    int idx;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(idx) : "v"(iy), "n"(TP), "v"(ix));
+    return origin + idx;
+}
+
 // TP > 0: compile-time row pitch, so the 16 taps are immediate offsets from one
 // per-pixel LDS address; TP == 0: the pitch is the staged box width.
 template <int INTERP, int TP>
@@ -77,7 +93,11 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
         bspline_weights6(tx, wx);
         bspline_weights6(ty, wy);
     }
-    const float* b = tile + (__mul24(iy - 1 - t.y0, pitch) + (ix - 1 - t.x0));
+    const float* b;
+    if constexpr (TP > 0)
+        b = tile_tap<TP>(tile - (TP * (t.y0 + 1) + t.x0 + 1), ix, iy);
+    else
+        b = tile + (__mul24(iy - 1 - t.y0, pitch) + (ix - 1 - t.x0));
     float v;
 #pragma unroll
     for (int yy = 0; yy < 4; ++yy) {
@@ -352,27 +372,49 @@ __device__ __forceinline__ float increment_change_quadratic_lanes(const float* d
 template <int INTERP, int TP>
 __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, const Tile& t, int lane) {
     const int pitch = TP > 0 ? TP : t.w;
-    auto load = [&](size_t g) -> float { return INTERP == kBspline ? a.coeff[g] : (float)a.target[g]; };
+    auto load = [&](uint32_t g) -> float {
+        return INTERP == kBspline ? load_global<float>(a.coeff, g * 4u) : (float)load_global<uint8_t>(a.target, g);
+    };
     if (t.w <= 2 * 32) {
         // Boxes up to 64 wide: half-wave h copies rows h, h + 2, ... (then the last
         // row in half-wave 0 when the height is odd); lane c copies columns c and
-        // c + 32, so a row costs two address increments instead of a division and
-        // 64-bit index arithmetic per element.
+        // c + 32.  Lanes past the box width load its last column and skip the
+        // store, so the loads of four row pairs are issued before the first wait
+        // instead of one load and one wait per element.
         const int c = lane & 31, h = lane >> 5;
         const bool lo = c < t.w, hi = c + 32 < t.w;
-        const size_t step = 2 * (size_t)a.width;
-        size_t g = (size_t)(t.y0 + h) * a.width + (t.x0 + c);
+        const uint32_t clo = (uint32_t)(lo ? c : t.w - 1), chi = (uint32_t)(hi ? c + 32 : t.w - 1);
+        const uint32_t step = 2u * (uint32_t)a.width;
+        uint32_t g = (uint32_t)(t.y0 + h) * (uint32_t)a.width + (uint32_t)t.x0;
         float* d = tile + (h * pitch + c);
-#pragma unroll 4
-        for (int k = 0; k < (t.h >> 1); ++k) {
-            if (lo) d[0] = load(g);
-            if (hi) d[32] = load(g + 32);
+        const int pairs = t.h >> 1;
+        int k = 0;
+        for (; k + 4 <= pairs; k += 4) {
+            float v0[4], v1[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v0[u] = load(g + u * step + clo);
+                v1[u] = load(g + u * step + chi);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (lo) d[u * 2 * pitch] = v0[u];
+                if (hi) d[u * 2 * pitch + 32] = v1[u];
+            }
+            g += 4 * step;
+            d += 8 * pitch;
+        }
+        for (; k < pairs; ++k) {
+            const float x0 = load(g + clo), x1 = load(g + chi);
+            if (lo) d[0] = x0;
+            if (hi) d[32] = x1;
             g += step;
             d += 2 * pitch;
         }
         if ((t.h & 1) && h == 0) {
-            if (lo) d[0] = load(g);
-            if (hi) d[32] = load(g + 32);
+            const float x0 = load(g + clo), x1 = load(g + chi);
+            if (lo) d[0] = x0;
+            if (hi) d[32] = x1;
         }
         wave_lds_sync();
         return;
@@ -382,8 +424,8 @@ __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, cons
     for (int i = lane; i < n; i += kWave) {
         int r = (int)(((float)i + 0.5f) * inv_w);
         int c = i - r * t.w;
-        size_t g = (size_t)(t.y0 + r) * a.width + (t.x0 + c);
-        tile[r * pitch + c] = INTERP == kBspline ? a.coeff[g] : (float)a.target[g];
+        const uint32_t g = (uint32_t)(t.y0 + r) * (uint32_t)a.width + (uint32_t)(t.x0 + c);
+        tile[r * pitch + c] = load(g);
     }
     wave_lds_sync();
 }
@@ -585,14 +627,22 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         // so it rounds finely), then the pixel coordinate of the centre is added
         // with one rounding per sample, as in icgn.wgsl:741-743.
         const float ux = 1.0f + q.L[0], vy = 1.0f + q.L[3];
+        // Quadratic: the halves of the squared-term coefficients; the sums over
+        // lx^2 and ly^2 run without the 1/2, which is applied (exactly) after the
+        // reduction.
+        const float hc0 = 0.5f * q.C[0], hc2 = 0.5f * q.C[2], hc3 = 0.5f * q.C[3], hc5 = 0.5f * q.C[5];
         auto accumulate = [&](auto sampler, float lx, float ly, float f, float gxv, float gyv, float iw,
                               float valid) {
             float ex, ey;
+            float xx = 0.0f, xy = 0.0f, yy = 0.0f;
             if (MODEL == kQuadratic) {
-                float dx, dy;
-                displacement<MODEL>(q, lx, ly, dx, dy);
-                ex = lx + dx;
-                ey = ly + dy;
+                // l + d(l) from the three basis products the Jacobian sums use as
+                // well, one rounding per term, the largest terms first.
+                xx = lx * lx;
+                xy = lx * ly;
+                yy = ly * ly;
+                ex = fmaf(hc0, xx, fmaf(q.C[1], xy, fmaf(hc2, yy, fmaf(q.L[1], ly, fmaf(ux, lx, q.T[0])))));
+                ey = fmaf(hc3, xx, fmaf(q.C[4], xy, fmaf(hc5, yy, fmaf(vy, ly, fmaf(q.L[2], lx, q.T[1])))));
             } else if (MODEL == kTranslation) {
                 ex = lx + q.T[0];
                 ey = ly + q.T[1];
@@ -614,13 +664,12 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 sj[4] += gyg * lx;
                 sj[5] += gyg * ly;
                 if (MODEL == kQuadratic) {
-                    float hx2 = 0.5f * lx * lx, xy = lx * ly, hy2 = 0.5f * ly * ly;
-                    sq[0] += gxg * hx2;
+                    sq[0] += gxg * xx;
                     sq[1] += gxg * xy;
-                    sq[2] += gxg * hy2;
-                    sq[3] += gyg * hx2;
+                    sq[2] += gxg * yy;
+                    sq[3] += gyg * xx;
                     sq[4] += gyg * xy;
-                    sq[5] += gyg * hy2;
+                    sq[5] += gyg * yy;
                 }
             }
         };
@@ -666,10 +715,17 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                     }
                     return;
                 }
+                // The column carry stays a branch.  A select-based carry (and a
+                // wave-uniform trip count) keeps its float increments in VGPRs,
+                // and at the 128-VGPR budget of the quadratic kernels the compiler
+                // then waited for each of the 16 LDS tap reads before issuing the
+                // next in the persistent batched kernel (measured: its subset-41
+                // launch 10.7 -> 15.1 ms).  The reference is addressed by a 32-bit
+                // byte offset in the global address space.
                 int col = col0, off = off0;
                 float lx = lx0, ly = ly0;
                 for (int p = lane; p < SS; p += kWave) {
-                    const float4 r = a.reference[off];
+                    const float3 r = load_global<float3>(a.reference, (uint32_t)off * 16u);
                     float iw = WEIGHTED ? expf(-0.5f * (lx * lx + ly * ly) * a.inv_var) : 1.0f;
                     if (partial && !((support[p >> 5] >> (p & 31)) & 1u)) iw = 0.0f;
                     accumulate(sampler, lx, ly, r.x - ref_mean, r.y, r.z, iw, 1.0f);
@@ -700,15 +756,18 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             const int c = lane & 31, h = lane >> 5;
             const bool on = c < S;
             const float lxc = (float)c - he;
+            // Lane-constant parts of the warped position.
             const float axl = fmaf(ux, lxc, q.T[0]);
-            const int bias = __mul24(t.y0 + 1, pitch) + t.x0 + 1;  // tile offset of (ix, iy) = iy * pitch + ix - bias
-            const float4* rp = a.reference + ((size_t)(oy + h) * a.width + ox + (on ? c : 0));
-            const size_t rstep = 2 * (size_t)a.width;
+            const float ayl = fmaf(q.L[2], lxc, q.T[1]);
+            // Tap (ix, iy) sits at iy * pitch + ix - (pitch (y0 + 1) + x0 + 1) in the slice.
+            const float* origin = tile - (__mul24(t.y0 + 1, pitch) + t.x0 + 1);
+            uint32_t boff = (uint32_t)((oy + h) * a.width + ox + (on ? c : 0)) * 16u;
+            const uint32_t rstep = (uint32_t)(2 * a.width) * 16u;
             float ly = (float)h - he;
             auto pixel = [&]() {
-                const float4 rv = *rp;
+                const float3 rv = load_global<float3>(a.reference, boff);
                 const float px = cx0 + fmaf(q.L[1], ly, axl);
-                const float py = cy0 + fmaf(q.L[2], lxc, fmaf(vy, ly, q.T[1]));
+                const float py = cy0 + fmaf(vy, ly, ayl);
                 int ix, iy;
                 float tx, ty;
                 floor_split(px, ix, tx);
@@ -716,7 +775,7 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 float wx[4], wy[4];
                 bspline_weights6(tx, wx);
                 bspline_weights6(ty, wy);
-                const float* b = tile + (__mul24(iy, pitch) + ix - bias);
+                const float* b = tile_tap<TP>(origin, ix, iy);
                 float cur;
 #pragma unroll
                 for (int yy = 0; yy < 4; ++yy) {
@@ -734,21 +793,25 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
                 sj[0] += gxg;
                 sj[3] += gyg;
                 if (MODEL != kTranslation) {
-                    sj[1] += gxg * lxc;
                     sj[2] += gxg * ly;
-                    sj[4] += gyg * lxc;
                     sj[5] += gyg * ly;
                 }
             };
             for (int k = 0; k < (S >> 1); ++k) {
                 if (on) pixel();
-                rp += rstep;
+                boff += rstep;
                 ly += 2.0f;
             }
             if ((S & 1) && on && h == 0) pixel();
+            // The lane's column is constant: its sums of g gx lx and g gy lx are
+            // lx times its sums of g gx and g gy.
+            if (MODEL != kTranslation) {
+                sj[1] = sj[0] * lxc;
+                sj[4] = sj[3] * lxc;
+            }
         };
         // The sampling path is chosen once per iteration (wave-uniform).
-        if (MODEL != kQuadratic && !WEIGHTED && INTERP == kBspline && S >= 30 && S <= 32 && use_tile)
+        if (TP > 0 && MODEL != kQuadratic && !WEIGHTED && INTERP == kBspline && S >= 30 && S <= 32 && use_tile)
             row_sweep_tile();
         else if (use_tile)
             sweep([&](float px, float py) { return tile_sample<INTERP, TP>(tile, t, px, py); });
@@ -766,9 +829,15 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
             sj[4] = wave_sum_bcast(sj[4]);
             sj[5] = wave_sum_bcast(sj[5]);
         }
-        if (MODEL == kQuadratic)
+        if (MODEL == kQuadratic) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) sq[k] = wave_sum_bcast(sq[k]);
+            // Sums of g lx^2 and g ly^2 to sums of g lx^2 / 2 and g ly^2 / 2.
+            sq[0] *= 0.5f;
+            sq[2] *= 0.5f;
+            sq[3] *= 0.5f;
+            sq[5] *= 0.5f;
+        }
 
         // Hardware reciprocals (1 ulp) of per-subset constants replace two of the
         // iteration's divisions; the norm keeps a correctly rounded one.
@@ -926,15 +995,18 @@ int solver_pixels_per_lane(int S, int model) {
 // Tile pitch of each register-cached variant (0: pitch = staged width).
 static int solver_pitch(int ppl) { return ppl == 4 ? 32 : 0; }
 
-// Row pitch of the uncached translation/affine B-spline variant for subsets
-// 30-32.  Their tile slice, (S + 12)^2 floats, then holds 40-44 rows of 44: the
-// zero-strain footprint with its re-staging margin is S + 9 wide and high.
+// Fixed row pitch of the uncached B-spline variants: translation/affine
+// subsets 30-32 (unweighted, the row sweep) and quadratic subsets up to 32
+// (weighted or not).  The zero-strain footprint with its re-staging margin is
+// S + 9 wide and high, so 44 columns hold it up to subset 35.
 constexpr int kRowSweepPitch = 44;
 
 int solver_tile_pitch(int S, int model, int interp, int weighted) {
     const int ppl = solver_pixels_per_lane(S, model);
     if (ppl > 0) return solver_pitch(ppl);
-    return (model != kQuadratic && interp == kBspline && !weighted && S >= 30 && S <= 32) ? kRowSweepPitch : 0;
+    if (interp != kBspline) return 0;
+    if (model == kQuadratic) return S <= 32 ? kRowSweepPitch : 0;
+    return (!weighted && S >= 30 && S <= 32) ? kRowSweepPitch : 0;
 }
 
 size_t solver_lds_bytes(int S, int model, int* tile_cap) {
@@ -948,7 +1020,8 @@ size_t solver_lds_bytes(int S, int model, int* tile_cap) {
         // Up to subset 32, room for strain beyond the margin; above it the
         // footprint with its margin (S + 9 square) only, so a 41 x 41 subset's
         // 10 KB slice leaves room for 4 waves per SIMD (16 per CU).
-        cap = S <= 32 ? (S + 12) * (S + 12) : (S + 9) * (S + 9);
+        // Quadratic subsets up to 32 keep S + 12 rows of the fixed 44-word pitch.
+        cap = S <= 32 ? (S + 12) * (model == kQuadratic ? kRowSweepPitch : S + 12) : (S + 9) * (S + 9);
         if (cap > 4096) cap = 4096;  // 16 KB of LDS per wave
     }
     *tile_cap = cap;
@@ -964,9 +1037,10 @@ static hipError_t launch_ppl(int ppl, const SolveArgs& a, hipStream_t s) {
     size_t lds = solver_lds_bytes(a.S, MODEL, &cap);
     if (cap != a.tile_cap) return hipErrorInvalidValue;
     dim3 grid((N + kSolveBlockWaves - 1) / kSolveBlockWaves), block(kWave * kSolveBlockWaves);
-    if constexpr (MODEL != kQuadratic && INTERP == kBspline && !WEIGHTED) {
-        // Uncached subsets 30-32: fixed row pitch for the row sweep's taps.
-        if (solver_tile_pitch(a.S, MODEL, INTERP, WEIGHTED) == kRowSweepPitch) {
+    if constexpr (INTERP == kBspline && (MODEL == kQuadratic || !WEIGHTED)) {
+        // Fixed row pitch: immediate tap offsets (and the row sweep for
+        // translation/affine subsets 30-32).
+        if (ppl == 0 && solver_tile_pitch(a.S, MODEL, INTERP, WEIGHTED) == kRowSweepPitch) {
             k_solve<MODEL, INTERP, 0, WEIGHTED, kRowSweepPitch><<<grid, block, lds, s>>>(a);
             return hipGetLastError();
         }
@@ -999,7 +1073,7 @@ static hipError_t launch_batched_ppl(const BatchLaunch& b, hipStream_t s) {
     if (b.total <= 0) return hipSuccess;
     if (b.waves != 1 && b.waves != 2 && b.waves != 4) return hipErrorInvalidValue;
     dim3 grid(b.grid), block(kWave * b.waves);
-    if constexpr (MODEL != kQuadratic && INTERP == kBspline && !WEIGHTED) {
+    if constexpr (INTERP == kBspline && (MODEL == kQuadratic || !WEIGHTED)) {
         if (b.ppl == 0 && b.pitch == kRowSweepPitch) {
             k_solve_batched<MODEL, INTERP, 0, WEIGHTED, kRowSweepPitch><<<grid, block, b.lds, s>>>(
                 b.pairs, b.block_start, b.npairs, b.total, b.wave_stride);

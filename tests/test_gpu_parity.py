@@ -365,3 +365,15 @@ def test_seeded_large_rotation_uses_clamped_gathers_and_matches_oracle(size, str
     print(stats, "accepted inside", ok[inside].mean(), "rmse", np.sqrt(np.mean(err**2)))
     assert ok[inside].mean() > 0.9 and np.sqrt(np.mean(err**2)) < 0.02
     assert stats["status_equal"] >= 0.999 and stats["disp_within_1e3"] >= 0.999
+
+
+def test_reference_of_2_28_pixels_is_refused():
+    """The solver addresses the 16-byte reference plane by 32-bit byte offsets, so set_reference refuses an
+    image of 2^28 pixels with a message, before any device work."""
+    from live_dic import ICGNError, ICGNWGPUFullPyramid
+
+    image = np.zeros((16384, 16384), np.uint8)
+    _say("set_reference 16384x16384 (2^28 pixels) refused")
+    with ICGNWGPUFullPyramid(pyramid_levels=1) as eng:
+        with pytest.raises(ICGNError, match=r"2\^28"):
+            eng.set_reference(image)
