@@ -38,15 +38,20 @@ HBM_PEAK_GBPS = 8000.0
 # FETCH_SIZE/WRITE_SIZE and scaled by a 256 MiB copy calibration (reads 2.0x FETCH_SIZE,
 # writes 1.0x WRITE_SIZE), and its L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS):
 # profiles/k_solve_affine_4k_s31_d8_pmc.md.
-MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_898 + 4_191) * 1024
-MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.317e7 / (4.317e7 + 1.661e6)
-# VALU wave-instructions of the same launch (rocprofv3 SQ_INSTS_VALU, same profile), and the
-# issue limit it is set against: one wave64 FP32 operation per 4 cycles per SIMD, which is the
-# published non-packed FP32 (= FP64) vector peak of 78.6 TFLOP/s over 256 CUs x 4 SIMDs x
-# 64 lanes x 2 FLOP per FMA, i.e. 6.0e8 wave-instructions/s per SIMD at 2.4 GHz.
-MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS = 4.3985e8
+MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_910 + 4_278) * 1024
+MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.3275e7 / (4.3275e7 + 1.6607e6)
+# FP32 arithmetic of the same launch (rocprofv3 SQ_INSTS_VALU_FMA_F32 + _MUL_F32 + _ADD_F32 wave
+# instructions and SQ_INSTS_VALU_FLOPS_FP32, same profile), set against the published
+# non-packed FP32 vector peak, 78.6 TFLOP/s (this build emits no packed FP32): one wave64 FP32
+# FMA, multiply or add per 4 cycles per SIMD, 6.0e8 wave-instructions/s per SIMD over
+# 256 CUs x 4 SIMDs at 2.4 GHz.  The launch's other VALU instructions (integer, conversion,
+# move, DPP, lane reads: 30 % of SQ_INSTS_VALU) issue faster and are reported beside it.
+MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS = 4.1624e8
+MEASURED_FINE_SOLVE_FP32_INSTRUCTIONS = 1.6250e8 + 7.478e7 + 5.4228e7
+MEASURED_FINE_SOLVE_FP32_FLOPS = 4.5651e8 * 64
 SIMDS = 256 * 4
-VALU_ISSUE_PEAK_PER_SIMD = 78.6e12 / (SIMDS * 64 * 2)
+FP32_VECTOR_PEAK_FLOPS = 78.6e12
+FP32_ISSUE_PEAK_PER_SIMD = FP32_VECTOR_PEAK_FLOPS / (SIMDS * 64 * 2)
 SUBSET, STRIDE = 31, 8
 WIDTH, HEIGHT = 3840, 2160
 FRAMES = 60
@@ -367,18 +372,24 @@ SMALL_SUBSET, SMALL_STRIDE = 21, 8
 
 
 def valu_limiter(avg_ms: float, alone_ms: float | None) -> dict:
-    """VALU issue rate of the fine solve per SIMD against the FP32 issue limit."""
-    rate = MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS / SIMDS / (avg_ms * 1e-3)
+    """FP32 VALU issue of the fine solve per SIMD against the non-packed FP32 issue peak, and its
+    FP32 FLOP rate against the matching 78.6 TFLOP/s."""
+    def rate(ms: float) -> float:
+        return MEASURED_FINE_SOLVE_FP32_INSTRUCTIONS / SIMDS / (ms * 1e-3)
+
     out = {
         "bound": "valu",
-        "unit": "VALU wave-instructions/s per SIMD",
-        "achieved": rate,
-        "peak": VALU_ISSUE_PEAK_PER_SIMD,
-        "frac": rate / VALU_ISSUE_PEAK_PER_SIMD,
+        "unit": "FP32 FMA/MUL/ADD wave-instructions/s per SIMD",
+        "achieved": rate(avg_ms),
+        "peak": FP32_ISSUE_PEAK_PER_SIMD,
+        "frac": rate(avg_ms) / FP32_ISSUE_PEAK_PER_SIMD,
+        "fp32_instructions_per_launch": MEASURED_FINE_SOLVE_FP32_INSTRUCTIONS,
         "valu_instructions_per_launch": MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS,
+        "fp32_tflops": MEASURED_FINE_SOLVE_FP32_FLOPS / (avg_ms * 1e-3) / 1e12,
+        "fp32_tflops_peak": FP32_VECTOR_PEAK_FLOPS / 1e12,
     }
     if alone_ms:
-        out["frac_alone"] = MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS / SIMDS / (alone_ms * 1e-3) / VALU_ISSUE_PEAK_PER_SIMD
+        out["frac_alone"] = rate(alone_ms) / FP32_ISSUE_PEAK_PER_SIMD
     return out
 
 

@@ -235,13 +235,19 @@ __device__ void quadratic_moments(const PrepArgs& a, int lane, int ox, int oy, d
     });
 }
 
-// Column-parallel Gauss-Jordan elimination with partial pivoting of the
-// P x 2P augmented matrix [H | I] whose left half is in LDS (row pitch 2P).
-// Lane e < 2P keeps column e in registers; each pivot column is broadcast
-// with readlane, and every lane applies the row exchange, the division by
-// the pivot and the row updates to its own column.  The result is written
-// back to LDS.  Returns false for an exactly singular matrix (a zero pivot),
-// which is when LAPACK's LU factorisation fails too.
+// Column-parallel Gauss-Jordan elimination of the P x 2P augmented matrix
+// [H | I] whose left half is in LDS (row pitch 2P), in diagonal order without
+// row exchanges.  H is a sum of outer products J J^T, symmetric positive
+// semi-definite, so its pivots (the diagonals of the successive Schur
+// complements) are non-negative and the elimination is stable without
+// pivoting, as a Cholesky factorisation is; partial pivoting would pick the
+// rows of the large quadratic terms first and exchange rows at most steps.
+// Lane e < 2P keeps column e in registers; each pivot column is broadcast with
+// readlane, and every lane applies the division by the pivot and the row
+// updates to its own column.  The result is written back to LDS.  Returns
+// false for an exactly singular matrix (a zero pivot), the failure of the
+// reference's LU inversion; a nearly singular one yields entries beyond the
+// float32 range, which the caller rejects.
 template <int P>
 __device__ bool invert(double* m, int lane) {
     constexpr int C = 2 * P;
@@ -255,30 +261,9 @@ __device__ bool invert(double* m, int lane) {
         double pc[P];
 #pragma unroll
         for (int r = 0; r < P; ++r) pc[r] = readlane_f64(col[r], c);
-        int pr = c;
-        double ps = fabs(pc[c]);
-#pragma unroll
-        for (int cand = c + 1; cand < P; ++cand) {
-            const double cs = fabs(pc[cand]);
-            if (cs > ps) {
-                pr = cand;
-                ps = cs;
-            }
-        }
-        if (ps == 0.0) {
+        if (pc[c] == 0.0) {
             singular = true;
             break;
-        }
-#pragma unroll
-        for (int r = c + 1; r < P; ++r) {
-            const bool sw = r == pr;
-            double x = col[c], y = col[r];
-            col[c] = sw ? y : x;
-            col[r] = sw ? x : y;
-            x = pc[c];
-            y = pc[r];
-            pc[c] = sw ? y : x;
-            pc[r] = sw ? x : y;
         }
         const double top = col[c] / pc[c];
         col[c] = top;
