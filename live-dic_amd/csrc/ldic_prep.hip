@@ -1,6 +1,6 @@
 // Per-subset reference statistics in float64 (wgpu_backend.py:639-823): the
 // anchored mean, energy, J^T f, sum J, H = sum w J J^T and its inverse by
-// Gauss-Jordan elimination with partial pivoting.  One wave64 per subset:
+// Gauss-Jordan elimination in diagonal order.  One wave64 per subset:
 // lanes stride over the subset pixels, DPP wave sums give every lane the
 // totals, and the inversion runs column-parallel on the wave's LDS slice.
 #include "ldic_device.h"
