@@ -178,8 +178,14 @@ __device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, f
         float maxx = cx - 0.5f + bx1 + q.T[0], maxy = cy - 0.5f + by1 + q.T[1];
         return minx >= 2.0f && miny >= 2.0f && maxx <= W - 3.0f && maxy <= H - 3.0f;
     }
-    bool inside = true;
+    // Every point inside <=> the extremes inside and no coordinate NaN.  The
+    // positions are the per-point ones; fminf/fmaxf skip a NaN operand, so a
+    // NaN point is caught by the unordered test instead.  (Testing the points
+    // one by one compiled to a 16-bit mask built lane by lane, twice the
+    // instructions.)
     const int count = MODEL == kQuadratic ? 9 : 4;
+    float x0 = 0.0f, x1 = 0.0f, y0 = 0.0f, y1 = 0.0f;
+    bool nan = false;
 #pragma unroll
     for (int p = 0; p < count; ++p) {
         float lx, ly;
@@ -193,9 +199,13 @@ __device__ __forceinline__ bool warp_inside(const Warp& q, float cx, float cy, f
         float dx, dy;
         displacement<MODEL>(q, lx, ly, dx, dy);
         float px = cx - 0.5f + lx + dx, py = cy - 0.5f + ly + dy;
-        inside = inside && px >= 2.0f && py >= 2.0f && px <= W - 3.0f && py <= H - 3.0f;
+        nan = nan || __builtin_isunordered(px, py);
+        x0 = p == 0 ? px : fminf(x0, px);
+        x1 = p == 0 ? px : fmaxf(x1, px);
+        y0 = p == 0 ? py : fminf(y0, py);
+        y1 = p == 0 ? py : fmaxf(y1, py);
     }
-    return inside;
+    return !nan && x0 >= 2.0f && y0 >= 2.0f && x1 <= W - 3.0f && y1 <= H - 3.0f;
 }
 
 // icgn.wgsl:437-513; false when a pivot falls below 1e-8.  The Gauss-Jordan
@@ -335,12 +345,16 @@ __device__ __forceinline__ float increment_change(const float* d, float he) {
         }
         return m;
     }
+    // The largest squared length, then one square root: sqrtf is correctly
+    // rounded and so monotone, and sqrtf(max) equals the max of the sqrtfs.  A
+    // NaN length is skipped by fmaxf in either form.
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float x = (c == 1 || c == 3) ? he : -he, y = c >= 2 ? he : -he;
-        m = fmaxf(m, hypot2(d[0] + d[1] * x + d[2] * y, d[3] + d[4] * x + d[5] * y));
+        const float a = d[0] + d[1] * x + d[2] * y, b = d[3] + d[4] * x + d[5] * y;
+        m = fmaxf(m, a * a + b * b);
     }
-    return m;
+    return sqrtf(m);
 }
 
 // Lane-parallel forms of the quadratic 3x3 tests: lane l evaluates grid point
@@ -375,7 +389,9 @@ __device__ __forceinline__ void stage_tile(const SolveArgs& a, float* tile, cons
     auto load = [&](uint32_t g) -> float {
         return INTERP == kBspline ? load_global<float>(a.coeff, g * 4u) : (float)load_global<uint8_t>(a.target, g);
     };
-    if (t.w <= 2 * 32) {
+    // A fixed pitch bounds the box width (t.w <= TP), so with TP <= 64 the
+    // element-wise path below is never taken and is not compiled.
+    if ((TP > 0 && TP <= 2 * 32) || t.w <= 2 * 32) {
         // Boxes up to 64 wide: half-wave h copies rows h, h + 2, ... (then the last
         // row in half-wave 0 when the height is odd); lane c copies columns c and
         // c + 32.  Lanes past the box width load its last column and skip the
@@ -818,20 +834,43 @@ __device__ __forceinline__ void solve_subset(const SolveArgs& a, const int subse
         else
             sweep([&](float px, float py) { return global_sample<INTERP>(a, px, py); });
 
-        sc = wave_sum_bcast(sc);
-        sc2 = wave_sum_bcast(sc2);
-        src = wave_sum_bcast(src);
-        sj[0] = wave_sum_bcast(sj[0]);
-        sj[3] = wave_sum_bcast(sj[3]);
-        if (MODEL != kTranslation) {
-            sj[1] = wave_sum_bcast(sj[1]);
-            sj[2] = wave_sum_bcast(sj[2]);
-            sj[4] = wave_sum_bcast(sj[4]);
-            sj[5] = wave_sum_bcast(sj[5]);
+        {
+            // All of the iteration's sums in one multi-value wave reduction.
+            constexpr int NS = MODEL == kTranslation ? 5 : MODEL == kQuadratic ? 15 : 9;
+            float r[NS];
+            r[0] = sc;
+            r[1] = sc2;
+            r[2] = src;
+            r[3] = sj[0];
+            r[4] = sj[3];
+            if constexpr (MODEL != kTranslation) {
+                r[5] = sj[1];
+                r[6] = sj[2];
+                r[7] = sj[4];
+                r[8] = sj[5];
+            }
+            if constexpr (MODEL == kQuadratic) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) r[9 + k] = sq[k];
+            }
+            wave_sums(r, lane);
+            sc = r[0];
+            sc2 = r[1];
+            src = r[2];
+            sj[0] = r[3];
+            sj[3] = r[4];
+            if constexpr (MODEL != kTranslation) {
+                sj[1] = r[5];
+                sj[2] = r[6];
+                sj[4] = r[7];
+                sj[5] = r[8];
+            }
+            if constexpr (MODEL == kQuadratic) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sq[k] = r[9 + k];
+            }
         }
         if (MODEL == kQuadratic) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sq[k] = wave_sum_bcast(sq[k]);
             // Sums of g lx^2 and g ly^2 to sums of g lx^2 / 2 and g ly^2 / 2.
             sq[0] *= 0.5f;
             sq[2] *= 0.5f;
