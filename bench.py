@@ -38,17 +38,17 @@ HBM_PEAK_GBPS = 8000.0
 # FETCH_SIZE/WRITE_SIZE and scaled by a 256 MiB copy calibration (reads 2.0x FETCH_SIZE,
 # writes 1.0x WRITE_SIZE), and its L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS):
 # profiles/k_solve_affine_4k_s31_d8_pmc.md.
-MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_910 + 4_278) * 1024
-MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.3275e7 / (4.3275e7 + 1.6607e6)
+MEASURED_FINE_SOLVE_TRAFFIC_BYTES = (2 * 116_892 + 4_286) * 1024
+MEASURED_FINE_SOLVE_L2_HIT_RATE = 4.3226e7 / (4.3226e7 + 1.6606e6)
 # FP32 arithmetic of the same launch (rocprofv3 SQ_INSTS_VALU_FMA_F32 + _MUL_F32 + _ADD_F32 wave
 # instructions and SQ_INSTS_VALU_FLOPS_FP32, same profile), set against the published
 # non-packed FP32 vector peak, 78.6 TFLOP/s (this build emits no packed FP32): one wave64 FP32
 # FMA, multiply or add per 4 cycles per SIMD, 6.0e8 wave-instructions/s per SIMD over
 # 256 CUs x 4 SIMDs at 2.4 GHz.  The launch's other VALU instructions (integer, conversion,
-# move, DPP, lane reads: 30 % of SQ_INSTS_VALU) issue faster and are reported beside it.
-MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS = 4.1624e8
-MEASURED_FINE_SOLVE_FP32_INSTRUCTIONS = 1.6250e8 + 7.478e7 + 5.4228e7
-MEASURED_FINE_SOLVE_FP32_FLOPS = 4.5651e8 * 64
+# move, DPP, lane reads: 25 % of SQ_INSTS_VALU) issue faster and are reported beside it.
+MEASURED_FINE_SOLVE_VALU_INSTRUCTIONS = 3.6887e8
+MEASURED_FINE_SOLVE_FP32_INSTRUCTIONS = 1.6092e8 + 7.3194e7 + 4.3653e7
+MEASURED_FINE_SOLVE_FP32_FLOPS = 4.4039e8 * 64
 SIMDS = 256 * 4
 FP32_VECTOR_PEAK_FLOPS = 78.6e12
 FP32_ISSUE_PEAK_PER_SIMD = FP32_VECTOR_PEAK_FLOPS / (SIMDS * 64 * 2)
