@@ -172,24 +172,38 @@ __global__ __launch_bounds__(256) void k_gauss_rows(const uint8_t* __restrict__ 
     out[(size_t)y * w + x] = acc;
 }
 
+// Each thread filters kGaussRows consecutive rows of its column from a register
+// window of kGaussRows + 6 input rows (one load per input row instead of seven
+// per output); the sums are formed in the same order as a one-output thread.
+constexpr int kGaussRows = 8;
+
 __global__ __launch_bounds__(256) void k_gauss_cols(const double* __restrict__ src, int w, int h, Taps7 t,
                                                     double* __restrict__ out) {
 #pragma clang fp contract(off)
-    int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    const int x = blockIdx.x * 256 + threadIdx.x, y0 = blockIdx.y * kGaussRows;
     if (x >= w) return;
-    double acc = t.w[0] * src[(size_t)reflect(y - 3, h) * w + x];
+    const int rows = min(kGaussRows, h - y0);
+    double v[kGaussRows + 6];
 #pragma unroll
-    for (int i = 1; i < 7; ++i) acc = acc + t.w[i] * src[(size_t)reflect(y + i - 3, h) * w + x];
-    out[(size_t)y * w + x] = acc;
+    for (int j = 0; j < kGaussRows + 6; ++j)
+        v[j] = j < rows + 6 ? src[(size_t)reflect(y0 + j - 3, h) * w + x] : 0.0;
+#pragma unroll
+    for (int r = 0; r < kGaussRows; ++r) {
+        if (r >= rows) break;
+        double acc = t.w[0] * v[r];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) acc = acc + t.w[i] * v[r + i];
+        out[(size_t)(y0 + r) * w + x] = acc;
+    }
 }
 
 hipError_t launch_gaussian(const uint8_t* src, int w, int h, const double* w7, double* tmp, double* out,
                            hipStream_t s) {
     Taps7 t;
     for (int i = 0; i < 7; ++i) t.w[i] = w7[i];
-    dim3 grid((w + 255) / 256, h);
+    dim3 grid((w + 255) / 256, h), grid_cols((w + 255) / 256, (h + kGaussRows - 1) / kGaussRows);
     k_gauss_rows<<<grid, 256, 0, s>>>(src, w, h, t, tmp);
-    k_gauss_cols<<<grid, 256, 0, s>>>(tmp, w, h, t, out);
+    k_gauss_cols<<<grid_cols, 256, 0, s>>>(tmp, w, h, t, out);
     return hipGetLastError();
 }
 
@@ -210,9 +224,14 @@ hipError_t launch_to_double(const uint8_t* src, int w, int h, double* out, hipSt
 // from a zero state (the start error decays as |z|^kLook < 1e-18) and its
 // anti-causal recursion kLook samples late; segments touching a line end use
 // the reference's exact boundary conditions.  Lines of the column pass are
-// adjacent in memory, so consecutive threads read consecutive addresses.
+// adjacent in memory, so consecutive threads read consecutive addresses.  The
+// recursion is serial along a line but its loads are not: they are issued
+// kChunk at a time, so a lane has kChunk loads in flight instead of one (the
+// row pass, whose lanes read 64 different cache lines per load, is
+// latency-bound otherwise).
 constexpr int kSeg = 64;
 constexpr int kLook = 32;
+constexpr int kChunk = 8;
 
 struct Spline {
     double z, gain, end_scale;
@@ -246,9 +265,17 @@ __global__ __launch_bounds__(256) void k_spline_causal(const double* __restrict_
     } else {
         c = sp.gain * at(in, line, start, w, COLUMNS);
     }
-    for (int i = start + 1; i < b; ++i) {
-        c = sp.gain * at(in, line, i, w, COLUMNS) + sp.z * c;
-        if (i >= a) out[COLUMNS ? (size_t)i * w + line : (size_t)line * w + i] = c;
+    for (int i0 = start + 1; i0 < b; i0 += kChunk) {
+        double x[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) x[k] = i0 + k < b ? at(in, line, i0 + k, w, COLUMNS) : 0.0;
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = i0 + k;
+            if (i >= b) break;
+            c = sp.gain * x[k] + sp.z * c;
+            if (i >= a) out[COLUMNS ? (size_t)i * w + line : (size_t)line * w + i] = c;
+        }
     }
 }
 
@@ -272,9 +299,17 @@ __global__ __launch_bounds__(256) void k_spline_anticausal(const double* __restr
     } else {
         c = sp.z * (0.0 - at(cp, line, stop, w, COLUMNS));
     }
-    for (int i = stop - 1; i >= a; --i) {
-        c = sp.z * (c - at(cp, line, i, w, COLUMNS));
-        if (i < b) out[COLUMNS ? (size_t)i * w + line : (size_t)line * w + i] = c;
+    for (int i0 = stop - 1; i0 >= a; i0 -= kChunk) {
+        double x[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) x[k] = i0 - k >= a ? at(cp, line, i0 - k, w, COLUMNS) : 0.0;
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const int i = i0 - k;
+            if (i < a) break;
+            c = sp.z * (c - x[k]);
+            if (i < b) out[COLUMNS ? (size_t)i * w + line : (size_t)line * w + i] = c;
+        }
     }
 }
 
