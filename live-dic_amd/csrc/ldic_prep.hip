@@ -415,11 +415,46 @@ __host__ __device__ inline int strip_subset_rows(int S, int D) {
     return fit < 1 ? 1 : (fit > kStripRows ? kStripRows : fit);
 }
 
+// Inverts, in each 16-lane row, the 12 x 12 matrix whose column e sits in
+// col[] of the row's lane e (e < 12).  Gauss-Jordan elimination in diagonal
+// order, in place: at pivot c, DPP row_newbcast gives every lane of the row the
+// pivot column; lane c, whose column has become the unit vector, turns into
+// column c of the inverse, formed exactly as the augmented identity column c
+// of [M | I] is in invert() (1 / pivot, then 0 - p_r * top), and every other
+// lane applies the same update as there.  So each entry goes through the same
+// operations as in invert(), without the columns that hold only the unit
+// vectors.  Returns false when a pivot is zero.
+template <int C>
+__device__ __forceinline__ double row_bcast_f64(double v) {
+    int2 p = *reinterpret_cast<int2*>(&v);
+    p.x = __builtin_amdgcn_update_dpp(0, p.x, 0x150 + C, 0xF, 0xF, false);  // row_newbcast:C
+    p.y = __builtin_amdgcn_update_dpp(0, p.y, 0x150 + C, 0xF, 0xF, false);
+    return *reinterpret_cast<double*>(&p);
+}
+
+__device__ __forceinline__ bool strip_invert(double (&col)[12], const int e) {
+    constexpr int P = 12;
+    bool singular = false;
+    static_for<P>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        double pc[P];
+#pragma unroll
+        for (int r = 0; r < P; ++r) pc[r] = row_bcast_f64<c>(col[r]);
+        singular = singular || pc[c] == 0.0;
+        const bool pivot = e == c;
+        const double top = (pivot ? 1.0 : col[c]) / pc[c];
+#pragma unroll
+        for (int r = 0; r < P; ++r)
+            if (r != c) col[r] = (pivot ? 0.0 : col[r]) - pc[r] * top;
+        col[c] = top;
+    });
+    return !singular;
+}
+
 __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows_per_block, int pitch) {
-    constexpr int P = 12, C = 24;
+    constexpr int P = 12;
     extern __shared__ double T[];  // kStripQ x pitch row sums
     __shared__ double mom[kStripRows * kStripMoments];
-    __shared__ double mats[kPrepWaves][P * C];
     const int strips_y = (a.grid_h + rows_per_block - 1) / rows_per_block;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int gx = blk / strips_y, gy0 = (blk - gx * strips_y) * rows_per_block;
@@ -496,35 +531,53 @@ __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows
     }
     __syncthreads();
 
-    // 3. One wave per subset: Hessian, first moments, inversion, outputs.
+    // 3. Four subsets per wave, one per 16-lane row: lane e < 12 of row g holds
+    //    column e of subset j0 + g's Hessian in registers, built from the
+    //    moments, and the rows invert their matrices side by side (strip_invert).
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double* mat = mats[wave];
-    for (int j = wave; j < rows; j += kPrepWaves) {
+    const int g = lane >> 4, e = lane & 15;
+    const int ec = e < P ? e : P - 1;  // lanes 12-15 shadow column 11
+    for (int j0 = wave * 4; j0 < rows; j0 += kPrepWaves * 4) {
+        const int j = min(j0 + g, rows - 1);
         const double* m = mom + j * kStripMoments;
         const double sum_i = m[strip_offset(27)];
         const double mean = sum_i / a.weight_sum;
         const double energy = fmax(0.0, m[strip_offset(28)] - mean * sum_i);
-        double jtf[12], sj[12];
-        static_for<12>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            constexpr int u = jac_mono(k), g = jac_grad(k);
-            constexpr int px = mono_px(u), py = mono_py(u);
-            const double sg = m[strip_offset(21 + g * 3 + px) + py];
-            const double si = m[strip_offset(15 + g * 3 + px) + py];
-            jtf[k] = mono_c(u) * (si - mean * sg);
-            sj[k] = mono_c(u) * sg;
+        const int ue = jac_mono(ec), ge = jac_grad(ec);
+        const double sg = m[strip_offset(21 + ge * 3 + mono_px(ue)) + mono_py(ue)];
+        const double si = m[strip_offset(15 + ge * 3 + mono_px(ue)) + mono_py(ue)];
+        const double jtf = mono_c(ue) * (si - mean * sg), sj = mono_c(ue) * sg;
+        double col[P];
+        static_for<P>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            constexpr int ur = jac_mono(r), gr = jac_grad(r);
+            const int st = gr + ge, px = mono_px(ur) + mono_px(ue), py = mono_py(ur) + mono_py(ue);
+            col[r] = mono_c(ur) * mono_c(ue) * m[strip_offset(st * 5 + px) + py];
         });
-        wave_lds_sync();  // the previous subset's inverse has been read
-        for (int e = lane; e < P * P; e += kWave) {
-            const int i = e / P, k = e - i * P;
-            const int ui = jac_mono(i), uk = jac_mono(k);
-            const int st = jac_grad(i) + jac_grad(k);
-            const int px = mono_px(ui) + mono_px(uk), py = mono_py(ui) + mono_py(uk);
-            mat[i * C + k] = mono_c(ui) * mono_c(uk) * m[strip_offset(st * 5 + px) + py];
+        const bool ok = strip_invert(col, e);
+        bool bad = false;
+#pragma unroll
+        for (int r = 0; r < P; ++r) bad = bad || !isfinite(col[r]) || fabs(col[r]) > (double)FLT_MAX;
+        const uint64_t bad_lanes = __ballot(bad && e < P);
+        const bool valid = ok && ((bad_lanes >> (16 * g)) & 0xFFFull) == 0 && energy > 1e-12;
+        if (j0 + g >= rows || e >= P) continue;
+        const int subset = (gy0 + j) * a.grid_w + gx;
+        float* meta = a.metadata + (size_t)subset * 28;
+        float* invp = a.inverse + (size_t)subset * P * P;
+        if (e == 0) {
+            float status = valid ? 1.0f : 0.0f;
+            if (a.valid) a.valid[subset] = valid ? 1 : 0;
+            if (a.selected && a.selected[subset] == 0) status = -3.0f;
+            meta[0] = (float)sqrt(energy);
+            meta[1] = status;
+            meta[2 + 2 * P] = a.D == a.S ? 0.0f : (float)mean;
+            meta[3 + 2 * P] = (float)a.weight_sum;
         }
-        wave_lds_sync();
-        finish_subset<kQuadratic>(a, (gy0 + j) * a.grid_w + gx, lane, mat, energy, jtf, sj, mean);
+        meta[2 + e] = (float)jtf;
+        meta[2 + P + e] = (float)sj;
+#pragma unroll
+        for (int r = 0; r < P; ++r) invp[r * P + e] = valid ? (float)col[r] : 0.0f;
     }
 }
 
