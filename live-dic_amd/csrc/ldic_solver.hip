@@ -243,20 +243,25 @@ __device__ __forceinline__ bool compose_quadratic(Warp& q, const float d[12], co
         float pc[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) pc[r] = readlane_f32(col[r], c);
+        // Partial pivoting on the magnitudes' bit patterns, which are wave-uniform, so
+        // the search runs in scalar integer instructions: for non-NaN floats |a| > |b|
+        // exactly when bits(|a|) > bits(|b|) as unsigned integers, and a NaN (bits
+        // above those of infinity) neither wins nor lets a later candidate win over
+        // it, as in the float comparison |pc[cand]| > |pc[pr]|.
+        constexpr uint32_t kInfBits = 0x7f800000u;
+        uint32_t pb = __float_as_uint(pc[c]) & 0x7fffffffu;
         int pr = c;
-        float ps = fabsf(pc[c]);
 #pragma unroll
         for (int cand = c + 1; cand < 6; ++cand) {
-            float cs = fabsf(pc[cand]);
-            if (cs > ps) {
+            const uint32_t cb = __float_as_uint(pc[cand]) & 0x7fffffffu;
+            if (cb <= kInfBits && pb <= kInfBits && cb > pb) {
                 pr = cand;
-                ps = cs;
+                pb = cb;
             }
         }
-        if (ps < 1e-8f) return false;
-        // The pivot row is wave-uniform; the exchange (rarely needed: the lifted operator of
-        // an increment is close to the identity) is skipped by a scalar branch.
-        pr = __builtin_amdgcn_readfirstlane(pr);
+        if (__uint_as_float(pb) < 1e-8f) return false;
+        // The exchange (rarely needed: the lifted operator of an increment is close to
+        // the identity) is skipped by a scalar branch.
         if (pr != c) {
 #pragma unroll
             for (int r = c + 1; r < 6; ++r) {
