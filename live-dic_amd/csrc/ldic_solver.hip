@@ -70,6 +70,14 @@ __device__ __forceinline__ const float* tile_tap(const float* origin, int ix, in
     return origin + idx;
 }
 
+// The same with the wave-uniform pitch of the staged box (a scalar operand), for
+// tiles without a compile-time pitch; pitches are below 2^24 too.
+__device__ __forceinline__ const float* tile_tap_dyn(const float* origin, int ix, int iy, int pitch) {
+    int idx;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(idx) : "v"(iy), "s"(pitch), "v"(ix));
+    return origin + idx;
+}
+
 // TP > 0: compile-time row pitch, so the 16 taps are immediate offsets from one
 // per-pixel LDS address; TP == 0: the pitch is the staged box width.
 template <int INTERP, int TP>
@@ -97,7 +105,7 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
     if constexpr (TP > 0)
         b = tile_tap<TP>(tile - (TP * (t.y0 + 1) + t.x0 + 1), ix, iy);
     else
-        b = tile + (__mul24(iy - 1 - t.y0, pitch) + (ix - 1 - t.x0));
+        b = tile_tap_dyn(tile - (pitch * (t.y0 + 1) + t.x0 + 1), ix, iy, pitch);
     float v;
 #pragma unroll
     for (int yy = 0; yy < 4; ++yy) {
