@@ -451,6 +451,59 @@ __device__ __forceinline__ bool strip_invert(double (&col)[12], const int e) {
     return !singular;
 }
 
+// Row-segment sums of one image row over the subset column's S pixels
+// (x = c - (S-1)/2), stored at out[q * pitch]: part 0 the 15 sums g_s g_t x^p
+// (quantities 0-14), part 1 the other 14 (15-28).  Every sum accumulates the same
+// products in the same order as when one thread forms all 29.
+template <int PART>
+__device__ __forceinline__ void strip_row_sums(const PrepArgs& a, const size_t row, const int S, const double half,
+                                               double* out, const int pitch) {
+    constexpr int Q0 = PART == 0 ? 0 : 15, NQ = PART == 0 ? 15 : kStripQ - 15;
+    double t[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) t[q] = 0.0;
+    auto add = [&](double I, double gx_, double gy_, int c) {
+        const double x = (double)c - half, x2 = x * x;
+        if constexpr (PART == 0) {
+            const double xp[5] = {1.0, x, x2, x2 * x, x2 * x2};
+            const double gg[3] = {gx_ * gx_, gx_ * gy_, gy_ * gy_};
+#pragma unroll
+            for (int st = 0; st < 3; ++st)
+#pragma unroll
+                for (int p = 0; p < 5; ++p) t[st * 5 + p] += gg[st] * xp[p];
+        } else {
+            const double xp[3] = {1.0, x, x2};
+            const double gi[2] = {gx_ * I, gy_ * I}, g[2] = {gx_, gy_};
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    t[k * 3 + p] += gi[k] * xp[p];
+                    t[6 + k * 3 + p] += g[k] * xp[p];
+                }
+            t[12] += I;
+            t[13] += I * I;
+        }
+    };
+    // Five columns' loads are issued together before their products.
+    constexpr int B = 5;
+    int c = 0;
+    for (; c + B <= S; c += B) {
+        double I[B], gxv[B], gyv[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            I[b] = PART == 1 ? a.intensity[row + c + b] : 0.0;
+            gxv[b] = a.grad_x[row + c + b];
+            gyv[b] = a.grad_y[row + c + b];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) add(I[b], gxv[b], gyv[b], c + b);
+    }
+    for (; c < S; ++c) add(PART == 1 ? a.intensity[row + c] : 0.0, a.grad_x[row + c], a.grad_y[row + c], c);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[(Q0 + q) * pitch] = t[q];
+}
+
 __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows_per_block, int pitch) {
     constexpr int P = 12;
     extern __shared__ double T[];  // kStripQ x pitch row sums
@@ -465,48 +518,17 @@ __global__ __launch_bounds__(256) void k_subset_prep_strips(PrepArgs a, int rows
     const int ox = gx * D, oy0 = gy0 * D;
     const double half = 0.5 * (double)(S - 1);
 
-    // 1. Row-segment sums.
-    for (int r = threadIdx.x; r < R; r += blockDim.x) {
-        double t[kStripQ];
-#pragma unroll
-        for (int q = 0; q < kStripQ; ++q) t[q] = 0.0;
-        const size_t row = (size_t)(oy0 + r) * a.width + ox;
-        auto add = [&](double I, double gx_, double gy_, int c) {
-            const double x = (double)c - half, x2 = x * x;
-            const double xp[5] = {1.0, x, x2, x2 * x, x2 * x2};
-            const double gg[3] = {gx_ * gx_, gx_ * gy_, gy_ * gy_};
-            const double gi[2] = {gx_ * I, gy_ * I}, g[2] = {gx_, gy_};
-#pragma unroll
-            for (int st = 0; st < 3; ++st)
-#pragma unroll
-                for (int p = 0; p < 5; ++p) t[st * 5 + p] += gg[st] * xp[p];
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int p = 0; p < 3; ++p) {
-                    t[15 + k * 3 + p] += gi[k] * xp[p];
-                    t[21 + k * 3 + p] += g[k] * xp[p];
-                }
-            t[27] += I;
-            t[28] += I * I;
-        };
-        // Five columns' loads are issued together before their products.
-        constexpr int B = 5;
-        int c = 0;
-        for (; c + B <= S; c += B) {
-            double I[B], gxv[B], gyv[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                I[b] = a.intensity[row + c + b];
-                gxv[b] = a.grad_x[row + c + b];
-                gyv[b] = a.grad_y[row + c + b];
-            }
-#pragma unroll
-            for (int b = 0; b < B; ++b) add(I[b], gxv[b], gyv[b], c + b);
+    // 1. Row-segment sums, each image row by two threads: waves 0-1 form the 15
+    //    sums g_s g_t x^p, waves 2-3 the other 14 (the split is wave-uniform).
+    {
+        const int part = threadIdx.x >> 7;
+        for (int r = threadIdx.x & 127; r < R; r += 128) {
+            const size_t row = (size_t)(oy0 + r) * a.width + ox;
+            if (part == 0)
+                strip_row_sums<0>(a, row, S, half, T + r, pitch);
+            else
+                strip_row_sums<1>(a, row, S, half, T + r, pitch);
         }
-        for (; c < S; ++c) add(a.intensity[row + c], a.grad_x[row + c], a.grad_y[row + c], c);
-#pragma unroll
-        for (int q = 0; q < kStripQ; ++q) T[q * pitch + r] = t[q];
     }
     __syncthreads();
 
