@@ -107,13 +107,14 @@ __device__ __forceinline__ float tile_sample(const float* tile, const Tile& t, f
     else
         b = tile_tap_dyn(tile - (pitch * (t.y0 + 1) + t.x0 + 1), ix, iy, pitch);
     float v;
+    const float* r = b;  // row pointers by addition (no multiply when the pitch is a register)
 #pragma unroll
     for (int yy = 0; yy < 4; ++yy) {
-        const float* r = b + yy * pitch;
         float row = wx[0] * r[0];
 #pragma unroll
         for (int xx = 1; xx < 4; ++xx) row += wx[xx] * r[xx];
         v = yy == 0 ? wy[0] * row : v + wy[yy] * row;
+        r += pitch;
     }
     return v;
 }
