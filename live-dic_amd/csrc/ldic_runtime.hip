@@ -1993,7 +1993,9 @@ int batch_build(ldic_batch* b) {
 
 // Times every grid candidate of every group on the device (each solve is
 // idempotent for fixed seeds, so the frame's own buffers are reused) and keeps
-// the fastest.
+// the finest of the fastest (below).
+constexpr float kTuneSlack = 0.03f;
+
 int batch_tune(ldic_batch* b) {
     CHECK(batch_enqueue(b));
     HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -2001,7 +2003,6 @@ int batch_tune(ldic_batch* b) {
         for (ldic_batch::Group& g : b->steps[j]) {
             g.timings.clear();
             float best = 1e30f;
-            std::pair<int, int> best_shape(g.launch.waves, g.launch.grid);
             for (const auto& cand : g.candidates) {
                 batch_shape(g, cand.first, cand.second);
                 HIP_CHECK(launch_solve_batched(g.launch, b->stream));  // warm
@@ -2012,9 +2013,21 @@ int batch_tune(ldic_batch* b) {
                 float ms = 0.0f;
                 HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
                 g.timings.push_back(ms);
-                if (ms < best) {
-                    best = ms;
+                best = std::min(best, ms);
+            }
+            // Among the shapes within kTuneSlack of the fastest (timed alone), the most
+            // workgroups, then the fewest waves per workgroup: in the frame the groups of a
+            // level run side by side on the graph's branches, and a finer static partition
+            // of the subsets finishes more evenly when some CUs start late.
+            std::pair<int, int> best_shape(g.launch.waves, g.launch.grid);
+            bool found = false;
+            for (size_t i = 0; i < g.candidates.size(); ++i) {
+                const auto& cand = g.candidates[i];
+                if (g.timings[i] > best * (1.0f + kTuneSlack)) continue;
+                if (!found || cand.second > best_shape.second ||
+                    (cand.second == best_shape.second && cand.first < best_shape.first)) {
                     best_shape = cand;
+                    found = true;
                 }
             }
             batch_shape(g, best_shape.first, best_shape.second);
