@@ -64,7 +64,9 @@ __device__ __forceinline__ float wave_sum(float v) {
 // The same sum, with the cross-row step done by row broadcasts: rows 1 and 3
 // add rows 0 and 2, then rows 2 and 3 add lane 31, and lane 63 holds
 // (r3 + r2) + (r1 + r0), bit-identical to wave_sum because float addition is
-// commutative.  Seven VALU operations instead of eleven.
+// commutative.  It compiles to ten VALU operations and one lane read (each
+// broadcast is a zeroed DPP move and an add); wave_sums below forms the same
+// sums for several values at once.
 __device__ __forceinline__ float wave_sum_bcast(float v) {
     v += dpp_f32<kQuadXor1>(v);
     v += dpp_f32<kQuadXor2>(v);
